@@ -1,0 +1,1211 @@
+// pathtrace.hip -- gfx950 wavefront path tracer behind the lucid_hip.h C ABI.
+//
+// One render = a sequence of wavefront batches ("waves").  A wave holds
+// P pixels x B consecutive sample indices as independent paths with their own
+// PCG state, so the random stream of every path is the one the reference's
+// render_tile seeds for (x, y, sample_offset + s) (pybind_renderer.hpp:274-293).
+// Per bounce the host enqueues, on one stream and without synchronising:
+//
+//   k_intersect  closest hit of every live path (LDS traversal stack)
+//   k_shade      emission / native light / environment with MIS weights,
+//                NEE light sampling (shadow ray queued), BSDF sampling,
+//                Russian roulette; live paths are compacted into the next
+//                queue with a wavefront ballot + one atomic per wavefront
+//   k_shadow     occlusion of the queued shadow rays; unoccluded NEE
+//                contributions are added to the path radiance before the next
+//                bounce's emission, i.e. in the reference's summation order
+//
+// and after the last bounce k_film adds the B per-sample radiances of each
+// pixel to the tile sum in sample order (the reference's `radiance = radiance
+// + sample_radiance`).  k_output applies 2^EV, clamps at 0, writes alpha 1
+// and flips rows (pybind_renderer.hpp:331-360).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lucid_hip.h"
+#include "scene_pack.h"
+#include "shading.h"
+
+namespace lucid {
+
+constexpr int kBlock = 128;      // 2 wavefronts per workgroup
+constexpr int kStack = 48;       // LDS traversal stack entries per lane
+
+// ------------------------------------------------------------- device scene --
+struct SceneDev {
+    const GNode* nodes;
+    const float4* v0;
+    const float4* e1;
+    const float4* e2;
+    const float4* fn;
+    const float4* vn;       // 3 per triangle, or null
+    const GMat* mats;
+    const GLight* lights;
+    const float* cdf;
+    const int32_t* native_ids;
+    const GVolume* vols;
+    const float* grid;
+    int32_t n_lights, n_native, n_vols, root;
+    float total_area;
+    float env[3];
+    int32_t any_vbo;
+};
+
+struct WaveDev {
+    float4* ray_o;   // origin, tmin
+    float4* ray_d;   // direction
+    float4* thr;     // throughput, last BSDF pdf
+    float4* rad;     // path radiance
+    ulonglong2* rng; // PCG state, increment
+    int4* meta;      // pixel, sample, depth, flags
+    float4* hit;     // t, primitive (bits), u, v
+    float4* sh_o;    // shadow origin, distance
+    float4* sh_d;    // shadow direction
+    float4* sh_c;    // pending NEE contribution
+    int32_t* q[2];   // live-path queues
+    int32_t* qs;     // shadow queue
+    int32_t* cnt;    // [0],[1] queue counts, [2] shadow count, [3] stack overflows
+    unsigned long long* rays;  // [0] closest, [1] shadow, [2..3] closest nodes/tris, [4..5] shadow nodes/tris
+    int32_t count_work;        // accumulate node / triangle fetch counts
+};
+
+struct CamDev {
+    float pos[3], fwd[3], right[3], up[3];
+    float aspect, scale;
+    int32_t fw, fh, tx, ty, tw, th;
+    int32_t samples_in_call;  // render_tile's `samples` (jitter only when > 1)
+    int32_t s0;               // absolute sample index of wave slot b = 0
+    int32_t P, B, max_depth;
+};
+
+// ---------------------------------------------------------------- helpers --
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Appends `pred` lanes to a queue: one atomic per wavefront.  Every lane of
+// the wavefront must call it (convergent).
+__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    int total = __popcll(mask);
+    int base = 0;
+    int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+    if (total > 0 && lane_id() == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
+    return base + rank;
+}
+
+__device__ __forceinline__ v3f inv_dir(v3f d) {  // intersection.hpp:412-419
+    return mk(fabsf(d.x) > 1e-8f ? 1.0f / d.x : 0.0f, fabsf(d.y) > 1e-8f ? 1.0f / d.y : 0.0f,
+              fabsf(d.z) > 1e-8f ? 1.0f / d.z : 0.0f);
+}
+// intersectAABB slab test (intersection.hpp:257-302), with the entry
+// distance returned for ordering and pruning.
+__device__ __forceinline__ bool slab1(float dc, float ic, float dmin, float dmax, float& tmin, float& tmax) {
+    if (fabsf(dc) < 1e-8f) return !((dmin > 0.0f && dmax > 0.0f) || (dmin < 0.0f && dmax < 0.0f));
+    float t1 = dmin * ic, t2 = dmax * ic;
+    float amin = smin(t1, t2), amax = smax(t1, t2);
+    tmin = smax(tmin, amin);
+    tmax = smin(tmax, amax);
+    return tmax >= tmin;
+}
+__device__ __forceinline__ bool aabb(v3f o, v3f d, v3f id, const float* b, float& tentry) {
+    float tmin = 0.0f, tmax = RAY_T_MAX;
+    if (!slab1(d.x, id.x, b[0] - o.x, b[3] - o.x, tmin, tmax)) return false;
+    if (!slab1(d.y, id.y, b[1] - o.y, b[4] - o.y, tmin, tmax)) return false;
+    if (!slab1(d.z, id.z, b[2] - o.z, b[5] - o.z, tmin, tmax)) return false;
+    tentry = tmin;
+    return tmax >= smax(tmin, 0.0f);
+}
+// Moller-Trumbore with precomputed edges (intersection.hpp:335-396).
+__device__ __forceinline__ bool tri_test(v3f o, v3f d, float4 A, float4 E1, float4 E2, float tmin, float tmax,
+                                         float& t, float& u, float& v) {
+    v3f e1 = mk(E1), e2 = mk(E2);
+    v3f pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    const float AE = 1e-6f * 1e-6f;
+    if (det > -AE && det < AE) return false;
+    v3f tvec = o - mk(A);
+    u = dot(tvec, pvec) / det;
+    if (u < 0.0f || u > 1.0f) return false;
+    v3f qvec = cross(tvec, e1);
+    v = dot(d, qvec) / det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qvec) / det;
+    return t >= tmin && t <= tmax;
+}
+
+// Pruning margin: a child box whose entry distance exceeds the current
+// closest t by more than 1 % cannot hold a triangle whose Moller-Trumbore t
+// ties or beats it; the reference never prunes, and the tie rule below
+// reproduces its "last tested wins" order.
+__device__ __forceinline__ float prune_bound(float best) { return best + smax(best * 0.01f, 1e-6f); }
+
+struct TraceHit {
+    float t, u, v;
+    int32_t prim;
+};
+
+// Closest hit over the whole scene.  Ties in t resolve to the larger
+// BVH-order position = the triangle the reference tests last.
+__device__ __noinline__ TraceHit trace_closest(const SceneDev& S, v3f o, v3f d, float tmin, float tmax,
+                                               int32_t* stk, int32_t* ovf, uint32_t& nn, uint32_t& nt) {
+    TraceHit h;
+    h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+    v3f id = inv_dir(d);
+    int sp = 0;
+    int32_t node = S.root;
+    for (;;) {
+        const GNode* n = S.nodes + node;
+        ++nn;
+        float4 b0 = ((const float4*)n->b)[0];
+        float4 b1 = ((const float4*)n->b)[1];
+        float4 b2 = ((const float4*)n->b)[2];
+        int2 ch = *(const int2*)n->child;
+        float bx0[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
+        float bx1[6] = {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        float te0 = 0.0f, te1 = 0.0f;
+        float lim = prune_bound(h.t);
+        bool h0 = ch.x != kEmptyChild && aabb(o, d, id, bx0, te0) && te0 <= lim;
+        bool h1 = ch.y != kEmptyChild && aabb(o, d, id, bx1, te1) && te1 <= lim;
+        int32_t c[2] = {ch.x, ch.y};
+        bool hh[2] = {h0, h1};
+        int first = (h1 && (!h0 || te1 < te0)) ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            int j = k == 0 ? first : 1 - first;
+            if (hh[j] && c[j] < 0) {
+                int32_t enc = ~c[j];
+                int32_t start = enc >> 3, count = enc & 7;
+                nt += count;
+                for (int i = 0; i < count; ++i) {
+                    int32_t prim = start + i;
+                    float t, u, v;
+                    if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
+                        if (t < h.t || prim > h.prim) {
+                            h.t = t; h.u = u; h.v = v; h.prim = prim;
+                        }
+                    }
+                }
+                hh[j] = false;
+            }
+        }
+        if (hh[0] && hh[1]) {
+            int32_t nearc = c[first], farc = c[1 - first];
+            if (sp < kStack) {
+                stk[sp * kBlock] = farc;
+                ++sp;
+            } else {
+                atomicAdd(ovf, 1);
+            }
+            node = nearc;
+            continue;
+        }
+        if (hh[0]) { node = c[0]; continue; }
+        if (hh[1]) { node = c[1]; continue; }
+        if (sp == 0) break;
+        --sp;
+        node = stk[sp * kBlock];
+    }
+    return h;
+}
+
+// Occlusion: any triangle with tmin <= t <= tmax (the reference's closest-hit
+// shadow query reports a hit exactly when one exists).
+__device__ __noinline__ bool trace_any(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk,
+                                       int32_t* ovf, uint32_t& nn, uint32_t& nt) {
+    v3f id = inv_dir(d);
+    int sp = 0;
+    int32_t node = S.root;
+    float lim = prune_bound(tmax);
+    for (;;) {
+        const GNode* n = S.nodes + node;
+        ++nn;
+        float4 b0 = ((const float4*)n->b)[0];
+        float4 b1 = ((const float4*)n->b)[1];
+        float4 b2 = ((const float4*)n->b)[2];
+        int2 ch = *(const int2*)n->child;
+        float bx0[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
+        float bx1[6] = {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        float te0 = 0.0f, te1 = 0.0f;
+        bool h0 = ch.x != kEmptyChild && aabb(o, d, id, bx0, te0) && te0 <= lim;
+        bool h1 = ch.y != kEmptyChild && aabb(o, d, id, bx1, te1) && te1 <= lim;
+        int32_t c[2] = {ch.x, ch.y};
+        bool hh[2] = {h0, h1};
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (hh[j] && c[j] < 0) {
+                int32_t enc = ~c[j];
+                int32_t start = enc >> 3, count = enc & 7;
+                for (int i = 0; i < count; ++i) {
+                    int32_t prim = start + i;
+                    float t, u, v;
+                    ++nt;
+                    if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, tmax, t, u, v)) return true;
+                }
+                hh[j] = false;
+            }
+        }
+        if (hh[0] && hh[1]) {
+            int first = te1 < te0 ? 1 : 0;
+            if (sp < kStack) {
+                stk[sp * kBlock] = c[1 - first];
+                ++sp;
+            } else {
+                atomicAdd(ovf, 1);
+            }
+            node = c[first];
+            continue;
+        }
+        if (hh[0]) { node = c[0]; continue; }
+        if (hh[1]) { node = c[1]; continue; }
+        if (sp == 0) break;
+        --sp;
+        node = stk[sp * kBlock];
+    }
+    return false;
+}
+
+__device__ __forceinline__ int32_t prim_mesh(const SceneDev& S, int32_t prim) { return __float_as_int(S.v0[prim].w); }
+
+// intersect_skipping_volume_boundaries (path_tracer.hpp:82-95): the closest
+// real surface; boundary-only meshes are skipped by re-querying past them.
+__device__ __forceinline__ TraceHit closest_real(const SceneDev& S, v3f o, v3f d, float tmin, int32_t* stk,
+                                                 int32_t* ovf, uint32_t& nn, uint32_t& nt) {
+    TraceHit h = trace_closest(S, o, d, tmin, RAY_T_MAX, stk, ovf, nn, nt);
+    if (!S.any_vbo) return h;
+    float smin_ = tmin;
+    for (int peek = 0; peek < 8; ++peek) {
+        if (h.prim < 0) return h;
+        if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return h;
+        smin_ = h.t + 1e-3f;
+        h = trace_closest(S, o, d, smin_, RAY_T_MAX, stk, ovf, nn, nt);
+    }
+    h.prim = -1;
+    return h;
+}
+// is_shadowed_skipping_volume_boundaries (path_tracer.hpp:103-119)
+__device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float tmax, int32_t* stk, int32_t* ovf,
+                                         uint32_t& nn, uint32_t& nt) {
+    if (!S.any_vbo) return trace_any(S, o, d, RAY_T_MIN, tmax, stk, ovf, nn, nt);
+    float toff = 0.0f;
+    v3f po = o;
+    for (int peek = 0; peek < 8; ++peek) {
+        float remaining = tmax - toff;
+        if (remaining <= 0.0f) return false;
+        TraceHit h = trace_closest(S, po, d, RAY_T_MIN, remaining, stk, ovf, nn, nt);
+        if (h.prim < 0) return false;
+        if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return true;
+        float skip = h.t + 1e-3f;
+        toff += skip;
+        po = po + d * skip;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- raygen --
+// Camera ray of render_tile (pybind_renderer.hpp:253-293).
+__device__ __forceinline__ v3f cam_dir(const CamDev& c, float nx, float ny) {
+    v3f d = mk(c.fwd[0] + c.right[0] * (nx * c.scale) + c.up[0] * (ny * c.scale),
+               c.fwd[1] + c.right[1] * (nx * c.scale) + c.up[1] * (ny * c.scale),
+               c.fwd[2] + c.right[2] * (nx * c.scale) + c.up[2] * (ny * c.scale));
+    return make_dir_or(d, mk(c.fwd[0], c.fwd[1], c.fwd[2]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; }
+    if (p >= n_paths) return;
+    int b = p / c.P, pix = p - b * c.P;
+    int px = pix % c.tw, py = pix / c.tw;
+    int gx = c.tx + px, gy = c.ty + py;
+    int s = c.s0 + b;
+    float ndc_x = (2.0f * (gx + 0.5f) / c.fw - 1.0f) * c.aspect;
+    float ndc_y = 1.0f - 2.0f * (gy + 0.5f) / c.fh;
+    Pcg rng = pcg_seed((uint32_t)gx, (uint32_t)gy, (uint32_t)s);
+    v3f d;
+    if (c.samples_in_call > 1) {
+        float jx = (randf(rng) - 0.5f) / c.fw;
+        float jy = (randf(rng) - 0.5f) / c.fh;
+        d = cam_dir(c, ndc_x + jx, ndc_y + jy);
+    } else {
+        d = cam_dir(c, ndc_x, ndc_y);
+    }
+    W.ray_o[p] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
+    W.ray_d[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    W.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    W.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    W.rng[p] = make_ulonglong2(rng.state, rng.inc);
+    W.meta[p] = make_int4(pix, s, 0, 0);
+    W.q[0][p] = p;
+}
+
+// ------------------------------------------------------------- intersect --
+__global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int qin) {
+    __shared__ int32_t stack[kStack * kBlock];
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    int n = W.cnt[qin];
+    if (i == 0) {
+        W.cnt[1 - qin] = 0;
+        W.cnt[2] = 0;
+        atomicAdd(&W.rays[0], (unsigned long long)n);
+    }
+    if (i >= n) return;
+    int p = W.q[qin][i];
+    float4 o = W.ray_o[p], d = W.ray_d[p];
+    uint32_t nn = 0, nt = 0;
+    TraceHit h = closest_real(S, mk(o), mk(d), o.w, stack + threadIdx.x, &W.cnt[3], nn, nt);
+    W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+    if (W.count_work) {
+        atomicAdd(&W.rays[2], (unsigned long long)nn);
+        atomicAdd(&W.rays[3], (unsigned long long)nt);
+    }
+}
+
+// ----------------------------------------------------------------- shade --
+struct NativeHit {
+    bool hit;
+    float t;
+    v3f point, normal, emission;
+    int32_t index;  // native light index (JSON order)
+};
+// intersectNativeLights (scene_lights.hpp:176-246); only area and sphere
+// lights are visible, and only for depth > 0.
+__device__ __noinline__ NativeHit native_lights(const SceneDev& S, v3f o, v3f d) {
+    NativeHit r;
+    r.hit = false; r.t = 1e30f; r.index = -1;
+    for (int i = 0; i < S.n_native; ++i) {
+        const GLight& l = S.lights[S.native_ids[i]];
+        v3f c = mk(l.pos[0], l.pos[1], l.pos[2]);
+        if (l.type == 1) {
+            if (!(l.radius > 1e-6f)) continue;
+            // intersectSphere (intersection.hpp:58-104)
+            v3f oc = o - c;
+            float bq = 2.0f * dot(oc, d);
+            float cq = len2(oc) - l.radius * l.radius;
+            float disc = bq * bq - 4.0f * 1.0f * cq;
+            if (disc < 0.0f) continue;
+            float sq = sqrtf(smax(0.0f, disc));
+            float t = (-bq - sq) / (2.0f * 1.0f);
+            if (t < RAY_T_MIN) {
+                t = (-bq + sq) / (2.0f * 1.0f);
+                if (t < RAY_T_MIN) continue;
+            }
+            if (t < r.t) {
+                r.hit = true; r.t = t; r.point = o + d * t;
+                r.normal = normalize_plain(r.point - c);
+                r.emission = mk(l.emission[0], l.emission[1], l.emission[2]);
+                r.index = i;
+            }
+        } else if (l.type == 4) {
+            // intersectRectangle / intersectEllipse (intersection.hpp:128-236)
+            v3f nrm = mk(l.normal[0], l.normal[1], l.normal[2]);
+            float den = dot(d, nrm);
+            if (fabsf(den) < 1e-6f) continue;
+            float t = dot(c - o, nrm) / den;
+            if (t < RAY_T_MIN) continue;
+            v3f hp = o + d * t;
+            v3f lh = hp - c;
+            float x = dot(lh, mk(l.right[0], l.right[1], l.right[2]));
+            float y = dot(lh, mk(l.up[0], l.up[1], l.up[2]));
+            if (l.shape == 2 || l.shape == 3) {
+                float nx = x / (l.sx * 0.5f), ny = y / (l.sy * 0.5f);
+                if (nx * nx + ny * ny > 1.0f) continue;
+            } else {
+                float hx = l.sx * 0.5f, hy = l.sy * 0.5f;
+                if (x * x > hx * hx || y * y > hy * hy) continue;
+            }
+            if (t < r.t && dot(nrm, d) < 0.0f) {
+                r.hit = true; r.t = t; r.point = hp; r.normal = nrm;
+                r.emission = mk(l.emission[0], l.emission[1], l.emission[2]);
+                r.index = i;
+            }
+        }
+    }
+    return r;
+}
+
+// selectLight (scene_lights.hpp:130-144): std::lower_bound on the CDF.
+__device__ __forceinline__ int select_light(const SceneDev& S, float u, float& prob) {
+    int lo = 0, hi = S.n_lights;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (S.cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    int idx = min(lo, S.n_lights - 1);
+    prob = area_ratio(S.lights[idx].area, S.total_area);
+    return idx;
+}
+
+__device__ __forceinline__ bool finite3(v3f a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+// One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570).
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, int qin, int32_t max_depth) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    int n = W.cnt[qin];
+    bool live = i < n;
+    bool cont = false, nee = false;
+    int p = live ? W.q[qin][i] : 0;
+    if (live) {
+        float4 ro = W.ray_o[p], rdv = W.ray_d[p], tv = W.thr[p];
+        int4 meta = W.meta[p];
+        float4 hv = W.hit[p];
+        v3f o = mk(ro), d = mk(rdv), thr = mk(tv);
+        float last_pdf = tv.w;
+        int depth = meta.z;
+        int32_t prim = __float_as_int(hv.y);
+        ulonglong2 rs = W.rng[p];
+        Pcg rng;
+        rng.state = rs.x; rng.inc = rs.y;
+        float4 Lv = W.rad[p];
+        v3f L = mk(Lv);
+        bool has_hit = prim >= 0;
+        NativeHit lh;
+        lh.hit = false;
+        if (depth > 0 && S.n_native > 0) lh = native_lights(S, o, d);
+        bool done = false;
+        if (lh.hit && (!has_hit || lh.t < hv.x)) {
+            v3f c = cmul(lh.emission, thr);
+            if (ALGO == LUCID_ALGO_MIS) {
+                float lp = 0.0f;
+                if (S.n_lights > 0 && lh.index >= 0) {
+                    int sli = S.native_ids[lh.index];
+                    float sp = area_ratio(S.lights[sli].area, S.total_area);
+                    if (sp > 0.0f) lp = pdf_light(S.lights[sli].area, o, lh.point, lh.normal) * sp;
+                }
+                float mw = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
+                c = c * mw;
+            }
+            L = L + c;
+            done = true;
+        } else if (!has_hit) {
+            L = L + cmul(mk(S.env[0], S.env[1], S.env[2]), thr);
+            done = true;
+        }
+        if (!done) {
+            float t = hv.x, u = hv.z, v = hv.w;
+            float4 v0 = S.v0[prim];
+            int32_t mesh = __float_as_int(v0.w);
+            GMat gm = S.mats[mesh];
+            Mat mat;
+            mat.albedo = mk(gm.albedo[0], gm.albedo[1], gm.albedo[2]);
+            mat.metallic = gm.metallic; mat.roughness = gm.roughness;
+            mat.transmission = gm.transmission; mat.ior = gm.ior;
+            v3f em = mk(gm.emission[0], gm.emission[1], gm.emission[2]);
+            v3f point = o + d * t;
+            float4 fnv = S.fn[prim];
+            v3f nrm = mk(fnv);
+            if (__float_as_int(fnv.w)) {
+                float w = 1.0f - u - v;
+                const float4* vn = S.vn + 3 * (size_t)prim;
+                v3f ni = mk(vn[0]) * w + mk(vn[1]) * u + mk(vn[2]) * v;
+                nrm = make_dir_or(ni, nrm);
+            }
+            v3f wo = -d;
+            bool front = dot(wo, nrm) > 0.0f;
+            v3f sn = (mat.transmission < 0.5f && !front) ? -nrm : nrm;
+            bool has_em = (gm.flags & MF_EMISSIVE) != 0;
+            if (ALGO == LUCID_ALGO_SIMPLE) {
+                L = L + cmul(em, thr);
+            } else if (ALGO == LUCID_ALGO_NEE) {
+                if (has_em && depth == 0) L = L + cmul(em, thr);
+            } else if (has_em) {
+                float mw = 1.0f;
+                if (S.n_lights > 0 && last_pdf > MIN_PDF) {
+                    int32_t li = __float_as_int(S.e2[prim].w);
+                    if (li >= 0) {
+                        float sp = area_ratio(S.lights[li].area, S.total_area);
+                        if (sp > 0.0f) {
+                            float lp = pdf_light(S.lights[li].area, o, point, nrm) * sp;
+                            if (lp > MIN_PDF) mw = power_heuristic(last_pdf, lp);
+                        }
+                    }
+                }
+                L = L + cmul(em, thr) * mw;
+            }
+            // next event estimation
+            if (ALGO != LUCID_ALGO_SIMPLE && S.n_lights > 0 && !has_em && !(mat.transmission > 0.5f)) {
+                float sp;
+                int li = select_light(S, randf(rng), sp);
+                if (sp > 1e-6f) {
+                    float a1 = randf(rng), a2 = randf(rng);
+                    LSample ls = sample_light(S.lights[li], point, a1, a2);
+                    if (ls.pdf > MIN_PDF) {
+                        float nl = dot(sn, ls.dir);
+                        if (nl > 1e-6f) {
+                            v3f f = eval_bsdf(mat, wo, ls.dir, sn);
+                            float pl = ls.pdf * sp;
+                            v3f cc;
+                            if (ALGO == LUCID_ALGO_MIS) {
+                                float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                                float mw = power_heuristic(pl, pb);
+                                cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
+                            } else {
+                                cc = cmul(bsdf_weight(f, nl, pl), ls.emission);
+                            }
+                            cc = cmul(cc, thr);
+                            W.sh_o[p] = make_float4(point.x, point.y, point.z, ls.dist);
+                            W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                            W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                            nee = true;
+                        }
+                    }
+                }
+            }
+            // BSDF sampling for the next bounce (at the last bounce the reference
+            // still samples, but nothing it draws reaches the result)
+            if (depth + 1 < max_depth) {
+            v3f smp_n = (mat.transmission > 0.5f) ? nrm : sn;
+            float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
+            BSample bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+            bool ok = bs.kind == 1 || bs.pdf > MIN_PDF;
+            v3f w = bs.fw;
+            if (ok && bs.kind == 0) {
+                float cs = fabsf(dot(smp_n, bs.wi));
+                if (cs <= 1e-6f || bs.pdf < MIN_PDF) ok = false;
+                else w = bsdf_weight(bs.fw, cs, bs.pdf);
+            }
+            if (ok) {
+                thr = cmul(thr, w);
+                if (ALGO == LUCID_ALGO_MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
+                if (depth >= 3) {
+                    float mx = max3(thr.x, thr.y, thr.z);
+                    float rr = smin(mx, 0.95f);
+                    if (randf(rng) > rr) ok = false;
+                    else thr = thr * (1.0f / rr);
+                }
+            }
+            if (ok && finite3(thr)) {
+                cont = true;
+                W.ray_o[p] = make_float4(point.x, point.y, point.z, RAY_T_MIN);
+                W.ray_d[p] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, 0.0f);
+                W.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+                W.rng[p] = make_ulonglong2(rng.state, rng.inc);
+                W.meta[p] = make_int4(meta.x, meta.y, depth + 1, 0);
+            }
+            }
+        }
+        W.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    int slot = wave_append(&W.cnt[1 - qin], cont);
+    if (cont) W.q[1 - qin][slot] = p;
+    int sslot = wave_append(&W.cnt[2], nee);
+    if (nee) W.qs[sslot] = p;
+}
+
+// ---------------------------------------------------------------- shadow --
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W) {
+    __shared__ int32_t stack[kStack * kBlock];
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    int n = W.cnt[2];
+    if (i == 0) atomicAdd(&W.rays[1], (unsigned long long)n);
+    if (i >= n) return;
+    int p = W.qs[i];
+    float4 o = W.sh_o[p], d = W.sh_d[p];
+    uint32_t nn = 0, nt = 0;
+    if (!shadowed(S, mk(o), mk(d), o.w, stack + threadIdx.x, &W.cnt[3], nn, nt)) {
+        float4 L = W.rad[p], c = W.sh_c[p];
+        W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+    }
+    if (W.count_work) {
+        atomicAdd(&W.rays[4], (unsigned long long)nn);
+        atomicAdd(&W.rays[5], (unsigned long long)nt);
+    }
+}
+
+// ------------------------------------------------------------------ film --
+__global__ __launch_bounds__(256) void k_film(WaveDev W, float4* film, int32_t P, int32_t B) {
+    int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= P) return;
+    float4 a = film[pix];
+    for (int b = 0; b < B; ++b) {
+        float4 r = W.rad[(size_t)b * P + pix];
+        a.x = a.x + r.x; a.y = a.y + r.y; a.z = a.z + r.z;
+    }
+    film[pix] = a;
+}
+__global__ __launch_bounds__(256) void k_film_clear(float4* film, int32_t P) {
+    int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix < P) film[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+// per-sample radiance out: dst[(pix*S + s)*3 + c]
+__global__ __launch_bounds__(256) void k_samples_out(WaveDev W, float* dst, int32_t P, int32_t B, int32_t s_base,
+                                                     int32_t S) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P * B) return;
+    int b = p / P, pix = p - b * P;
+    float4 r = W.rad[p];
+    size_t k = ((size_t)pix * S + s_base + b) * 3;
+    dst[k] = r.x; dst[k + 1] = r.y; dst[k + 2] = r.z;
+}
+// apply_camera_sensitivity + clamp + alpha + Y flip (pybind_renderer.hpp:331-360)
+__global__ __launch_bounds__(256) void k_output(const float4* film, float* out, int32_t tw, int32_t th, float sens,
+                                                int32_t flip) {
+    int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= tw * th) return;
+    int px = pix % tw, py = pix / tw;
+    float4 a = film[pix];
+    float r = smax(a.x * sens, 0.0f), g = smax(a.y * sens, 0.0f), b = smax(a.z * sens, 0.0f);
+    int row = flip ? (th - 1 - py) : py;
+    size_t k = ((size_t)row * tw + px) * 4;
+    *(float4*)(out + k) = make_float4(r, g, b, 1.0f);
+}
+// primary-hit export: mesh, original triangle, u, v, t
+__global__ __launch_bounds__(256) void k_hits_out(SceneDev S, WaveDev W, int32_t* mesh, int32_t* tri, float* u,
+                                                  float* v, float* t, int32_t P, int32_t B, int32_t S_total,
+                                                  int32_t s_base) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P * B) return;
+    int b = p / P, pix = p - b * P;
+    size_t k = (size_t)pix * S_total + s_base + b;
+    float4 h = W.hit[p];
+    int32_t prim = __float_as_int(h.y);
+    if (prim >= 0) {
+        mesh[k] = __float_as_int(S.v0[prim].w);
+        tri[k] = __float_as_int(S.e1[prim].w);
+        u[k] = h.z; v[k] = h.w; t[k] = h.x;
+    } else {
+        mesh[k] = -1; tri[k] = -1; u[k] = 0.0f; v[k] = 0.0f; t[k] = 0.0f;
+    }
+}
+
+}  // namespace lucid
+
+// ============================================================ host side ==
+using namespace lucid;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct lucid_scene {
+    std::atomic<int> refs{1};
+    int device = 0;
+    PackedScene host;
+    std::vector<void*> allocs;
+    SceneDev dev;
+};
+
+struct lucid_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t wave_ev[2] = {nullptr, nullptr};  // pacing: at most two waves in flight
+    std::string err;
+    std::atomic<int> cancel{0};
+    std::mutex mu;
+    int64_t wave_paths = 0;
+    // wave buffers
+    int64_t cap_paths = 0;
+    std::vector<void*> wave_allocs;
+    WaveDev W{};
+    float4* film = nullptr;
+    int64_t film_cap = 0;
+    float* out_dev = nullptr;
+    int64_t out_cap = 0;
+    lucid_stats stats{};
+    int32_t profiling = 0;
+    // LUCID_PROFILE_KERNELS: one event pair per launch, kind 0/1/2
+    std::vector<hipEvent_t> kev;
+    std::vector<int> kev_kind;
+    size_t kev_used = 0;
+};
+
+namespace {
+
+int fail(lucid_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+#define HIP_OK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, LUCID_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int upload(lucid_ctx* ctx, lucid_scene* s, const std::vector<T>& v, const T** out) {
+    *out = nullptr;
+    if (v.empty()) return LUCID_OK;
+    void* p = nullptr;
+    if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (scene)");
+    s->allocs.push_back(p);
+    HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T*>(p);
+    return LUCID_OK;
+}
+
+void free_wave(lucid_ctx* ctx) {
+    for (void* p : ctx->wave_allocs) (void)hipFree(p);
+    ctx->wave_allocs.clear();
+    ctx->cap_paths = 0;
+    ctx->W = WaveDev{};
+}
+
+int ensure_wave(lucid_ctx* ctx, int64_t n) {
+    if (n <= ctx->cap_paths) return LUCID_OK;
+    free_wave(ctx);
+    auto alloc = [&](size_t bytes, void** out) -> bool {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return false;
+        ctx->wave_allocs.push_back(p);
+        *out = p;
+        return true;
+    };
+    WaveDev& W = ctx->W;
+    bool ok = alloc(n * 16, (void**)&W.ray_o) && alloc(n * 16, (void**)&W.ray_d) && alloc(n * 16, (void**)&W.thr) &&
+              alloc(n * 16, (void**)&W.rad) && alloc(n * 16, (void**)&W.rng) && alloc(n * 16, (void**)&W.meta) &&
+              alloc(n * 16, (void**)&W.hit) && alloc(n * 16, (void**)&W.sh_o) && alloc(n * 16, (void**)&W.sh_d) &&
+              alloc(n * 16, (void**)&W.sh_c) && alloc(n * 4, (void**)&W.q[0]) && alloc(n * 4, (void**)&W.q[1]) &&
+              alloc(n * 4, (void**)&W.qs) && alloc(64, (void**)&W.cnt) && alloc(64, (void**)&W.rays);
+    if (!ok) {
+        free_wave(ctx);
+        return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (wave buffers)");
+    }
+    ctx->cap_paths = n;
+    return LUCID_OK;
+}
+
+template <class T>
+int ensure_buf(lucid_ctx* ctx, T** p, int64_t* cap, int64_t n) {
+    if (n <= *cap) return LUCID_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed");
+    *cap = n;
+    return LUCID_OK;
+}
+
+__global__ void k_zero_counters(int32_t* cnt, unsigned long long* rays) {
+    if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 8) rays[threadIdx.x] = 0ull;
+}
+
+inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// Records a start event of a timed launch of `kind`; returns its slot or -1.
+int kev_begin(lucid_ctx* ctx, int kind, hipStream_t st) {
+    if (!(ctx->profiling & LUCID_PROFILE_KERNELS)) return -1;
+    size_t i = ctx->kev_used;
+    while (ctx->kev.size() < 2 * (i + 1)) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->kev.push_back(e);
+    }
+    if (ctx->kev_kind.size() < i + 1) ctx->kev_kind.resize(i + 1);
+    ctx->kev_kind[i] = kind;
+    ctx->kev_used = i + 1;
+    (void)hipEventRecord(ctx->kev[2 * i], st);
+    return (int)i;
+}
+void kev_end(lucid_ctx* ctx, int slot, hipStream_t st) {
+    if (slot >= 0) (void)hipEventRecord(ctx->kev[2 * (size_t)slot + 1], st);
+}
+
+int validate(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    if (!scene || !cam || !pr) return fail(ctx, LUCID_ERR_INVALID, "null scene, camera or params");
+    if (pr->tile_w <= 0 || pr->tile_h <= 0 || pr->full_w <= 0 || pr->full_h <= 0)
+        return fail(ctx, LUCID_ERR_INVALID, "tile and image sizes must be positive");
+    if ((int64_t)pr->tile_w * pr->tile_h > (1LL << 28)) return fail(ctx, LUCID_ERR_INVALID, "tile too large");
+    if (pr->samples < 0 || pr->max_depth < 0) return fail(ctx, LUCID_ERR_INVALID, "negative samples or depth");
+    if (pr->algorithm < 0 || pr->algorithm > 4) return fail(ctx, LUCID_ERR_INVALID, "unknown algorithm");
+    if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
+    return LUCID_OK;
+}
+
+// Runs the waves.  mode 0: film (render_tile), 1: per-sample radiance into
+// samples_dev, 2: primary hits only.
+struct RunOut {
+    float* samples_dev = nullptr;
+    int32_t *mesh = nullptr, *tri = nullptr;
+    float *u = nullptr, *v = nullptr, *t = nullptr;
+};
+
+int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr, int mode,
+              hipStream_t st, RunOut* ro, bool* cancelled) {
+    *cancelled = false;
+    const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
+    const int S = pr->samples;
+    int64_t target = ctx->wave_paths > 0 ? ctx->wave_paths : (int64_t)1 << 21;
+    int64_t B = std::max<int64_t>(1, std::min<int64_t>(S, target / P));
+    int rc = ensure_wave(ctx, P * B);
+    if (rc) return rc;
+    rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
+    if (rc) return rc;
+    const SceneDev& SD = scene->dev;
+    WaveDev W = ctx->W;
+    W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
+    ctx->kev_used = 0;
+    k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
+    k_film_clear<<<blocks(P, 256), 256, 0, st>>>(ctx->film, (int32_t)P);
+    CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
+    CamDev cd;
+    std::memcpy(cd.pos, cb.pos, 12); std::memcpy(cd.fwd, cb.fwd, 12);
+    std::memcpy(cd.right, cb.right, 12); std::memcpy(cd.up, cb.up, 12);
+    cd.aspect = cb.aspect; cd.scale = cb.scale;
+    cd.fw = pr->full_w; cd.fh = pr->full_h; cd.tx = pr->tile_x; cd.ty = pr->tile_y;
+    cd.tw = pr->tile_w; cd.th = pr->tile_h;
+    cd.samples_in_call = S;
+    cd.P = (int32_t)P;
+    cd.max_depth = pr->max_depth;
+    uint64_t waves = 0;
+    const int max_depth = mode == 2 ? 1 : pr->max_depth;
+    // traceVolumeSimple / traceVolumeMIS reduce exactly to traceSimple /
+    // traceMIS on a scene without participating media (no distance-sampling
+    // draws, unit transmittance).
+    int algo = pr->algorithm;
+    if (algo == LUCID_ALGO_VOLUME_SIMPLE || algo == LUCID_ALGO_VOLUME_MIS) {
+        if (scene->host.any_volume)
+            return fail(ctx, LUCID_ERR_INVALID, "volume integrators are not available in this build");
+        algo = algo == LUCID_ALGO_VOLUME_SIMPLE ? LUCID_ALGO_SIMPLE : LUCID_ALGO_MIS;
+    }
+    for (int s0 = 0; s0 < S; s0 += (int)B) {
+        // Pacing: wait for the wave before the previous one, so at most two
+        // waves are queued and a cancel takes effect within one wave.
+        if (waves >= 2) (void)hipEventSynchronize(ctx->wave_ev[waves & 1]);
+        if (ctx->cancel.load(std::memory_order_relaxed)) {
+            *cancelled = true;
+            break;
+        }
+        int b = (int)std::min<int64_t>(B, S - s0);
+        int64_t n = P * b;
+        cd.s0 = pr->sample_offset + s0;
+        cd.B = b;
+        k_raygen<<<blocks(n, kBlock), kBlock, 0, st>>>(W, cd, (int32_t)n);
+        int q = 0;
+        for (int depth = 0; depth < max_depth; ++depth) {
+            int ks = kev_begin(ctx, 0, st);
+            k_intersect<<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q);
+            kev_end(ctx, ks, st);
+            if (mode == 2) break;
+            ks = kev_begin(ctx, 1, st);
+            switch (algo) {
+                case LUCID_ALGO_SIMPLE:
+                    k_shade<LUCID_ALGO_SIMPLE><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                case LUCID_ALGO_MIS:
+                    k_shade<LUCID_ALGO_MIS><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                default:
+                    k_shade<LUCID_ALGO_NEE><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+            }
+            kev_end(ctx, ks, st);
+            ks = kev_begin(ctx, 2, st);
+            k_shadow<<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W);
+            kev_end(ctx, ks, st);
+            q = 1 - q;
+        }
+        if (mode == 0) {
+            k_film<<<blocks(P, 256), 256, 0, st>>>(W, ctx->film, (int32_t)P, b);
+        } else if (mode == 1) {
+            k_samples_out<<<blocks(n, 256), 256, 0, st>>>(W, ro->samples_dev, (int32_t)P, b, s0, S);
+        } else {
+            k_hits_out<<<blocks(n, 256), 256, 0, st>>>(SD, W, ro->mesh, ro->tri, ro->u, ro->v, ro->t, (int32_t)P, b,
+                                                       S, s0);
+        }
+        (void)hipEventRecord(ctx->wave_ev[waves & 1], st);
+        ++waves;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, LUCID_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    ctx->stats.paths = (uint64_t)P * S;
+    ctx->stats.waves = waves;
+    return LUCID_OK;
+}
+
+int finish_stats(lucid_ctx* ctx) {
+    unsigned long long rays[6] = {0, 0, 0, 0, 0, 0};
+    int32_t cnt[4] = {0, 0, 0, 0};
+    HIP_OK(hipMemcpyAsync(rays, ctx->W.rays, sizeof(rays), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(cnt, ctx->W.cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    ctx->stats.rays_closest = rays[0];
+    ctx->stats.rays_shadow = rays[1];
+    ctx->stats.stack_overflows = (uint64_t)cnt[3];
+    ctx->stats.nodes_closest = rays[2];
+    ctx->stats.tris_closest = rays[3];
+    ctx->stats.nodes_shadow = rays[4];
+    ctx->stats.tris_shadow = rays[5];
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->stats.ms_render = ms;
+    double kms[3] = {0, 0, 0};
+    uint64_t kn[3] = {0, 0, 0};
+    for (size_t i = 0; i < ctx->kev_used; ++i) {
+        float e = 0.0f;
+        if (hipEventElapsedTime(&e, ctx->kev[2 * i], ctx->kev[2 * i + 1]) == hipSuccess) {
+            kms[ctx->kev_kind[i]] += e;
+            kn[ctx->kev_kind[i]] += 1;
+        }
+    }
+    ctx->stats.ms_intersect = kms[0]; ctx->stats.ms_shade = kms[1]; ctx->stats.ms_shadow = kms[2];
+    ctx->stats.launches_intersect = kn[0]; ctx->stats.launches_shade = kn[1]; ctx->stats.launches_shadow = kn[2];
+    if (cnt[3] != 0) return fail(ctx, LUCID_ERR_DEVICE, "BVH traversal stack overflow");
+    return LUCID_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI ==
+extern "C" {
+
+int lucid_create(int32_t device, lucid_ctx** out) {
+    if (!out) return LUCID_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return LUCID_ERR_DEVICE;
+    if (device < 0 || device >= n) return LUCID_ERR_INVALID;
+    lucid_ctx* ctx = new (std::nothrow) lucid_ctx();
+    if (!ctx) return LUCID_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->wave_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->wave_ev[1], hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return LUCID_ERR_DEVICE;
+    }
+    *out = ctx;
+    return LUCID_OK;
+}
+
+void lucid_destroy(lucid_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_wave(ctx);
+    if (ctx->film) (void)hipFree(ctx->film);
+    if (ctx->out_dev) (void)hipFree(ctx->out_dev);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->wave_ev[0]);
+    (void)hipEventDestroy(ctx->wave_ev[1]);
+    for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* lucid_last_error(const lucid_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t num_meshes,
+                       const lucid_light_desc* lights, int32_t num_lights, const float env_color[3],
+                       float env_strength, lucid_scene** out) {
+    if (!ctx || !out || !env_color) return fail(ctx, LUCID_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    lucid_scene* s = new (std::nothrow) lucid_scene();
+    if (!s) return fail(ctx, LUCID_ERR_NOMEM, "out of host memory");
+    s->device = ctx->device;
+    std::string err;
+    if (!pack_scene(meshes, num_meshes, lights, num_lights, env_color, env_strength, s->host, err)) {
+        delete s;
+        return fail(ctx, LUCID_ERR_INVALID, err);
+    }
+    const PackedScene& ps = s->host;
+    if (ps.max_depth + 2 > kStack) {
+        delete s;
+        return fail(ctx, LUCID_ERR_INVALID, "BVH deeper than the traversal stack");
+    }
+    SceneDev& D = s->dev;
+    std::memset(&D, 0, sizeof(D));
+    auto up4 = [&](const std::vector<float>& v, const float4** dst) {
+        const float* p = nullptr;
+        int rc = upload(ctx, s, v, &p);
+        *dst = reinterpret_cast<const float4*>(p);
+        return rc;
+    };
+    int rc = upload(ctx, s, ps.nodes, &D.nodes);
+    if (!rc) rc = up4(ps.tri_v0, &D.v0);
+    if (!rc) rc = up4(ps.tri_e1, &D.e1);
+    if (!rc) rc = up4(ps.tri_e2, &D.e2);
+    if (!rc) rc = up4(ps.tri_fn, &D.fn);
+    if (!rc) rc = up4(ps.tri_vn, &D.vn);
+    if (!rc) rc = upload(ctx, s, ps.mats, &D.mats);
+    if (!rc) rc = upload(ctx, s, ps.lights, &D.lights);
+    if (!rc) rc = upload(ctx, s, ps.cdf, &D.cdf);
+    if (!rc) rc = upload(ctx, s, ps.native_light_ids, &D.native_ids);
+    if (!rc) rc = upload(ctx, s, ps.volumes, &D.vols);
+    if (!rc) rc = upload(ctx, s, ps.grid, &D.grid);
+    if (rc) {
+        lucid_scene_release(s);
+        return rc;
+    }
+    D.n_lights = (int32_t)ps.lights.size();
+    D.n_native = (int32_t)ps.native_light_ids.size();
+    D.n_vols = (int32_t)ps.volumes.size();
+    D.root = ps.root;
+    D.total_area = ps.total_area;
+    std::memcpy(D.env, ps.env, sizeof(D.env));
+    D.any_vbo = 0;
+    for (const GMat& m : ps.mats)
+        if (m.flags & MF_VOLUME_BOUNDARY) D.any_vbo = 1;
+    *out = s;
+    return LUCID_OK;
+}
+
+void lucid_scene_retain(lucid_scene* s) {
+    if (s) s->refs.fetch_add(1);
+}
+void lucid_scene_release(lucid_scene* s) {
+    if (!s) return;
+    if (s->refs.fetch_sub(1) == 1) {
+        (void)hipSetDevice(s->device);
+        (void)hipDeviceSynchronize();
+        for (void* p : s->allocs) (void)hipFree(p);
+        delete s;
+    }
+}
+int32_t lucid_scene_num_triangles(const lucid_scene* s) { return s ? s->host.num_triangles : 0; }
+
+int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                             const lucid_render_params* pr, float* out_rgba_dev, void* stream) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!out_rgba_dev) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    HIP_OK(hipEventRecord(ctx->ev0, st));
+    bool cancelled = false;
+    rc = run_waves(ctx, scene, cam, pr, 0, st, nullptr, &cancelled);
+    if (rc) return rc;
+    CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
+    int64_t P = (int64_t)pr->tile_w * pr->tile_h;
+    k_output<<<blocks(P, 256), 256, 0, st>>>(ctx->film, out_rgba_dev, pr->tile_w, pr->tile_h, cb.sensitivity,
+                                             cancelled ? 0 : 1);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(ctx->ev1, st));
+    return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
+}
+
+int lucid_synchronize(lucid_ctx* ctx) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return finish_stats(ctx);
+}
+
+int lucid_render_tile(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                      float* out_rgba) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!out_rgba) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    int64_t n = (int64_t)pr->tile_w * pr->tile_h * 4;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        rc = ensure_buf(ctx, &ctx->out_dev, &ctx->out_cap, n);
+        if (rc) return rc;
+    }
+    int rrc = lucid_render_tile_device(ctx, scene, cam, pr, ctx->out_dev, nullptr);
+    if (rrc != LUCID_OK && rrc != LUCID_ERR_CANCELLED) return rrc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipMemcpyAsync(out_rgba, ctx->out_dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    rc = finish_stats(ctx);
+    return rc ? rc : rrc;
+}
+
+int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                         float* out_rgb) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!out_rgb) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    size_t n = (size_t)pr->tile_w * pr->tile_h * pr->samples * 3;
+    float* dev = nullptr;
+    if (n && hipMalloc(&dev, n * sizeof(float)) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed");
+    RunOut ro;
+    ro.samples_dev = dev;
+    bool cancelled = false;
+    HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
+    rc = run_waves(ctx, scene, cam, pr, 1, ctx->stream, &ro, &cancelled);
+    if (!rc) {
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        if (n) (void)hipMemcpyAsync(out_rgb, dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, LUCID_ERR_DEVICE, "synchronize failed");
+    }
+    if (!rc) rc = finish_stats(ctx);
+    if (dev) (void)hipFree(dev);
+    if (!rc && cancelled) rc = LUCID_ERR_CANCELLED;
+    return rc;
+}
+
+int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                       int32_t* mesh, int32_t* tri, float* u, float* v, float* t) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!mesh || !tri || !u || !v || !t) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    size_t n = (size_t)pr->tile_w * pr->tile_h * pr->samples;
+    RunOut ro;
+    void* buf = nullptr;
+    if (n && hipMalloc(&buf, n * 20) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed");
+    ro.mesh = (int32_t*)buf;
+    ro.tri = ro.mesh + n;
+    ro.u = (float*)(ro.tri + n);
+    ro.v = ro.u + n;
+    ro.t = ro.v + n;
+    bool cancelled = false;
+    HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
+    rc = run_waves(ctx, scene, cam, pr, 2, ctx->stream, &ro, &cancelled);
+    if (!rc) {
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        if (n) {
+            (void)hipMemcpyAsync(mesh, ro.mesh, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+            (void)hipMemcpyAsync(tri, ro.tri, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+            (void)hipMemcpyAsync(u, ro.u, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+            (void)hipMemcpyAsync(v, ro.v, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+            (void)hipMemcpyAsync(t, ro.t, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, LUCID_ERR_DEVICE, "synchronize failed");
+    }
+    if (!rc) rc = finish_stats(ctx);
+    if (buf) (void)hipFree(buf);
+    return rc;
+}
+
+void lucid_cancel(lucid_ctx* ctx) {
+    if (ctx) ctx->cancel.store(1);
+}
+int32_t lucid_is_cancelled(const lucid_ctx* ctx) { return ctx ? ctx->cancel.load() : 0; }
+void lucid_reset_cancel(lucid_ctx* ctx) {
+    if (ctx) ctx->cancel.store(0);
+}
+
+int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out) {
+    if (!ctx || !out) return LUCID_ERR_INVALID;
+    *out = ctx->stats;
+    return LUCID_OK;
+}
+
+int lucid_set_profiling(lucid_ctx* ctx, int32_t flags) {
+    if (!ctx || flags < 0 || flags > 3) return LUCID_ERR_INVALID;
+    ctx->profiling = flags;
+    return LUCID_OK;
+}
+
+int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
+    if (!ctx || paths < 0) return LUCID_ERR_INVALID;
+    ctx->wave_paths = paths;
+    return LUCID_OK;
+}
+
+}  // extern "C"

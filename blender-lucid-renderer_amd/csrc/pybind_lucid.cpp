@@ -1,0 +1,553 @@
+// pybind_lucid.cpp -- the `lucidrenderer` Python module (host side, C++).
+//
+// Mirrors the reference's render-session API, PYBIND11_MODULE(lucidrenderer)
+// in LucidRenderer/cpp_renderer/src/pybind_module.cpp:38-543 and class
+// PyRenderer in include/pybind_renderer.hpp:62-1746: same class and method
+// names, arguments, defaults, return conventions (SUM output, bottom row
+// first, alpha 1, zeros without scene or camera, bool from load_scene_json)
+// and GIL release during rendering.  All rendering goes through the C ABI of
+// liblucid_hip.so (lucid_hip.h); there is no CPU fallback: without a usable
+// HIP device a render raises RuntimeError.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "json_lite.h"
+#include "lucid_hip.h"
+
+namespace py = pybind11;
+using lucid::json::Value;
+
+namespace {
+
+// Host copy of one mesh: the arrays a lucid_mesh_desc points into.
+struct HostMesh {
+    std::vector<float> vertices;
+    std::vector<int32_t> indices;
+    std::vector<float> tri_normals;
+    std::vector<float> grid;
+    lucid_mesh_desc desc{};
+};
+
+struct HostScene {
+    std::vector<std::unique_ptr<HostMesh>> meshes;
+    std::vector<lucid_light_desc> lights;
+    float env_color[3] = {0.05f, 0.05f, 0.05f};  // Environment defaults (core/scene.hpp:175-178)
+    float env_strength = 1.0f;
+};
+
+float f(const Value& v) { return (float)lucid::json::num(v); }
+float f_at(const Value& v, size_t i) { return (float)lucid::json::at(v, i); }
+
+// Material Output "Surface" socket linked? (pybind_renderer.hpp:1386-1398).
+// Node trees are otherwise not evaluated.
+bool surface_linked(const Value& mat) {
+    const Value* nt = mat.get("node_tree");
+    if (!nt || nt->is_null() || !nt->is_object()) return true;
+    const Value* nodes = nt->get("nodes");
+    if (!nodes || nodes->kind != Value::Arr) return true;
+    for (const Value& n : nodes->arr) {
+        const Value* ty = n.get("type");
+        if (!ty || ty->kind != Value::Str || ty->s != "ShaderNodeOutputMaterial") continue;
+        const Value* ins = n.get("inputs");
+        if (ins && ins->kind == Value::Arr) {
+            for (const Value& s : ins->arr) {
+                const Value* nm = s.get("name");
+                if (nm && nm->kind == Value::Str && nm->s == "Surface") {
+                    const Value* l = s.get("is_linked");
+                    return l && l->kind == Value::Bool && l->b;
+                }
+            }
+        }
+        return false;
+    }
+    return true;
+}
+
+// loadSceneFromJsonString (pybind_renderer.hpp:1258-1542) and
+// parseNativeLights field reading (:1544-1746); unit conversions happen in
+// the library (lucid_scene_create).
+void parse_scene(const std::string& text, HostScene& sc) {
+    Value j = lucid::json::parse(text);
+    if (!j.is_object()) throw std::runtime_error("scene JSON must be an object");
+    const Value* meshes = j.get("meshes");
+    if (!meshes) return;  // the reference returns the empty scene here
+    if (!meshes->is_array()) throw std::runtime_error("\"meshes\" must be an array");
+    if (meshes->kind == Value::Arr) {
+        for (const Value& mj : meshes->arr) {
+            auto hm = std::make_unique<HostMesh>();
+            lucid_mesh_desc& d = hm->desc;
+            std::memset(&d, 0, sizeof(d));
+            if (const Value* vs = mj.get("vertices")) {
+                int64_t n = vs->size();
+                hm->vertices.resize(3 * n);
+                if (vs->kind == Value::NumArr) {
+                    if (n > 0 && (vs->shape.size() < 2 || vs->shape[1] < 3 || !vs->regular))
+                        throw std::runtime_error("vertices must be [x, y, z] triples");
+                    int64_t stride = n > 0 ? (int64_t)vs->flat.size() / n : 3;
+                    for (int64_t i = 0; i < n; ++i)
+                        for (int k = 0; k < 3; ++k) hm->vertices[3 * i + k] = (float)vs->flat[i * stride + k];
+                } else {
+                    for (int64_t i = 0; i < n; ++i)
+                        for (int k = 0; k < 3; ++k) hm->vertices[3 * i + k] = f_at(vs->arr[i], k);
+                }
+            }
+            int64_t ntn = 0;
+            if (const Value* tn = mj.get("triangle_normals")) {
+                ntn = tn->size();
+                hm->tri_normals.resize(9 * ntn);
+                if (tn->kind == Value::NumArr) {
+                    if (ntn > 0 && (tn->flat.size() != (size_t)(9 * ntn) || !tn->regular))
+                        throw std::runtime_error("triangle_normals must be 3 x [x, y, z] per triangle");
+                    for (int64_t i = 0; i < 9 * ntn; ++i) hm->tri_normals[i] = (float)tn->flat[i];
+                } else {
+                    for (int64_t i = 0; i < ntn; ++i) {
+                        const Value& t3 = tn->arr[i];
+                        if (t3.kind != Value::NumArr || t3.flat.size() < 9)
+                            throw std::runtime_error("triangle_normals entry malformed");
+                        for (int k = 0; k < 9; ++k) hm->tri_normals[9 * i + k] = (float)t3.flat[k];
+                    }
+                }
+            }
+            bool smooth = false;
+            if (const Value* sm = mj.get("smooth")) smooth = lucid::json::boolean(*sm);
+            if (const Value* ts = mj.get("triangles")) {
+                int64_t n = ts->size();
+                hm->indices.resize(3 * n);
+                if (ts->kind == Value::NumArr) {
+                    if (n > 0 && (ts->shape.size() < 2 || ts->shape[1] < 3 || !ts->regular))
+                        throw std::runtime_error("triangles must be index triples");
+                    int64_t stride = n > 0 ? (int64_t)ts->flat.size() / n : 3;
+                    for (int64_t i = 0; i < n; ++i)
+                        for (int k = 0; k < 3; ++k) hm->indices[3 * i + k] = (int32_t)ts->flat[i * stride + k];
+                } else {
+                    for (int64_t i = 0; i < n; ++i)
+                        for (int k = 0; k < 3; ++k) hm->indices[3 * i + k] = (int32_t)lucid::json::at(ts->arr[i], k);
+                }
+            }
+            // Material defaults (pybind_renderer.hpp:1347-1350) and Mesh's
+            // default-constructed Material (core/material.hpp:414-421) when absent.
+            d.base_color[0] = d.base_color[1] = d.base_color[2] = 0.8f;
+            d.metallic = 0.0f;
+            d.roughness = 0.5f;
+            d.transmission = 0.0f;
+            d.ior = 1.45f;
+            d.surface_linked = 1;
+            if (const Value* mat = mj.get("material")) {
+                if (const Value* bc = mat->get("base_color"))
+                    for (int k = 0; k < 3; ++k) d.base_color[k] = f_at(*bc, k);
+                if (const Value* v = mat->get("metallic")) d.metallic = f(*v);
+                if (const Value* v = mat->get("roughness")) d.roughness = f(*v);
+                if (const Value* em = mat->get("emission"))
+                    for (int k = 0; k < 3; ++k) d.emission[k] = f_at(*em, k);
+                if (const Value* v = mat->get("transmission")) d.transmission = f(*v);
+                if (const Value* v = mat->get("ior")) d.ior = f(*v);
+                d.surface_linked = surface_linked(*mat) ? 1 : 0;
+                const Value* vol = mat->get("volume");
+                if (vol && vol->is_object()) {
+                    d.has_volume = 1;
+                    d.vol_color[0] = d.vol_color[1] = d.vol_color[2] = 1.0f;
+                    if (const Value* c = vol->get("color"); c && c->is_array() && c->size() >= 3)
+                        for (int k = 0; k < 3; ++k) d.vol_color[k] = f_at(*c, k);
+                    if (const Value* c = vol->get("absorption_color"); c && c->is_array() && c->size() >= 3)
+                        for (int k = 0; k < 3; ++k) d.vol_absorption[k] = f_at(*c, k);
+                    if (const Value* v = vol->get("density")) d.vol_density = f(*v);
+                    if (const Value* v = vol->get("anisotropy")) d.vol_anisotropy = f(*v);
+                    const Value* sg = vol->get("smoke_grid");
+                    if (sg && sg->is_object()) {
+                        const Value* dims = sg->get("dims");
+                        const Value* wmin = sg->get("world_min");
+                        const Value* wmax = sg->get("world_max");
+                        const Value* path = sg->get("sidecar_path");
+                        if (dims && dims->is_array() && dims->size() == 3 && wmin && wmin->is_array() && wmax &&
+                            wmax->is_array() && path) {
+                            int dd[3];
+                            for (int k = 0; k < 3; ++k) {
+                                dd[k] = (int)lucid::json::at(*dims, k);
+                                d.grid_min[k] = f_at(*wmin, k);
+                                d.grid_max[k] = f_at(*wmax, k);
+                            }
+                            size_t nvox = (size_t)dd[0] * dd[1] * dd[2];
+                            std::ifstream in(lucid::json::string(*path), std::ios::binary);
+                            if (in && dd[0] > 0 && dd[1] > 0 && dd[2] > 0) {
+                                hm->grid.assign(nvox, 0.0f);
+                                in.read(reinterpret_cast<char*>(hm->grid.data()), nvox * sizeof(float));
+                                for (int k = 0; k < 3; ++k) d.grid_dims[k] = dd[k];
+                            } else {
+                                std::cerr << "[Volume] failed to open sidecar " << lucid::json::string(*path)
+                                          << " -- heterogeneous grid skipped\n";
+                            }
+                        }
+                    }
+                }
+            }
+            d.smooth = smooth ? 1 : 0;
+            d.num_vertices = (int32_t)(hm->vertices.size() / 3);
+            d.vertices = hm->vertices.data();
+            d.num_triangles = (int32_t)(hm->indices.size() / 3);
+            d.indices = hm->indices.data();
+            d.tri_normals = hm->tri_normals.empty() ? nullptr : hm->tri_normals.data();
+            d.num_tri_normals = (int32_t)ntn;
+            d.grid = hm->grid.empty() ? nullptr : hm->grid.data();
+            for (int64_t i = 0; i < 3LL * d.num_triangles; ++i)
+                if (hm->indices[i] < 0 || hm->indices[i] >= d.num_vertices)
+                    throw std::runtime_error("triangle index out of range");
+            sc.meshes.push_back(std::move(hm));
+        }
+    } else if (meshes->size() != 0) {
+        throw std::runtime_error("\"meshes\" entries must be objects");
+    }
+    const Value* lights = j.get("native_lights");
+    if (!(lights && lights->kind == Value::Arr)) {
+        lights = j.get("lights");
+        if (!(lights && lights->kind == Value::Arr)) lights = nullptr;
+    }
+    if (lights) {
+        for (const Value& lj : lights->arr) {
+            lucid_light_desc l;
+            std::memset(&l, 0, sizeof(l));
+            std::string type = "POINT";
+            if (const Value* t = lj.get("type")) type = lucid::json::string(*t);
+            if (const Value* p = lj.get("position"))
+                for (int k = 0; k < 3; ++k) l.position[k] = f_at(*p, k);
+            if (const Value* dv = lj.get("direction")) {
+                l.has_direction = 1;
+                for (int k = 0; k < 3; ++k) l.direction[k] = f_at(*dv, k);
+            }
+            l.color[0] = l.color[1] = l.color[2] = 1.0f;
+            if (const Value* c = lj.get("color"))
+                for (int k = 0; k < 3; ++k) l.color[k] = f_at(*c, k);
+            l.energy = 1.0f;
+            if (const Value* e = lj.get("energy")) l.energy = f(*e);
+            if (const Value* r = lj.get("radius")) l.radius = f(*r);
+            l.spot_size = (float)(3.14159265358979323846 / 4.0f);
+            if (const Value* s = lj.get("spot_size")) l.spot_size = f(*s);
+            if (const Value* s = lj.get("spot_blend")) l.spot_blend = f(*s);
+            l.size = 1.0f;
+            if (const Value* s = lj.get("size")) l.size = f(*s);
+            l.size_y = l.size;
+            if (const Value* s = lj.get("size_y")) l.size_y = f(*s);
+            std::string shape = "SQUARE";
+            if (const Value* s = lj.get("shape")) shape = lucid::json::string(*s);
+            l.shape = shape == "DISK" ? LUCID_SHAPE_DISK
+                      : shape == "ELLIPSE" ? LUCID_SHAPE_ELLIPSE
+                      : shape == "RECTANGLE" ? LUCID_SHAPE_RECTANGLE
+                                             : LUCID_SHAPE_SQUARE;
+            if (const Value* r = lj.get("right")) {
+                l.has_right = 1;
+                for (int k = 0; k < 3; ++k) l.right[k] = f_at(*r, k);
+            }
+            if (const Value* u = lj.get("up")) {
+                l.has_up = 1;
+                for (int k = 0; k < 3; ++k) l.up[k] = f_at(*u, k);
+            }
+            if (type == "POINT") l.type = LUCID_LIGHT_POINT;
+            else if (type == "SUN") l.type = LUCID_LIGHT_SUN;
+            else if (type == "SPOT") l.type = LUCID_LIGHT_SPOT;
+            else if (type == "AREA") l.type = LUCID_LIGHT_AREA;
+            else continue;  // the reference keeps a default Light; it is skipped here
+            sc.lights.push_back(l);
+        }
+    }
+    const Value* env = j.get("environment");
+    if (env && env->is_object()) {
+        if (const Value* c = env->get("color"); c && c->is_array())
+            for (int k = 0; k < 3; ++k) sc.env_color[k] = f_at(*c, k);
+        if (const Value* s = env->get("strength")) sc.env_strength = f(*s);
+    }
+}
+
+int algo_code(const std::string& a) {  // pybind_renderer.hpp:301-327
+    if (a == "simple") return LUCID_ALGO_SIMPLE;
+    if (a == "mis") return LUCID_ALGO_MIS;
+    if (a == "volume_simple") return LUCID_ALGO_VOLUME_SIMPLE;
+    if (a == "volume_mis") return LUCID_ALGO_VOLUME_MIS;
+    return LUCID_ALGO_NEE;
+}
+
+class Renderer {
+public:
+    Renderer() { std::cerr << "[PyRenderer] HIP wavefront path tracer (liblucid_hip)\n"; }
+    ~Renderer() {
+        if (scene_) lucid_scene_release(scene_);
+        if (ctx_) lucid_destroy(ctx_);
+    }
+
+    void cancel() {
+        cancel_.store(true);
+        std::lock_guard<std::mutex> l(ctx_mu_);
+        if (ctx_) lucid_cancel(ctx_);
+    }
+    bool is_cancelled() const { return cancel_.load(); }
+    void reset_cancel() {
+        cancel_.store(false);
+        std::lock_guard<std::mutex> l(ctx_mu_);
+        if (ctx_) lucid_reset_cancel(ctx_);
+    }
+
+    bool load_scene_json(const std::string& text) {
+        auto sc = std::make_unique<HostScene>();
+        try {
+            parse_scene(text, *sc);
+        } catch (const std::exception& e) {
+            std::cerr << "[PyRenderer] Failed to load scene: " << e.what() << "\n";
+            scene_loaded_ = false;
+            return false;
+        }
+        std::lock_guard<std::mutex> l(render_mu_);
+        host_ = std::move(sc);
+        if (scene_) {
+            lucid_scene_release(scene_);
+            scene_ = nullptr;
+        }
+        scene_loaded_ = true;
+        std::cerr << "[PyRenderer] Scene loaded: " << host_->meshes.size() << " meshes\n";
+        return true;
+    }
+
+    void set_camera(float px, float py, float pz, float dx, float dy, float dz, float ux, float uy, float uz,
+                    float fov) {
+        cam_.pos[0] = px; cam_.pos[1] = py; cam_.pos[2] = pz;
+        cam_.dir[0] = dx; cam_.dir[1] = dy; cam_.dir[2] = dz;
+        cam_.up[0] = ux; cam_.up[1] = uy; cam_.up[2] = uz;
+        cam_.fov_deg = fov;
+        camera_set_ = true;
+    }
+    void set_algorithm(const std::string& a) { algorithm_ = a; }
+    std::string get_algorithm() const { return algorithm_; }
+    bool is_scene_loaded() const { return scene_loaded_; }
+    bool is_camera_set() const { return camera_set_; }
+    int get_mesh_count() const { return scene_loaded_ && host_ ? (int)host_->meshes.size() : 0; }
+
+    std::vector<float> render_tile(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset,
+                                   int max_depth) {
+        reset_cancel();
+        if (tw < 0 || th < 0) throw std::length_error("negative tile size");
+        std::vector<float> px((size_t)tw * th * 4, 0.0f);
+        if (!scene_loaded_ || !camera_set_) {
+            std::cerr << "[PyRenderer] Scene or camera not set\n";
+            return px;
+        }
+        if (tw == 0 || th == 0) return px;
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_scene();
+        lucid_render_params pr = params(tx, ty, tw, th, fw, fh, samples, offset, max_depth);
+        int rc = lucid_render_tile(ctx_, scene_, &cam_, &pr, px.data());
+        if (rc == LUCID_ERR_CANCELLED) {
+            std::cerr << "[PyRenderer] Render cancelled\n";
+            return px;
+        }
+        check(rc);
+        return px;
+    }
+
+    // Extension: per-sample radiance, shape (tile_h * tile_w, samples, 3).
+    py::array_t<float> debug_render_samples(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset,
+                                            int max_depth) {
+        require_ready();
+        std::vector<float> buf((size_t)tw * th * std::max(samples, 0) * 3);
+        {
+            py::gil_scoped_release rel;
+            std::lock_guard<std::mutex> l(render_mu_);
+            ensure_scene();
+            lucid_render_params pr = params(tx, ty, tw, th, fw, fh, samples, offset, max_depth);
+            check(lucid_render_samples(ctx_, scene_, &cam_, &pr, buf.data()));
+        }
+        py::array_t<float> a({(py::ssize_t)tw * th, (py::ssize_t)samples, (py::ssize_t)3});
+        std::memcpy(a.mutable_data(), buf.data(), buf.size() * sizeof(float));
+        return a;
+    }
+
+    // Extension: closest hit of every (jittered) camera ray; arrays of
+    // tile_h * tile_w * samples entries, pixel-major, top row first.
+    py::dict debug_primary_hits(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset) {
+        require_ready();
+        size_t n = (size_t)tw * th * std::max(samples, 0);
+        py::array_t<int32_t> mesh(n), tri(n);
+        py::array_t<float> u(n), v(n), t(n);
+        int32_t *pm = mesh.mutable_data(), *pt = tri.mutable_data();
+        float *pu = u.mutable_data(), *pv = v.mutable_data(), *ptt = t.mutable_data();
+        {
+            py::gil_scoped_release rel;
+            std::lock_guard<std::mutex> l(render_mu_);
+            ensure_scene();
+            lucid_render_params pr = params(tx, ty, tw, th, fw, fh, samples, offset, 1);
+            check(lucid_primary_hits(ctx_, scene_, &cam_, &pr, pm, pt, pu, pv, ptt));
+        }
+        py::dict d;
+        d["mesh"] = mesh; d["tri"] = tri; d["u"] = u; d["v"] = v; d["t"] = t;
+        return d;
+    }
+
+    py::dict last_render_stats() const {
+        py::dict d;
+        if (!ctx_) return d;
+        lucid_stats s{};
+        lucid_get_stats(ctx_, &s);
+        d["rays_closest"] = s.rays_closest;
+        d["rays_shadow"] = s.rays_shadow;
+        d["paths"] = s.paths;
+        d["waves"] = s.waves;
+        d["stack_overflows"] = s.stack_overflows;
+        d["ms_render"] = s.ms_render;
+        return d;
+    }
+    void set_wave_paths(int64_t n) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        check(lucid_set_wave_paths(ctx_, n));
+    }
+
+    // object names (pybind_renderer.hpp:1140-1253)
+    void set_object_names(const std::vector<std::string>& n) { names_ = n; }
+    std::string get_object_name(int32_t id) const {
+        if (id < 0) return "Environment";
+        if ((size_t)id < names_.size()) return names_[id];
+        return "Object_" + std::to_string(id);
+    }
+    std::string object_path_string(const std::vector<int32_t>& ids) const {
+        if (ids.empty()) return "Camera";
+        std::string path;
+        size_t start = ids.size();
+        int32_t last = ids[start - 1];
+        if (last <= -2) {
+            path = "Light(?)";
+            start--;
+        } else if (last == -1) {
+            path = "Environment(?)";
+            start--;
+        } else {
+            path = "(terminated)";
+        }
+        for (size_t i = start; i > 0; --i) {
+            path += " \xE2\x86\x92 ";
+            path += get_object_name(ids[i - 1]);
+        }
+        path += " \xE2\x86\x92 Camera";
+        return path;
+    }
+
+    std::string repr() const {
+        return "<lucidrenderer.Renderer scene_loaded=" + std::string(scene_loaded_ ? "True" : "False") +
+               " meshes=" + std::to_string(get_mesh_count()) + " algorithm='" + algorithm_ + "'>";
+    }
+
+private:
+    std::atomic<bool> cancel_{false};
+    std::mutex ctx_mu_, render_mu_;
+    lucid_ctx* ctx_ = nullptr;
+    lucid_scene* scene_ = nullptr;
+    std::unique_ptr<HostScene> host_;
+    bool scene_loaded_ = false, camera_set_ = false;
+    lucid_camera cam_{};
+    std::string algorithm_ = "nee";
+    std::vector<std::string> names_;
+
+    void require_ready() {
+        if (!scene_loaded_ || !camera_set_) throw std::runtime_error("scene or camera not set");
+    }
+    void ensure_ctx() {
+        std::lock_guard<std::mutex> l(ctx_mu_);
+        if (ctx_) return;
+        lucid_ctx* c = nullptr;
+        int rc = lucid_create(0, &c);
+        if (rc != LUCID_OK)
+            throw std::runtime_error("lucidrenderer: no usable HIP device (lucid_create returned " +
+                                     std::to_string(rc) + "); this build has no CPU fallback");
+        ctx_ = c;
+    }
+    void ensure_scene() {
+        ensure_ctx();
+        if (scene_) return;
+        std::vector<lucid_mesh_desc> md;
+        for (auto& m : host_->meshes) md.push_back(m->desc);
+        lucid_scene* s = nullptr;
+        int rc = lucid_scene_create(ctx_, md.data(), (int32_t)md.size(), host_->lights.data(),
+                                    (int32_t)host_->lights.size(), host_->env_color, host_->env_strength, &s);
+        check(rc);
+        scene_ = s;
+    }
+    lucid_render_params params(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset,
+                               int max_depth) const {
+        lucid_render_params p{};
+        p.tile_x = tx; p.tile_y = ty; p.tile_w = tw; p.tile_h = th;
+        p.full_w = fw; p.full_h = fh;
+        p.samples = samples; p.sample_offset = offset; p.max_depth = max_depth;
+        p.algorithm = algo_code(algorithm_);
+        return p;
+    }
+    void check(int rc) {
+        if (rc != LUCID_OK)
+            throw std::runtime_error(std::string("lucidrenderer: ") + (ctx_ ? lucid_last_error(ctx_) : "error") +
+                                     " (code " + std::to_string(rc) + ")");
+    }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(lucidrenderer, m) {
+    m.doc() = "Lucid Path Tracer - MI355X (HIP/gfx950) wavefront backend with the reference Python API";
+    py::class_<Renderer>(m, "Renderer")
+        .def(py::init<>(), "Create a new renderer instance")
+        .def("cancel", &Renderer::cancel, "Request cancellation of the current render (any thread)")
+        .def("is_cancelled", &Renderer::is_cancelled)
+        .def("reset_cancel", &Renderer::reset_cancel)
+        .def("load_scene_json", &Renderer::load_scene_json, py::arg("json_str"), "Load scene from JSON string")
+        .def("set_camera", &Renderer::set_camera, py::arg("pos_x"), py::arg("pos_y"), py::arg("pos_z"),
+             py::arg("dir_x"), py::arg("dir_y"), py::arg("dir_z"), py::arg("up_x"), py::arg("up_y"),
+             py::arg("up_z"), py::arg("fov_deg"))
+        .def("set_algorithm", &Renderer::set_algorithm, py::arg("algorithm"))
+        .def("get_algorithm", &Renderer::get_algorithm)
+        .def("set_object_names", &Renderer::set_object_names, py::arg("names"))
+        .def("get_object_name", &Renderer::get_object_name, py::arg("object_id"))
+        .def("object_path_string", &Renderer::object_path_string, py::arg("object_ids"))
+        .def("render_tile", &Renderer::render_tile, py::call_guard<py::gil_scoped_release>(), py::arg("tile_x"),
+             py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"),
+             py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,
+             "Render a tile and return the RGBA float list (SUM over samples, bottom row first)")
+        .def("render_tile_gpu", &Renderer::render_tile, py::call_guard<py::gil_scoped_release>(),
+             py::arg("tile_x"), py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"),
+             py::arg("full_h"), py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,
+             "Same as render_tile (every render runs on the GPU)")
+        .def(
+            "render_tile_numpy",
+            [](Renderer& self, int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset, int md) {
+                std::vector<float> px;
+                {
+                    py::gil_scoped_release rel;
+                    px = self.render_tile(tx, ty, tw, th, fw, fh, samples, offset, md);
+                }
+                py::array_t<float> a({(py::ssize_t)th, (py::ssize_t)tw, (py::ssize_t)4});
+                std::memcpy(a.mutable_data(), px.data(), px.size() * sizeof(float));
+                return a;
+            },
+            py::arg("tile_x"), py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"),
+            py::arg("full_h"), py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,
+            "Render a tile and return a NumPy array of shape (height, width, 4)")
+        .def("debug_render_samples", &Renderer::debug_render_samples, py::arg("tile_x"), py::arg("tile_y"),
+             py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"), py::arg("samples") = 1,
+             py::arg("sample_offset") = 0, py::arg("max_depth") = 8)
+        .def("debug_primary_hits", &Renderer::debug_primary_hits, py::arg("tile_x"), py::arg("tile_y"),
+             py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"), py::arg("samples") = 1,
+             py::arg("sample_offset") = 0)
+        .def("last_render_stats", &Renderer::last_render_stats)
+        .def("set_wave_paths", &Renderer::set_wave_paths, py::arg("paths"))
+        .def("is_scene_loaded", &Renderer::is_scene_loaded)
+        .def("is_camera_set", &Renderer::is_camera_set)
+        .def("get_mesh_count", &Renderer::get_mesh_count)
+        .def("__repr__", &Renderer::repr);
+    m.attr("__version__") = "1.0.0";
+    m.attr("openmp_enabled") = false;
+    m.attr("dawn_enabled") = false;
+    m.attr("hip_enabled") = true;
+}

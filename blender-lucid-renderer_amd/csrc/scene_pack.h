@@ -1,0 +1,126 @@
+// scene_pack.h -- host-side packing of a Lucid scene into the device layout.
+//
+// The device layout (all arrays 16-byte aligned, structure of arrays):
+//   nodes   GNode[]   binary BVH, child boxes stored in the parent (64 B).
+//                     Top levels: a BVH over mesh root boxes; below: the
+//                     per-mesh BVHs exactly as the reference builds them
+//                     (median split on the longest axis, leaves <= 4 triangles;
+//                     include/core/scene.hpp:65-152), so a triangle's global
+//                     position in BVH order is the reference's exact-tie key.
+//   tri_v0/e1/e2      float4 per triangle in BVH order: vertex 0, edge 1,
+//                     edge 2 (the Moller-Trumbore operands), with the mesh
+//                     index, original triangle index and light index in .w.
+//   tri_fn            float4 face normal (w = smooth flag)
+//   tri_vn            3 x float4 vertex normals per triangle (smooth meshes)
+//   mats              GMat per mesh; lights GLight[] (emissive triangles
+//                     first, then native lights, scene_lights.hpp:56-122) with
+//                     the area CDF.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lucid_hip.h"
+
+namespace lucid {
+
+constexpr int32_t kEmptyChild = 0x7fffffff;
+
+// Leaf child encoding: ~((first_triangle << 3) | count), count <= 4.
+inline int32_t encode_leaf(int32_t start, int32_t count) { return ~((start << 3) | count); }
+
+struct alignas(16) GNode {
+    float b[12];        // child 0 min xyz, max xyz, child 1 min xyz, max xyz
+    int32_t child[2];   // >= 0 node index, < 0 leaf (encode_leaf), kEmptyChild
+    int32_t pad[2];
+};
+static_assert(sizeof(GNode) == 64, "GNode is 64 bytes");
+
+enum MatFlags : int32_t { MF_EMISSIVE = 1, MF_VOLUME_BOUNDARY = 2, MF_VOLUME = 4 };
+
+struct alignas(16) GMat {
+    float albedo[3];
+    float metallic;
+    float roughness;      // clamped to >= 0.01 (getMaterialParams, path_tracer.hpp:61-62)
+    float transmission;
+    float ior;
+    int32_t flags;
+    float emission[3];
+    int32_t volume;       // index into volumes, -1 if none
+};
+static_assert(sizeof(GMat) == 48, "GMat is 48 bytes");
+
+struct alignas(16) GLight {
+    int32_t type;         // 0 emissive triangle, 1 point, 2 sun, 3 spot, 4 area
+    int32_t shape;
+    float area;
+    float radius;
+    float pos[3];
+    float spot_cos;       // cos(spotAngle * 0.5), computed on the host
+    float normal[3];
+    float spot_blend;
+    float emission[3];
+    float sx;
+    float v0[3];
+    float sy;
+    float v1[3];
+    int32_t pad0;
+    float v2[3];
+    int32_t pad1;
+    float right[3];
+    int32_t pad2;
+    float up[3];
+    int32_t pad3;
+};
+static_assert(sizeof(GLight) == 144, "GLight is 144 bytes");
+
+struct alignas(16) GVolume {
+    float color[3];
+    float density;
+    float absorption[3];
+    float anisotropy;
+    float wmin[3];
+    float grid_max;
+    float wmax[3];
+    int32_t has_grid;
+    int32_t dims[3];
+    int32_t grid_offset;  // element offset into the scene's grid buffer
+};
+static_assert(sizeof(GVolume) == 80, "GVolume is 80 bytes");
+
+struct PackedScene {
+    std::vector<GNode> nodes;
+    std::vector<float> tri_v0, tri_e1, tri_e2, tri_fn, tri_vn;  // 4 floats per entry
+    std::vector<GMat> mats;
+    std::vector<GVolume> volumes;
+    std::vector<float> grid;     // all heterogeneous grids, concatenated
+    std::vector<GLight> lights;
+    std::vector<float> cdf;
+    std::vector<int32_t> native_light_ids;  // light index of native light i (i in JSON order)
+    int32_t native_start = 0;
+    float total_area = 0.0f;
+    float env[3] = {0.05f, 0.05f, 0.05f};    // color * strength
+    int32_t num_triangles = 0;
+    int32_t num_meshes = 0;
+    int32_t root = 0;
+    int32_t max_depth = 0;                   // deepest node path (for the traversal stack)
+    bool any_smooth = false;
+    bool any_volume = false;
+};
+
+// Packs the scene; returns false and sets err on invalid input.
+bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_light_desc* lights,
+                int32_t num_lights, const float env_color[3], float env_strength, PackedScene& out,
+                std::string& err);
+
+// Camera basis and ray-generation constants of render_tile, computed on the
+// host in IEEE float exactly as set_camera / render_tile do
+// (pybind_renderer.hpp:131-160, 211-215).
+struct CameraBasis {
+    float pos[3], fwd[3], right[3], up[3];
+    float aspect, scale, sensitivity;
+};
+CameraBasis camera_basis(const lucid_camera& cam, int32_t full_w, int32_t full_h);
+
+}  // namespace lucid

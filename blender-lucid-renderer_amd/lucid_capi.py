@@ -1,0 +1,228 @@
+"""ctypes binding of liblucid_hip.so (the C ABI declared in include/lucid_hip.h).
+
+This is the Python-side counterpart of the pybind11 module: it talks to the
+library through plain pointers and POD structs only, which is how a non-C++
+host (or a benchmark that keeps its buffers in device memory) binds the path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+ALGOS = {"simple": 0, "nee": 1, "mis": 2, "volume_simple": 3, "volume_mis": 4}
+LIGHT_TYPES = {"POINT": 1, "SUN": 2, "SPOT": 3, "AREA": 4}
+SHAPES = {"SQUARE": 0, "RECTANGLE": 1, "DISK": 2, "ELLIPSE": 3}
+ERRORS = {0: "LUCID_OK", 1: "LUCID_ERR_INVALID", 2: "LUCID_ERR_DEVICE", 3: "LUCID_ERR_CANCELLED",
+          4: "LUCID_ERR_NOMEM"}
+PROFILE_WORK, PROFILE_KERNELS = 1, 2
+
+F3 = C.c_float * 3
+I3 = C.c_int32 * 3
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_int32), ("vertices", C.POINTER(C.c_float)), ("num_triangles", C.c_int32),
+                ("indices", C.POINTER(C.c_int32)), ("tri_normals", C.POINTER(C.c_float)),
+                ("num_tri_normals", C.c_int32), ("smooth", C.c_int32), ("base_color", F3), ("metallic", C.c_float),
+                ("roughness", C.c_float), ("emission", F3), ("transmission", C.c_float), ("ior", C.c_float),
+                ("has_volume", C.c_int32), ("vol_color", F3), ("vol_absorption", F3), ("vol_density", C.c_float),
+                ("vol_anisotropy", C.c_float), ("grid_dims", I3), ("grid_min", F3), ("grid_max", F3),
+                ("grid", C.POINTER(C.c_float)), ("surface_linked", C.c_int32)]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", F3), ("has_direction", C.c_int32), ("direction", F3),
+                ("color", F3), ("energy", C.c_float), ("radius", C.c_float), ("spot_size", C.c_float),
+                ("spot_blend", C.c_float), ("size", C.c_float), ("size_y", C.c_float), ("shape", C.c_int32),
+                ("has_right", C.c_int32), ("has_up", C.c_int32), ("right", F3), ("up", F3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", F3), ("dir", F3), ("up", F3), ("fov_deg", C.c_float), ("exposure_ev", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("tile_x", C.c_int32), ("tile_y", C.c_int32), ("tile_w", C.c_int32), ("tile_h", C.c_int32),
+                ("full_w", C.c_int32), ("full_h", C.c_int32), ("samples", C.c_int32), ("sample_offset", C.c_int32),
+                ("max_depth", C.c_int32), ("algorithm", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays_closest", C.c_uint64), ("rays_shadow", C.c_uint64), ("paths", C.c_uint64),
+                ("waves", C.c_uint64), ("stack_overflows", C.c_uint64), ("ms_render", C.c_double),
+                ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_shadow", C.c_uint64),
+                ("tris_shadow", C.c_uint64), ("ms_intersect", C.c_double), ("ms_shade", C.c_double),
+                ("ms_shadow", C.c_double), ("launches_intersect", C.c_uint64), ("launches_shade", C.c_uint64),
+                ("launches_shadow", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes); every symbol include/lucid_hip.h declares
+SIGNATURES = {
+    "lucid_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "lucid_destroy": (None, [C.c_void_p]),
+    "lucid_last_error": (C.c_char_p, [C.c_void_p]),
+    "lucid_scene_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.c_int32, C.POINTER(LightDesc), C.c_int32,
+                                     C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_void_p)]),
+    "lucid_scene_retain": (None, [C.c_void_p]),
+    "lucid_scene_release": (None, [C.c_void_p]),
+    "lucid_scene_num_triangles": (C.c_int32, [C.c_void_p]),
+    "lucid_render_tile": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
+                                    C.POINTER(C.c_float)]),
+    "lucid_render_tile_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
+                                           C.c_void_p, C.c_void_p]),
+    "lucid_synchronize": (C.c_int, [C.c_void_p]),
+    "lucid_render_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
+                                       C.POINTER(C.c_float)]),
+    "lucid_primary_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "lucid_cancel": (None, [C.c_void_p]),
+    "lucid_is_cancelled": (C.c_int32, [C.c_void_p]),
+    "lucid_reset_cancel": (None, [C.c_void_p]),
+    "lucid_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
+    "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None) -> C.CDLL:
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or os.path.join(_HERE, "liblucid_hip.so")
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built: run `make -C blender-lucid-renderer_amd`")
+    L = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def _f3(v, default=(0.0, 0.0, 0.0)):
+    v = default if v is None else v
+    return F3(*(float(x) for x in v))
+
+
+class SceneDescs:
+    """POD descriptors of a reference-schema scene dict, owning the arrays."""
+
+    def __init__(self, scene: dict):
+        self._keep = []
+        meshes = scene.get("meshes", [])
+        self.meshes = (MeshDesc * max(1, len(meshes)))()
+        self.n_meshes = len(meshes)
+        for i, m in enumerate(meshes):
+            d = self.meshes[i]
+            v = np.ascontiguousarray(np.asarray(m.get("vertices", []), np.float32).reshape(-1, 3))
+            t = np.ascontiguousarray(np.asarray(m.get("triangles", []), np.int32).reshape(-1, 3))
+            self._keep += [v, t]
+            d.num_vertices, d.vertices = v.shape[0], v.ctypes.data_as(C.POINTER(C.c_float))
+            d.num_triangles, d.indices = t.shape[0], t.ctypes.data_as(C.POINTER(C.c_int32))
+            tn = m.get("triangle_normals")
+            if tn is not None and len(tn):
+                tn = np.ascontiguousarray(np.asarray(tn, np.float32).reshape(-1, 9))
+                self._keep.append(tn)
+                d.tri_normals, d.num_tri_normals = tn.ctypes.data_as(C.POINTER(C.c_float)), tn.shape[0]
+            d.smooth = 1 if m.get("smooth") else 0
+            mat = m.get("material") or {}
+            d.base_color = _f3(mat.get("base_color"), (0.8, 0.8, 0.8))
+            d.metallic = float(mat.get("metallic", 0.0))
+            d.roughness = float(mat.get("roughness", 0.5))
+            d.emission = _f3(mat.get("emission"))
+            d.transmission = float(mat.get("transmission", 0.0))
+            d.ior = float(mat.get("ior", 1.45))
+            d.surface_linked = 1
+            vol = mat.get("volume")
+            if isinstance(vol, dict):
+                d.has_volume = 1
+                d.vol_color = _f3(vol.get("color"), (1.0, 1.0, 1.0))
+                d.vol_absorption = _f3(vol.get("absorption_color"))
+                d.vol_density = float(vol.get("density", 0.0))
+                d.vol_anisotropy = float(vol.get("anisotropy", 0.0))
+                sg = vol.get("smoke_grid")
+                if isinstance(sg, dict) and os.path.exists(sg.get("sidecar_path", "")):
+                    dims = [int(x) for x in sg["dims"]]
+                    g = np.fromfile(sg["sidecar_path"], np.float32, count=dims[0] * dims[1] * dims[2])
+                    self._keep.append(g)
+                    d.grid_dims, d.grid_min, d.grid_max = I3(*dims), _f3(sg["world_min"]), _f3(sg["world_max"])
+                    d.grid = g.ctypes.data_as(C.POINTER(C.c_float))
+        lights = scene.get("native_lights", scene.get("lights")) or []
+        self.lights = (LightDesc * max(1, len(lights)))()
+        self.n_lights = len(lights)
+        for i, l in enumerate(lights):
+            d = self.lights[i]
+            d.type = LIGHT_TYPES[l.get("type", "POINT")]
+            d.position = _f3(l.get("position"))
+            if "direction" in l:
+                d.has_direction, d.direction = 1, _f3(l["direction"])
+            d.color = _f3(l.get("color"), (1.0, 1.0, 1.0))
+            d.energy = float(l.get("energy", 1.0))
+            d.radius = float(l.get("radius", 0.0))
+            d.spot_size = float(l.get("spot_size", np.float32(np.pi / 4.0)))
+            d.spot_blend = float(l.get("spot_blend", 0.0))
+            d.size = float(l.get("size", 1.0))
+            d.size_y = float(l.get("size_y", d.size))
+            d.shape = SHAPES.get(l.get("shape", "SQUARE"), 0)
+            if "right" in l:
+                d.has_right, d.right = 1, _f3(l["right"])
+            if "up" in l:
+                d.has_up, d.up = 1, _f3(l["up"])
+        env = scene.get("environment") or {}
+        self.env = _f3(env.get("color"), (0.05, 0.05, 0.05))
+        self.env_strength = float(env.get("strength", 1.0))
+
+
+class Context:
+    """Owns a lucid_ctx; raises RuntimeError on any non-OK status."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.lucid_create(device, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"lucid_create failed: {ERRORS.get(rc, rc)}")
+        self.h = h
+
+    def check(self, rc, ok=(0,)):
+        if rc not in ok:
+            raise RuntimeError(f"{ERRORS.get(rc, rc)}: {self.L.lucid_last_error(self.h).decode()}")
+        return rc
+
+    def scene(self, scene: dict):
+        d = SceneDescs(scene)
+        out = C.c_void_p()
+        self.check(self.L.lucid_scene_create(self.h, d.meshes, d.n_meshes, d.lights, d.n_lights, d.env,
+                                             d.env_strength, C.byref(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        self.check(self.L.lucid_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lucid_destroy(self.h)
+            self.h = None
+
+
+def camera(cam: dict, exposure_ev: float = 0.0) -> Camera:
+    return Camera(_f3(cam["pos"]), _f3(cam["dir"]), _f3(cam["up"]), float(cam["fov"]), float(exposure_ev))
+
+
+def params(tx, ty, tw, th, fw, fh, samples, offset, max_depth, algorithm) -> RenderParams:
+    return RenderParams(tx, ty, tw, th, fw, fh, samples, offset, max_depth, ALGOS.get(algorithm, 1))

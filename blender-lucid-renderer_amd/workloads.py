@@ -1,0 +1,189 @@
+"""Procedural scenes for the path-tracing workloads, in Lucid's scene-JSON schema.
+
+The schema is the one ``load_scene_json`` reads (reference:
+LucidRenderer/cpp_renderer/include/pybind_renderer.hpp:1258-1746): ``meshes``
+with ``vertices``, ``triangles``, optional ``triangle_normals`` / ``smooth``
+and a constant ``material``; ``native_lights``; ``environment``.
+
+Workloads (all seeded; the seed is part of the returned spec):
+
+* ``closed_box``    -- 5 axis-aligned walls + 2 boxes rotated about y (34
+  triangles), one 0.5 m x 0.5 m square area light of radiance 15, diffuse
+  Principled materials with base colours U[0.2, 0.8].  256^2, 16 spp, depth 8.
+* ``icospheres``    -- 200 seed-placed icospheres (subdivision 4, 5120
+  triangles each = 1,024,000 triangles) with seeded radial displacement and
+  smooth normals; per object roughness U[0.05, 1], metallic Bernoulli(0.3),
+  IOR U[1.3, 1.7]; one point, one sun and 4 area lights.  1024^2, 64 spp,
+  depth 12.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+
+import numpy as np
+
+
+def _quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3)."""
+    v = np.array([p0, p1, p2, p3], dtype=np.float64)
+    t = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int64)
+    return v, t
+
+
+def _box(center, half, angle_y):
+    """Axis-aligned box of half extents `half`, rotated by angle_y about +y."""
+    hx, hy, hz = half
+    corners = np.array([[sx * hx, sy * hy, sz * hz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+    c, s = math.cos(angle_y), math.sin(angle_y)
+    rot = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    v = corners @ rot.T + np.asarray(center)
+    # corner index = 4*ix + 2*iy + iz
+    faces = [
+        (0, 1, 3, 2),  # -x
+        (4, 6, 7, 5),  # +x
+        (0, 4, 5, 1),  # -y
+        (2, 3, 7, 6),  # +y
+        (0, 2, 6, 4),  # -z
+        (1, 5, 7, 3),  # +z
+    ]
+    t = []
+    for a, b, cc, d in faces:
+        t.append([a, b, cc])
+        t.append([a, cc, d])
+    return v, np.array(t, dtype=np.int64)
+
+
+def _diffuse(color):
+    return {"base_color": [float(x) for x in color], "metallic": 0.0, "roughness": 1.0,
+            "emission": [0.0, 0.0, 0.0], "transmission": 0.0, "ior": 1.45}
+
+
+def _mesh(v, t, material, normals=None, smooth=False):
+    m = {"vertices": np.asarray(v, np.float32).tolist(), "triangles": np.asarray(t, np.int64).tolist(),
+         "material": material}
+    if normals is not None:
+        m["triangle_normals"] = np.asarray(normals, np.float32).reshape(-1, 3, 3).tolist()
+        m["smooth"] = bool(smooth)
+    return m
+
+
+def closed_box(seed: int = 1) -> dict:
+    """34-triangle closed box lit by a 0.25 m^2 area light of radiance 15."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    walls = [
+        _quad([-1, 0, -1], [-1, 0, 1], [1, 0, 1], [1, 0, -1]),      # floor  (y = 0)
+        _quad([-1, 2, -1], [1, 2, -1], [1, 2, 1], [-1, 2, 1]),      # ceiling (y = 2)
+        _quad([-1, 0, -1], [1, 0, -1], [1, 2, -1], [-1, 2, -1]),    # back (z = -1)
+        _quad([-1, 0, -1], [-1, 2, -1], [-1, 2, 1], [-1, 0, 1]),    # left (x = -1)
+        _quad([1, 0, -1], [1, 0, 1], [1, 2, 1], [1, 2, -1]),        # right (x = +1)
+    ]
+    for v, t in walls:
+        meshes.append(_mesh(v, t, _diffuse(rng.uniform(0.2, 0.8, 3))))
+    boxes = [
+        _box([-0.4, 0.6, -0.3], [0.3, 0.6, 0.3], float(rng.uniform(0.1, 0.6))),
+        _box([0.45, 0.3, 0.35], [0.3, 0.3, 0.3], float(-rng.uniform(0.1, 0.6))),
+    ]
+    for v, t in boxes:
+        meshes.append(_mesh(v, t, _diffuse(rng.uniform(0.2, 0.8, 3))))
+    radiance, size = 15.0, 0.5
+    light = {"type": "AREA", "name": "Light", "position": [0.0, 1.98, 0.0], "direction": [0.0, -1.0, 0.0],
+             "right": [1.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0], "shape": "SQUARE", "size": size,
+             "color": [1.0, 1.0, 1.0], "energy": radiance * math.pi * size * size}
+    return {"meshes": meshes, "native_lights": [light],
+            "environment": {"color": [0.05, 0.05, 0.05], "strength": 1.0}}
+
+
+CLOSED_BOX_CAMERA = {"pos": (0.0, 1.0, 3.6), "dir": (0.0, 0.0, -1.0), "up": (0.0, 1.0, 0.0), "fov": 40.0}
+
+
+def _icosphere(level: int):
+    """Unit icosphere: (vertices (n,3), triangles (m,3)), 20*4^level triangles."""
+    p = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0], [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
+         [p, 0, -1], [p, 0, 1], [-p, 0, -1], [-p, 0, 1]]
+    v = [list(np.array(x, float) / np.linalg.norm(x)) for x in v]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            k = (a, b) if a < b else (b, a)
+            if k not in cache:
+                m = (np.array(v[a]) + np.array(v[b])) / 2.0
+                v.append(list(m / np.linalg.norm(m)))
+                cache[k] = len(v) - 1
+            return cache[k]
+
+        nf = []
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return np.array(v, np.float64), np.array(f, np.int64)
+
+
+_ICO_CACHE = {}
+
+
+def icospheres(seed: int = 2, count: int = 200, level: int = 4, smooth: bool = True) -> dict:
+    """`count` displaced icospheres with per-object Principled materials."""
+    rng = np.random.default_rng(seed)
+    if level not in _ICO_CACHE:
+        _ICO_CACHE[level] = _icosphere(level)
+    base_v, base_t = _ICO_CACHE[level]
+    meshes = []
+    for i in range(count):
+        radius = rng.uniform(0.35, 0.9)
+        center = np.array([rng.uniform(-9, 9), rng.uniform(0.4, 5.0), rng.uniform(-9, 9)])
+        # seeded radial displacement: a few random low-frequency lobes
+        k = rng.normal(size=(4, 3))
+        k /= np.linalg.norm(k, axis=1, keepdims=True)
+        amp = rng.uniform(0.02, 0.12, size=4)
+        disp = 1.0 + (amp[None, :] * np.sin(3.0 * base_v @ k.T + rng.uniform(0, 6.28, size=4))).sum(1)
+        pos = base_v * (radius * disp)[:, None] + center
+        # smooth vertex normals: area-weighted face normals
+        a, b, c = pos[base_t[:, 0]], pos[base_t[:, 1]], pos[base_t[:, 2]]
+        fn = np.cross(b - a, c - a)
+        vn = np.zeros_like(pos)
+        for j in range(3):
+            np.add.at(vn, base_t[:, j], fn)
+        vn /= np.linalg.norm(vn, axis=1, keepdims=True)
+        normals = vn[base_t] if smooth else None
+        metallic = 1.0 if rng.uniform() < 0.3 else 0.0
+        mat = {"base_color": rng.uniform(0.15, 0.9, 3).tolist(), "metallic": metallic,
+               "roughness": float(rng.uniform(0.05, 1.0)), "emission": [0.0, 0.0, 0.0],
+               "transmission": 0.0, "ior": float(rng.uniform(1.3, 1.7))}
+        meshes.append(_mesh(pos, base_t, mat, normals, smooth))
+    lights = [
+        {"type": "POINT", "name": "Point", "position": [0.0, 9.0, 2.0], "color": [1.0, 0.95, 0.9],
+         "energy": 2500.0, "radius": 0.0},
+        {"type": "SUN", "name": "Sun", "position": [0.0, 20.0, 0.0], "direction": [-0.3, -1.0, -0.4],
+         "color": [1.0, 1.0, 1.0], "energy": 2.0},
+    ]
+    for j, (x, z) in enumerate([(-6, -6), (6, -6), (-6, 6), (6, 6)]):
+        lights.append({"type": "AREA", "name": f"Area{j}", "position": [float(x), 8.0, float(z)],
+                       "direction": [0.0, -1.0, 0.0], "right": [1.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0],
+                       "shape": "RECTANGLE", "size": 2.0, "size_y": 1.0, "color": [1.0, 1.0, 1.0],
+                       "energy": 600.0})
+    return {"meshes": meshes, "native_lights": lights,
+            "environment": {"color": [0.05, 0.06, 0.08], "strength": 1.0}}
+
+
+ICOSPHERES_CAMERA = {"pos": (0.0, 6.0, 20.0), "dir": (0.0, -0.25, -1.0), "up": (0.0, 1.0, 0.0), "fov": 55.0}
+
+
+def to_json(scene: dict) -> str:
+    return json.dumps(scene, separators=(",", ":"))
+
+
+WORKLOADS = {
+    "closed_box": {"make": closed_box, "camera": CLOSED_BOX_CAMERA, "width": 256, "height": 256, "spp": 16,
+                   "max_depth": 8, "algorithm": "mis"},
+    "icospheres": {"make": icospheres, "camera": ICOSPHERES_CAMERA, "width": 1024, "height": 1024, "spp": 64,
+                   "max_depth": 12, "algorithm": "mis"},
+}

@@ -1,0 +1,200 @@
+/*
+ * lucid_hip.h -- C ABI of the MI355X path-tracing library (liblucid_hip.so).
+ *
+ * Plain pointers, sizes and POD structs only.  The host owns every host
+ * buffer passed in or out; the library owns device memory.  No exception
+ * crosses this boundary: every call returns a status code and the last error
+ * text of a context is available from lucid_last_error().
+ *
+ * Reference interfaces each entry point replaces (paths relative to
+ * LucidRenderer/cpp_renderer/ of daikiad/blender-lucid-renderer):
+ *   lucid_scene_create   PyRenderer::loadSceneFromJsonString, the mesh /
+ *                        native-light / environment conversion
+ *                        (include/pybind_renderer.hpp:1258-1746) plus
+ *                        finalizeMeshBounds + BVH::build
+ *                        (include/core/scene.hpp:65-227) and the GPU scene
+ *                        packer pack_scene_for_path_tracer
+ *                        (src/gpu/path_tracer.cpp:162-469)
+ *   lucid_render_tile    PyRenderer::render_tile
+ *                        (include/pybind_renderer.hpp:185-361) and the Dawn
+ *                        dispatch PathTracer::render (src/gpu/path_tracer.cpp:591-822)
+ *   lucid_cancel & co.   PyRenderer::cancel / is_cancelled / reset_cancel
+ *                        (include/pybind_renderer.hpp:86-102)
+ *   lucid_primary_hits   intersectScene on the camera rays of render_tile
+ *                        (include/geometry/bvh.hpp:155-177), exposed for the
+ *                        bit-exact primary-hit check
+ *   lucid_render_samples traceMIS & co. per sample (include/integrator/
+ *                        path_tracer.hpp:128-835), per-sample radiance out
+ */
+#ifndef LUCID_HIP_H
+#define LUCID_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LUCID_OK 0
+#define LUCID_ERR_INVALID 1   /* bad argument */
+#define LUCID_ERR_DEVICE 2    /* HIP runtime / device failure */
+#define LUCID_ERR_CANCELLED 3 /* render stopped by lucid_cancel */
+#define LUCID_ERR_NOMEM 4     /* device allocation failed */
+
+/* integrators: the algorithm strings of PyRenderer::set_algorithm
+   (include/pybind_renderer.hpp:301-327); unknown strings mean NEE */
+#define LUCID_ALGO_SIMPLE 0
+#define LUCID_ALGO_NEE 1
+#define LUCID_ALGO_MIS 2
+#define LUCID_ALGO_VOLUME_SIMPLE 3
+#define LUCID_ALGO_VOLUME_MIS 4
+
+/* native light types (include/light/light.hpp:22-28) and area shapes (:30-35) */
+#define LUCID_LIGHT_POINT 1
+#define LUCID_LIGHT_SUN 2
+#define LUCID_LIGHT_SPOT 3
+#define LUCID_LIGHT_AREA 4
+#define LUCID_SHAPE_SQUARE 0
+#define LUCID_SHAPE_RECTANGLE 1
+#define LUCID_SHAPE_DISK 2
+#define LUCID_SHAPE_ELLIPSE 3
+
+typedef struct lucid_ctx lucid_ctx;
+typedef struct lucid_scene lucid_scene;
+
+/* One mesh of the scene JSON "meshes" array (pybind_renderer.hpp:1266-1507). */
+typedef struct lucid_mesh_desc {
+    int32_t num_vertices;
+    const float* vertices;    /* 3 floats per vertex */
+    int32_t num_triangles;
+    const int32_t* indices;   /* 3 per triangle */
+    const float* tri_normals; /* 9 floats per triangle ("triangle_normals") or NULL */
+    int32_t num_tri_normals;  /* triangles [0, num_tri_normals) carry normals */
+    int32_t smooth;           /* "smooth" */
+    float base_color[3];
+    float metallic, roughness;
+    float emission[3];
+    float transmission, ior;
+    int32_t has_volume;       /* "volume" object present */
+    float vol_color[3], vol_absorption[3];
+    float vol_density, vol_anisotropy;
+    int32_t grid_dims[3];     /* heterogeneous grid, x fastest */
+    float grid_min[3], grid_max[3];
+    const float* grid;        /* dims product floats, or NULL */
+    int32_t surface_linked;   /* Material Output Surface linked (0 + volume => boundary-only) */
+} lucid_mesh_desc;
+
+/* One entry of "native_lights" (pybind_renderer.hpp:1544-1746); the library
+   applies the reference's watt -> radiance conversions. */
+typedef struct lucid_light_desc {
+    int32_t type;             /* LUCID_LIGHT_* */
+    float position[3];
+    int32_t has_direction;
+    float direction[3];
+    float color[3];
+    float energy;
+    float radius;
+    float spot_size;
+    float spot_blend;
+    float size, size_y;
+    int32_t shape;            /* LUCID_SHAPE_* */
+    int32_t has_right, has_up;
+    float right[3], up[3];
+} lucid_light_desc;
+
+/* set_camera arguments (pybind_renderer.hpp:131-160) plus the film exposure. */
+typedef struct lucid_camera {
+    float pos[3];
+    float dir[3];
+    float up[3];
+    float fov_deg;
+    float exposure_ev;
+} lucid_camera;
+
+/* render_tile arguments (pybind_renderer.hpp:185-191). */
+typedef struct lucid_render_params {
+    int32_t tile_x, tile_y, tile_w, tile_h;
+    int32_t full_w, full_h;
+    int32_t samples;
+    int32_t sample_offset;
+    int32_t max_depth;
+    int32_t algorithm;        /* LUCID_ALGO_* */
+} lucid_render_params;
+
+/* Counters of the last render on a context. */
+typedef struct lucid_stats {
+    uint64_t rays_closest;    /* camera + extension closest-hit queries */
+    uint64_t rays_shadow;     /* shadow (any-hit) queries */
+    uint64_t paths;           /* camera paths traced */
+    uint64_t waves;           /* wavefront batches */
+    uint64_t stack_overflows; /* traversal stack overflows (must stay 0) */
+    double ms_render;         /* device time of the last render (HIP events) */
+    /* with LUCID_PROFILE_WORK: BVH nodes (64 B) and triangles (48 B) fetched */
+    uint64_t nodes_closest, tris_closest, nodes_shadow, tris_shadow;
+    /* with LUCID_PROFILE_KERNELS: summed HIP-event time and launch count of
+       the intersect, shade and shadow kernels */
+    double ms_intersect, ms_shade, ms_shadow;
+    uint64_t launches_intersect, launches_shade, launches_shadow;
+} lucid_stats;
+
+#define LUCID_PROFILE_WORK 1    /* count node / triangle fetches */
+#define LUCID_PROFILE_KERNELS 2 /* time every kernel launch with HIP events */
+
+/* Context: one device, one in-flight render. */
+int lucid_create(int32_t device, lucid_ctx** out);
+void lucid_destroy(lucid_ctx* ctx);
+const char* lucid_last_error(const lucid_ctx* ctx);
+
+/* Scene: packed and uploaded once; refcounted (create holds one reference). */
+int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t num_meshes,
+                       const lucid_light_desc* lights, int32_t num_lights, const float env_color[3],
+                       float env_strength, lucid_scene** out);
+void lucid_scene_retain(lucid_scene* scene);
+void lucid_scene_release(lucid_scene* scene);
+int32_t lucid_scene_num_triangles(const lucid_scene* scene);
+
+/* render_tile: writes tile_w*tile_h*4 floats to out_rgba (host memory):
+   RGBA, bottom row first, RGB = max(0, 2^EV * sum over `samples` of the
+   path radiance), alpha = 1.  On LUCID_ERR_CANCELLED the partial sums are
+   written top row first, as the reference's early return does. */
+int lucid_render_tile(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                      const lucid_render_params* params, float* out_rgba);
+
+/* Same, with the output left in device memory (out_rgba_dev, tile_w*tile_h*4
+   floats) and no host copy; `stream` is a hipStream_t or NULL for the
+   library's own stream.  Returns after the work is queued. */
+int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                             const lucid_render_params* params, float* out_rgba_dev, void* stream);
+int lucid_synchronize(lucid_ctx* ctx);
+
+/* Per-sample radiance (before exposure and clamp) of every pixel and sample:
+   out_rgb[((py*tile_w + px)*samples + s)*3 + c], tile order (top row first). */
+int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                         const lucid_render_params* params, float* out_rgb);
+
+/* Closest hit of every (jittered) camera ray of a tile: mesh index, original
+   triangle index, raw barycentric u, v and t (t = 0, ids = -1 on a miss). */
+int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                       const lucid_render_params* params, int32_t* mesh, int32_t* tri, float* u, float* v,
+                       float* t);
+
+/* Cooperative cancel, callable from any thread; polled between wavefront
+   batches.  render entry points clear the flag on entry. */
+void lucid_cancel(lucid_ctx* ctx);
+int32_t lucid_is_cancelled(const lucid_ctx* ctx);
+void lucid_reset_cancel(lucid_ctx* ctx);
+
+int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out);
+
+/* Paths per wavefront batch (0 = automatic). */
+int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
+
+/* Instrumentation flags (LUCID_PROFILE_*), 0 = off (default). */
+int lucid_set_profiling(lucid_ctx* ctx, int32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUCID_HIP_H */

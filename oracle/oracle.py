@@ -1,0 +1,375 @@
+"""ctypes front end of the CPU oracle (liblucid_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``.  The product never imports it.
+
+The oracle restates the reference CPU tracer (LucidRenderer/cpp_renderer/
+include/**) in plain float C++; see lucid_oracle.cpp for the per-function
+file:line citations.  Scenes are given in the reference's scene-JSON schema
+(pybind_renderer.hpp:1258-1746) as a Python dict.
+
+How each part is pinned (checks live in tests/test_oracle_*.py):
+
+* seed_random_xyz / pcg32 / randf -- PCG32 core checked against the published
+  pcg32 demo stream (pcg32_srandom_r(42, 54)) and the textbook mixers
+  (MurmurHash3 fmix32, SplitMix64, Jenkins 32-bit) recomputed independently
+  in numpy; randf by its closed form.
+* Moller-Trumbore, AABB slab, sphere / rectangle / ellipse tests -- the
+  reference's known answers (test_geometry.cpp) plus brute-force
+  plane/barycentric solutions in float64.
+* Fresnel, GGX D / G1 / G2 -- reference known answers (test_bsdf.cpp:
+  F(1, 1.5) = 0.04, TIR = 1) plus closed forms evaluated in float64.
+* BSDF sampling and pdf -- Monte Carlo: the sampled-direction histogram
+  matches pdf_bsdf, and pdf_bsdf integrates to (1 - transmission) over the
+  hemisphere.
+* light pdf, MIS power heuristic, bsdf_sample_weight -- reference known
+  answers (test_light.cpp 2.25 sr^-1; test_integrator.cpp 0.8 weight;
+  heuristic formula and complementarity).
+* BVH build and closest-hit traversal -- brute force over all triangles on
+  random rays (same closest t, tie broken by the reference's leaf order).
+* traceSimple / traceNEE / traceMIS -- whole-path statistics from the
+  reference's tests: NEE and MIS agree within 3 sigma on the Cornell box
+  (test_path_tracer.cpp:193-217), env-only scene returns the env colour
+  exactly (:132-152), white furnace bound (:314-361), closed-form point light
+  f*I/d^2 (test_gpu_path_tracer.cpp:281-349), and a direct-lighting integral
+  computed by numerical quadrature.
+* traceVolumeMIS -- homogeneous-slab transmittance against exp(-sigma*d)
+  closed form; scatter vs no-scatter energy bound.  Per-sample agreement with
+  the reference binary is "parity unpinned" (the reference cannot be built
+  here: it needs mp-units 2.4.0, nlohmann_json 3.11.3 and Dawn).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import struct
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+LIGHT_TYPES = {"POINT": 1, "SUN": 2, "SPOT": 3, "AREA": 4}
+AREA_SHAPES = {"SQUARE": 0, "RECTANGLE": 1, "DISK": 2, "ELLIPSE": 3}
+
+
+class _MeshDesc(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_int32),
+        ("vertices", C.POINTER(C.c_float)),
+        ("num_triangles", C.c_int32),
+        ("indices", C.POINTER(C.c_int32)),
+        ("tri_normals", C.POINTER(C.c_float)),
+        ("num_tri_normals", C.c_int32),
+        ("smooth", C.c_int32),
+        ("base_color", C.c_float * 3),
+        ("metallic", C.c_float),
+        ("roughness", C.c_float),
+        ("emission", C.c_float * 3),
+        ("transmission", C.c_float),
+        ("ior", C.c_float),
+        ("has_volume", C.c_int32),
+        ("vol_color", C.c_float * 3),
+        ("vol_absorption", C.c_float * 3),
+        ("vol_density", C.c_float),
+        ("vol_anisotropy", C.c_float),
+        ("grid_dims", C.c_int32 * 3),
+        ("grid_min", C.c_float * 3),
+        ("grid_max", C.c_float * 3),
+        ("grid", C.POINTER(C.c_float)),
+        ("surface_linked", C.c_int32),
+    ]
+
+
+class _LightDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("position", C.c_float * 3),
+        ("has_direction", C.c_int32),
+        ("direction", C.c_float * 3),
+        ("color", C.c_float * 3),
+        ("energy", C.c_float),
+        ("radius", C.c_float),
+        ("spot_size", C.c_float),
+        ("spot_blend", C.c_float),
+        ("size", C.c_float),
+        ("size_y", C.c_float),
+        ("shape", C.c_int32),
+        ("has_right", C.c_int32),
+        ("has_up", C.c_int32),
+        ("right", C.c_float * 3),
+        ("up", C.c_float * 3),
+    ]
+
+
+_libs = {}
+
+
+def lib(fast: bool = False) -> C.CDLL:
+    """Loads liblucid_oracle.so (parity build) or the -ffast-math baseline build."""
+    name = "liblucid_oracle_fast.so" if fast else "liblucid_oracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+    L = C.CDLL(path)
+    vp, fp, ip, i32p, i64p, dp = C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_longlong), C.POINTER(C.c_double)
+    L.orc_scene_new.restype = vp
+    L.orc_scene_free.argtypes = [vp]
+    L.orc_scene_add_mesh.argtypes = [vp, C.POINTER(_MeshDesc)]
+    L.orc_scene_add_light.argtypes = [vp, C.POINTER(_LightDesc)]
+    L.orc_scene_set_env.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.orc_scene_finalize.argtypes = [vp]
+    L.orc_set_camera.argtypes = [vp] + [C.c_float] * 10
+    L.orc_set_exposure.argtypes = [vp, C.c_float]
+    L.orc_num_lights.argtypes = [vp]
+    L.orc_native_light_start.argtypes = [vp]
+    L.orc_light_cdf.argtypes = [vp, ip]
+    L.orc_light_cdf.restype = C.c_float
+    L.orc_light_total_area.argtypes = [vp]
+    L.orc_light_total_area.restype = C.c_float
+    L.orc_light_emission.argtypes = [vp, ip, fp]
+    L.orc_render_tile.argtypes = [vp, C.c_char_p] + [ip] * 11 + [fp, dp, dp, i64p]
+    L.orc_trace_pixel.argtypes = [vp, C.c_char_p] + [ip] * 8 + [fp]
+    L.orc_primary_hits.argtypes = [vp] + [ip] * 8 + [i32p, i32p, i32p, fp, fp, fp]
+    L.orc_intersect.argtypes = [vp, ip, fp, fp, fp, fp, i32p, i32p, i32p, fp, fp, fp]
+    L.orc_mesh_bvh.argtypes = [vp, ip, C.POINTER(C.c_int), fp, i32p, i32p]
+    L.orc_seed.argtypes = [C.c_uint32] * 3 + [C.POINTER(C.c_uint64)] * 2
+    L.orc_pcg32_stream.argtypes = [C.c_uint64, C.c_uint64, ip, C.POINTER(C.c_uint32)]
+    L.orc_randf_stream.argtypes = [C.c_uint32] * 3 + [ip, fp]
+    for nm in ("orc_jenkins", "orc_fmix32"):
+        getattr(L, nm).argtypes = [C.c_uint32]
+        getattr(L, nm).restype = C.c_uint32
+    L.orc_splitmix64.argtypes = [C.c_uint64]
+    L.orc_splitmix64.restype = C.c_uint64
+    L.orc_intersect_triangle.argtypes = [fp] * 5 + [C.c_float, C.c_float, fp]
+    L.orc_intersect_aabb.argtypes = [fp] * 4
+    L.orc_intersect_sphere.argtypes = [fp, fp, fp, C.c_float, fp]
+    for nm, n in (("orc_fresnel_dielectric", 2), ("orc_fresnel_schlick", 2), ("orc_ggx_d", 2), ("orc_ggx_g1", 2),
+                  ("orc_ggx_g2", 3), ("orc_power_heuristic", 2), ("orc_pdf_w_from_area", 3), ("orc_hg_eval", 2)):
+        getattr(L, nm).argtypes = [C.c_float] * n
+        getattr(L, nm).restype = C.c_float
+    L.orc_bsdf_weight.argtypes = [fp, C.c_float, C.c_float, fp]
+    L.orc_eval_bsdf.argtypes = [fp, fp, fp, fp, fp, fp]
+    L.orc_sample_bsdf.argtypes = [fp, fp, fp, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, fp]
+    L.orc_sample_light.argtypes = [vp, ip, fp, C.c_float, C.c_float, fp]
+    L.orc_select_light.argtypes = [vp, C.c_float, fp]
+    L.orc_pdf_light_area.argtypes = [C.c_float, fp, fp, fp]
+    L.orc_pdf_light_area.restype = C.c_float
+    _libs[name] = L
+    return L
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f3(v, default=(0.0, 0.0, 0.0)):
+    v = default if v is None else v
+    return (C.c_float * 3)(float(v[0]), float(v[1]), float(v[2]))
+
+
+def surface_linked(material: dict) -> bool:
+    """Material Output 'Surface' socket linked? (pybind_renderer.hpp:1386-1398)."""
+    nt = material.get("node_tree")
+    if not nt:
+        return True
+    nodes = nt.get("nodes", []) if isinstance(nt, dict) else []
+    for n in nodes:
+        if n.get("type") == "ShaderNodeOutputMaterial":
+            for s in n.get("inputs", []):
+                if s.get("name") == "Surface":
+                    return bool(s.get("is_linked", False))
+            return False
+    return True
+
+
+class OracleScene:
+    """A scene loaded into the oracle, from a reference-schema scene dict."""
+
+    def __init__(self, scene: dict, fast: bool = False):
+        self.L = lib(fast)
+        self.h = self.L.orc_scene_new()
+        self._ntri = []
+        for m in scene.get("meshes", []):
+            self._add_mesh(m)
+        lights = scene.get("native_lights", scene.get("lights"))
+        for l in lights or []:
+            self._add_light(l)
+        env = scene.get("environment")
+        if isinstance(env, dict):
+            col = env.get("color", [0.05, 0.05, 0.05])
+            self.L.orc_scene_set_env(self.h, float(col[0]), float(col[1]), float(col[2]), float(env.get("strength", 1.0)))
+        self.L.orc_scene_finalize(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_scene_free(self.h)
+        except Exception:
+            pass
+
+    def _add_mesh(self, m: dict):
+        verts = np.ascontiguousarray(np.asarray(m.get("vertices", []), dtype=np.float32).reshape(-1, 3))
+        tris = np.ascontiguousarray(np.asarray(m.get("triangles", []), dtype=np.int32).reshape(-1, 3))
+        d = _MeshDesc()
+        d.num_vertices = verts.shape[0]
+        d.vertices = _fptr(verts)
+        d.num_triangles = tris.shape[0]
+        d.indices = _iptr(tris)
+        keep = [verts, tris]
+        tn = m.get("triangle_normals")
+        if tn is not None and len(tn) > 0:
+            tn = np.ascontiguousarray(np.asarray(tn, dtype=np.float32).reshape(-1, 9)[: tris.shape[0]])
+            d.tri_normals = _fptr(tn)
+            d.num_tri_normals = tn.shape[0]
+            keep.append(tn)
+        d.smooth = 1 if m.get("smooth", False) else 0
+        mat = m.get("material") or {}
+        d.base_color = _f3(mat.get("base_color"), (0.8, 0.8, 0.8))
+        d.metallic = float(mat.get("metallic", 0.0))
+        d.roughness = float(mat.get("roughness", 0.5))
+        d.emission = _f3(mat.get("emission"))
+        d.transmission = float(mat.get("transmission", 0.0))
+        d.ior = float(mat.get("ior", 1.45))
+        d.surface_linked = 1 if surface_linked(mat) else 0
+        vol = mat.get("volume")
+        if isinstance(vol, dict):
+            d.has_volume = 1
+            d.vol_color = _f3(vol.get("color"), (1.0, 1.0, 1.0))
+            d.vol_absorption = _f3(vol.get("absorption_color"))
+            d.vol_density = float(vol.get("density", 0.0))
+            d.vol_anisotropy = float(vol.get("anisotropy", 0.0))
+            sg = vol.get("smoke_grid")
+            if isinstance(sg, dict) and os.path.exists(sg.get("sidecar_path", "")):
+                dims = [int(x) for x in sg["dims"]]
+                n = dims[0] * dims[1] * dims[2]
+                g = np.fromfile(sg["sidecar_path"], dtype=np.float32, count=n)
+                if g.size == n:
+                    g = np.ascontiguousarray(g)
+                    d.grid_dims = (C.c_int32 * 3)(*dims)
+                    d.grid_min = _f3(sg["world_min"])
+                    d.grid_max = _f3(sg["world_max"])
+                    d.grid = _fptr(g)
+                    keep.append(g)
+        rc = self.L.orc_scene_add_mesh(self.h, C.byref(d))
+        del keep
+        if rc != 0:
+            raise ValueError("oracle rejected mesh (bad triangle index)")
+        self._ntri.append(int(tris.shape[0]))
+
+    def _add_light(self, l: dict):
+        d = _LightDesc()
+        d.type = LIGHT_TYPES.get(l.get("type", "POINT"), 0)
+        d.position = _f3(l.get("position"))
+        if "direction" in l:
+            d.has_direction = 1
+            d.direction = _f3(l["direction"])
+        d.color = _f3(l.get("color"), (1.0, 1.0, 1.0))
+        d.energy = float(l.get("energy", 1.0))
+        d.radius = float(l.get("radius", 0.0))
+        d.spot_size = float(l.get("spot_size", np.float32(np.pi / 4.0)))
+        d.spot_blend = float(l.get("spot_blend", 0.0))
+        d.size = float(l.get("size", 1.0))
+        d.size_y = float(l.get("size_y", d.size))
+        d.shape = AREA_SHAPES.get(l.get("shape", "SQUARE"), 0)
+        if "right" in l:
+            d.has_right = 1
+            d.right = _f3(l["right"])
+        if "up" in l:
+            d.has_up = 1
+            d.up = _f3(l["up"])
+        if self.L.orc_scene_add_light(self.h, C.byref(d)) != 0:
+            raise ValueError(f"unknown light type {l.get('type')}")
+
+    # ---- camera / render ----
+    def set_camera(self, pos, direction, up, fov_deg):
+        self.L.orc_set_camera(self.h, *[float(x) for x in pos], *[float(x) for x in direction],
+                              *[float(x) for x in up], float(fov_deg))
+
+    def set_exposure(self, ev: float):
+        self.L.orc_set_exposure(self.h, float(ev))
+
+    def render_tile(self, algorithm, tx, ty, tw, th, fw, fh, samples=1, sample_offset=0, max_depth=8,
+                    fp64=False, threads=0, stats=False, strategies=False):
+        """Returns (rgba_flat, extras).  rgba_flat mirrors PyRenderer::render_tile:
+        tw*th*4 floats, bottom row first, RGB = clamp(SUM * 2^EV).
+        extras: 'stats' (th*tw, 6) float64 [sum, sumsq] per channel (tile order,
+        top row first), 'strat' (th*tw, 2, 6), 'counts' (closest, shadow)."""
+        out = np.zeros(tw * th * 4, dtype=np.float32)
+        st = np.zeros((th * tw, 6), dtype=np.float64) if stats else None
+        sd = np.zeros((th * tw, 2, 6), dtype=np.float64) if strategies else None
+        counts = np.zeros(2, dtype=np.int64)
+        self.L.orc_render_tile(self.h, algorithm.encode(), tx, ty, tw, th, fw, fh, samples, sample_offset, max_depth,
+                               1 if fp64 else 0, threads, _fptr(out),
+                               st.ctypes.data_as(C.POINTER(C.c_double)) if st is not None else None,
+                               sd.ctypes.data_as(C.POINTER(C.c_double)) if sd is not None else None,
+                               counts.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return out, {"stats": st, "strat": sd, "counts": counts}
+
+    def trace_pixel(self, algorithm, gx, gy, fw, fh, samples_in_call, sample_offset, nsamples, max_depth=8):
+        out = np.zeros((nsamples, 3), dtype=np.float32)
+        self.L.orc_trace_pixel(self.h, algorithm.encode(), gx, gy, fw, fh, samples_in_call, sample_offset,
+                               nsamples, max_depth, _fptr(out))
+        return out
+
+    def primary_hits(self, tx, ty, tw, th, fw, fh, samples=1, sample_offset=0):
+        n = tw * th * samples
+        mesh = np.zeros(n, np.int32); tri = np.zeros(n, np.int32); pos = np.zeros(n, np.int32)
+        u = np.zeros(n, np.float32); v = np.zeros(n, np.float32); t = np.zeros(n, np.float32)
+        self.L.orc_primary_hits(self.h, tx, ty, tw, th, fw, fh, samples, sample_offset, _iptr(mesh), _iptr(tri),
+                                _iptr(pos), _fptr(u), _fptr(v), _fptr(t))
+        return {"mesh": mesh, "tri": tri, "pos": pos, "u": u, "v": v, "t": t}
+
+    def intersect(self, o, d, tmin, tmax):
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        n = o.shape[0]
+        tmin = np.ascontiguousarray(np.broadcast_to(np.float32(tmin), (n,)), np.float32) if np.isscalar(tmin) else np.ascontiguousarray(tmin, np.float32)
+        tmax = np.ascontiguousarray(np.broadcast_to(np.float32(tmax), (n,)), np.float32) if np.isscalar(tmax) else np.ascontiguousarray(tmax, np.float32)
+        mesh = np.zeros(n, np.int32); tri = np.zeros(n, np.int32); pos = np.zeros(n, np.int32)
+        u = np.zeros(n, np.float32); v = np.zeros(n, np.float32); t = np.zeros(n, np.float32)
+        self.L.orc_intersect(self.h, n, _fptr(o), _fptr(d), _fptr(tmin), _fptr(tmax), _iptr(mesh), _iptr(tri),
+                             _iptr(pos), _fptr(u), _fptr(v), _fptr(t))
+        return {"mesh": mesh, "tri": tri, "pos": pos, "u": u, "v": v, "t": t}
+
+    def mesh_bvh(self, mi):
+        """(boxes (n,6), links (n,4) = left,right,tri_start,tri_count, tri_idx)."""
+        n = C.c_int(0)
+        self.L.orc_mesh_bvh(self.h, mi, C.byref(n), None, None, None)
+        boxes = np.zeros((n.value, 6), np.float32)
+        links = np.zeros((n.value, 4), np.int32)
+        tri_idx = np.zeros(self._ntri[mi], np.int32)
+        self.L.orc_mesh_bvh(self.h, mi, C.byref(n), _fptr(boxes), _iptr(links), _iptr(tri_idx))
+        return boxes, links, tri_idx
+
+    def lights_info(self):
+        n = self.L.orc_num_lights(self.h)
+        em = np.zeros((n, 4), np.float32)
+        for i in range(n):
+            self.L.orc_light_emission(self.h, i, _fptr(em[i]))
+        cdf = np.array([self.L.orc_light_cdf(self.h, i) for i in range(n)], np.float32)
+        return {"n": n, "native_start": self.L.orc_native_light_start(self.h), "cdf": cdf,
+                "emission_area": em, "total_area": self.L.orc_light_total_area(self.h)}
+
+    def sample_light(self, i, p, u1, u2):
+        out = np.zeros(11, np.float32)
+        self.L.orc_sample_light(self.h, i, _fptr(np.asarray(p, np.float32)), u1, u2, _fptr(out))
+        return out
+
+    def select_light(self, u):
+        p = np.zeros(1, np.float32)
+        i = self.L.orc_select_light(self.h, u, _fptr(p))
+        return i, float(p[0])
+
+
+def flip_to_top_first(rgba_flat: np.ndarray, tw: int, th: int) -> np.ndarray:
+    """(th, tw, 4) image with row 0 = top, from a bottom-row-first buffer."""
+    return rgba_flat.reshape(th, tw, 4)[::-1]
