@@ -121,7 +121,8 @@ def test_env_only_scene_exact(lucid):
     scene = {"meshes": [], "environment": {"color": [0.5, 0.6, 0.7], "strength": 1.0}}
     r = make_renderer(lucid, scene, BOX_CAM, "mis")
     img = np.asarray(r.render_tile(0, 0, 16, 16, 16, 16, 3, 0, 8), np.float32).reshape(-1, 4)
-    np.testing.assert_array_equal(img[:, :3], np.float32([0.5, 0.6, 0.7]) * np.float32(3))
+    want = np.broadcast_to(np.float32([0.5, 0.6, 0.7]) * np.float32(3), img[:, :3].shape)
+    np.testing.assert_array_equal(img[:, :3], want)
 
 
 def test_cancel_before_render_is_reset(lucid):
