@@ -1,0 +1,200 @@
+"""Pins the oracle's BSDF, light and MIS arithmetic (bsdf/*.hpp, light/*.hpp,
+units/render_units.hpp:1393-1448).
+
+* Fresnel, GGX: reference known answers (test_bsdf.cpp) and closed forms.
+* GGX D is a normalised NDF: its projected integral over the hemisphere is 1
+  (checked by float64 quadrature for roughness where the reference's 1e-6
+  denominator guard is negligible).
+* sampleBSDF vs evalBSDF/pdfBSDF: the Monte Carlo estimate of the albedo
+  E[f cos / pdf] from sample_bsdf equals the quadrature of f cos over the
+  hemisphere -- a mismatch between the sampler and the evaluator fails it.
+* Lights: area pdf 2.25 sr^-1 (test_light.cpp:341-355), watt -> radiance
+  conversions (pybind_renderer.hpp:1593-1731), CDF selection, points sampled
+  on the emitter.
+* MIS: power heuristic formula and complementarity, bsdf_sample_weight 0.8
+  (test_integrator.cpp:87-138).
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+L = oracle.lib()
+
+
+def _f(a):
+    return (C.c_float * len(a))(*a)
+
+
+def test_fresnel_known_answers():
+    assert abs(L.orc_fresnel_dielectric(1.0, 1.5) - 0.04) < 1e-5
+    assert abs(L.orc_fresnel_dielectric(1.0, 1 / 1.5) - 0.04) < 1e-5
+    assert abs(L.orc_fresnel_dielectric(0.0, 1.5) - 1.0) < 1e-6           # grazing
+    assert L.orc_fresnel_dielectric(0.7, 1.0) < 1e-6                      # eta = 1
+    assert L.orc_fresnel_dielectric(0.3, 1.5) == 1.0                      # TIR (sin > 1/eta)
+    assert L.orc_fresnel_dielectric(0.9, 1.5) < 1.0
+    assert L.orc_fresnel_schlick(1.0, 0.04) == np.float32(0.04)
+    assert L.orc_fresnel_schlick(0.0, 0.04) == 1.0
+
+
+def test_ggx_closed_forms():
+    for r in (0.3, 0.6, 1.0):
+        a2 = (r * r) ** 2
+        peak = a2 / math.pi / (a2 * a2 + 1e-6)  # D(1) with the reference's 1e-6 guard (ggx.hpp:92)
+        assert abs(L.orc_ggx_d(1.0, r) - peak) / peak < 1e-5
+        assert abs(L.orc_ggx_g1(1.0, r) - 1.0) < 1e-6
+        assert L.orc_ggx_g1(0.0, r) == 0.0
+        nl, nv = 0.6, 0.8
+        ll = math.sqrt(a2 + (1 - a2) * nl * nl)
+        lv = math.sqrt(a2 + (1 - a2) * nv * nv)
+        assert abs(L.orc_ggx_g2(nl, nv, r) - 2 * nl * nv / (nv * ll + nl * lv)) < 1e-6
+        g2, g1l, g1v = L.orc_ggx_g2(nl, nv, r), L.orc_ggx_g1(nl, r), L.orc_ggx_g1(nv, r)
+        assert g1l * g1v - 1e-6 <= g2 <= min(g1l, g1v) + 1e-6        # test_bsdf.cpp:290-299
+        assert abs(g2 - L.orc_ggx_g2(nv, nl, r)) < 1e-6                 # symmetric (:283-288)
+
+
+@pytest.mark.parametrize("r", [0.5, 0.7, 1.0])
+def test_ggx_ndf_normalised(r):
+    n = 20000
+    mu = (np.arange(n) + 0.5) / n                      # cos(theta_h) on (0, 1)
+    D = np.array([L.orc_ggx_d(float(m), r) for m in mu])
+    integral = (D * mu).sum() / n * 2 * math.pi        # int D cos dw over the hemisphere
+    assert abs(integral - 1.0) < 2e-3
+
+
+def _mat(albedo=0.6, metallic=0.0, rough=0.5, trans=0.0, ior=1.45):
+    return _f([albedo, albedo, albedo, metallic, rough, trans, ior])
+
+
+def _eval(mat, wo, wi, n=(0, 0, 1)):
+    f = (C.c_float * 3)()
+    p = (C.c_float * 1)()
+    L.orc_eval_bsdf(mat, _f(wo), _f(wi), _f(n), f, p)
+    return np.array(f), p[0]
+
+
+@pytest.mark.parametrize("metallic,rough", [(0.0, 1.0), (0.0, 0.4), (1.0, 0.5), (0.5, 0.7)])
+def test_sampler_agrees_with_evaluator(metallic, rough):
+    mat = _mat(metallic=metallic, rough=rough)
+    th = 0.6
+    wo = (math.sin(th), 0.0, math.cos(th))
+    # quadrature of f * cos over the hemisphere
+    nt, nphi = 160, 160
+    q = 0.0
+    for i in range(nt):
+        ct = (i + 0.5) / nt
+        st = math.sqrt(1 - ct * ct)
+        for j in range(nphi):
+            ph = 2 * math.pi * (j + 0.5) / nphi
+            f, _ = _eval(mat, wo, (st * math.cos(ph), st * math.sin(ph), ct))
+            q += f[0] * ct
+    q *= (1.0 / nt) * (2 * math.pi / nphi)
+    # Monte Carlo with the sampler (evaluated weights only)
+    rng = np.random.default_rng(7)
+    out = (C.c_float * 9)()
+    est = []
+    for k in range(40000):
+        u = rng.random(3)
+        L.orc_sample_bsdf(mat, _f(wo), _f((0, 0, 1)), float(u[0]), float(u[1]), float(u[2]), 1, 2, k, out)
+        wi, kind, fw, pdf = np.array(out[0:3]), out[3], np.array(out[4:7]), out[7]
+        if kind == 1:
+            est.append(fw[0])
+        elif pdf > 1e-6 and wi[2] > 1e-6:
+            est.append(fw[0] * wi[2] / pdf)
+        else:
+            est.append(0.0)
+    est = np.array(est)
+    se = est.std() / math.sqrt(len(est))
+    assert abs(est.mean() - q) < 4 * se + 2e-3, (est.mean(), q, se)
+
+
+def test_pdf_matches_sample_histogram_diffuse():
+    mat = _mat(rough=1.0)
+    wo = (0.0, 0.0, 1.0)
+    rng = np.random.default_rng(1)
+    out = (C.c_float * 9)()
+    cts = []
+    for k in range(20000):
+        u = rng.random(3)
+        L.orc_sample_bsdf(mat, _f(wo), _f((0, 0, 1)), float(u[0]), float(u[1]), float(u[2]), 0, 0, k, out)
+        cts.append(out[2])
+    h, edges = np.histogram(cts, bins=10, range=(0, 1))
+    # expected mass per cos-bin from pdf_bsdf (azimuthally symmetric at normal view)
+    exp = []
+    for a, b in zip(edges[:-1], edges[1:]):
+        m = 0.0
+        for ct in np.linspace(a, b, 41)[:-1] + (b - a) / 80:
+            _, p = _eval(mat, wo, (math.sqrt(1 - ct * ct), 0.0, ct))
+            m += p * (b - a) / 40
+        exp.append(m * 2 * math.pi * len(cts))
+    exp = np.array(exp)
+    chi2 = ((h - exp) ** 2 / exp).sum()
+    assert chi2 < 30.0
+
+
+def test_mis_and_weight_known_answers():
+    pf, pg = 0.5, 2.0
+    w = L.orc_power_heuristic(pf, pg)
+    assert abs(w - pf * pf / (pf * pf + pg * pg)) < 1e-6
+    assert abs(w + L.orc_power_heuristic(pg, pf) - 1.0) < 1e-6
+    assert abs(L.orc_power_heuristic(0.5, 0.0) - 1.0) < 1e-3
+    out = (C.c_float * 3)()
+    L.orc_bsdf_weight(_f([0.5, 0.5, 0.5]), 0.8, 0.5, out)
+    assert np.allclose(list(out), [0.8, 0.8, 0.8], atol=1e-6)
+    L.orc_bsdf_weight(_f([0.5, 0.5, 0.5]), 0.8, 1e-7, out)
+    assert list(out) == [0.0, 0.0, 0.0]
+    assert abs(L.orc_pdf_w_from_area(3.0, 4.0, 1.0) - 2.25) < 1e-6
+    assert L.orc_pdf_w_from_area(3.0, 1e-7, 1.0) == 0.0
+
+
+def test_area_light_pdf_known_answer():
+    # test_light.cpp:341-355: d^2 / (A cos) = 9 / (4 * 1) = 2.25
+    p = L.orc_pdf_light_area(4.0, _f((0, 0, 0)), _f((0, 0, 3)), _f((0, 0, -1)))
+    assert abs(p - 2.25) < 1e-6
+
+
+def test_native_light_conversions_and_sampling():
+    lights = [
+        {"type": "POINT", "position": [0, 3, 0], "energy": 100.0},
+        {"type": "POINT", "position": [2, 3, 0], "energy": 100.0, "radius": 0.1},
+        {"type": "SUN", "direction": [0, -1, 0], "energy": 3.0, "color": [1, 0.5, 0.25]},
+        {"type": "SPOT", "position": [0, 2, 0], "direction": [0, -1, 0], "energy": 50.0, "spot_size": 1.0},
+        {"type": "AREA", "position": [0, 2, 0], "direction": [0, -1, 0], "size": 0.5, "size_y": 0.25,
+         "shape": "RECTANGLE", "energy": 10.0, "right": [1, 0, 0], "up": [0, 0, 1]},
+        {"type": "AREA", "position": [0, 2, 0], "direction": [0, -1, 0], "size": 0.5, "shape": "DISK",
+         "energy": 10.0},
+    ]
+    o = oracle.OracleScene({"meshes": [], "native_lights": lights})
+    info = o.lights_info()
+    em = info["emission_area"]
+    assert info["n"] == 6 and info["native_start"] == 0
+    np.testing.assert_allclose(em[0, 0], 100 / (4 * math.pi), rtol=1e-6)
+    np.testing.assert_allclose(em[1, 0], 100 / (math.pi * 4 * math.pi * 0.01), rtol=1e-5)
+    np.testing.assert_allclose(em[2, :3], [3.0, 1.5, 0.75], rtol=1e-6)
+    np.testing.assert_allclose(em[3, 0], 50 / (2 * math.pi * (1 - math.cos(0.5))), rtol=1e-5)
+    np.testing.assert_allclose(em[4, 0], 10 / (math.pi * 0.125), rtol=1e-6)
+    np.testing.assert_allclose(em[5, 3], math.pi * 0.0625, rtol=1e-6)
+    # areas: point(1 if delta), sphere 4 pi r^2, sun 1, spot 1, rect, disk
+    np.testing.assert_allclose(em[:, 3], [1, 4 * math.pi * 0.01, 1, 1, 0.125, math.pi * 0.0625], rtol=1e-5)
+    cdf = info["cdf"]
+    assert np.all(np.diff(cdf) > 0) and abs(cdf[-1] - 1) < 1e-6
+    i, p = o.select_light(float(cdf[2]) - 1e-4)
+    assert i == 2 and abs(p - em[2, 3] / em[:, 3].sum()) < 1e-6
+    # rectangle samples stay on the emitter; distance/pdf consistent
+    for u1, u2 in [(0.0, 0.0), (0.99, 0.5), (0.3, 0.9)]:
+        s = o.sample_light(4, [0, 0, 0], u1, u2)
+        assert abs(s[0]) <= 0.25 + 1e-6 and abs(s[2]) <= 0.125 + 1e-6 and abs(s[1] - 2) < 1e-6
+        d = math.dist(s[:3], [0, 0, 0])
+        assert abs(s[6] - d) < 1e-5
+        cos_l = s[1] / d
+        assert abs(s[7] - d * d / (0.125 * cos_l)) / s[7] < 1e-5
+    # spot: outside the cone -> no emission
+    s = o.sample_light(3, [5, 0, 0], 0.5, 0.5)
+    assert s[8] == 0.0
+    # delta point light: emission / d^2, pdf 1
+    s = o.sample_light(0, [0, 0, 0], 0.5, 0.5)
+    assert abs(s[8] - em[0, 0] / 9) < 1e-6 and s[7] == 1.0
