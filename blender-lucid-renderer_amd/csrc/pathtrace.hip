@@ -37,7 +37,7 @@
 namespace lucid {
 
 constexpr int kBlock = 128;      // 2 wavefronts per workgroup
-constexpr int kStack = 48;       // LDS traversal stack entries per lane
+constexpr int kStackMax = 64;    // LDS traversal stack entries per lane, upper bound
 
 // ------------------------------------------------------------- device scene --
 struct SceneDev {
@@ -153,124 +153,85 @@ struct TraceHit {
     int32_t prim;
 };
 
-// Closest hit over the whole scene.  Ties in t resolve to the larger
-// BVH-order position = the triangle the reference tests last.
-__device__ __noinline__ TraceHit trace_closest(const SceneDev& S, v3f o, v3f d, float tmin, float tmax,
-                                               int32_t* stk, int32_t* ovf, uint32_t& nn, uint32_t& nt) {
-    TraceHit h;
+// Slab test of one child box given as scalars (no local arrays, so nothing
+// is spilled to scratch).
+__device__ __forceinline__ bool box_hit(v3f o, v3f d, v3f id, float x0, float y0, float z0, float x1, float y1,
+                                        float z1, float& tentry) {
+    float tmin = 0.0f, tmax = RAY_T_MAX;
+    if (!slab1(d.x, id.x, x0 - o.x, x1 - o.x, tmin, tmax)) return false;
+    if (!slab1(d.y, id.y, y0 - o.y, y1 - o.y, tmin, tmax)) return false;
+    if (!slab1(d.z, id.z, z0 - o.z, z1 - o.z, tmin, tmax)) return false;
+    tentry = tmin;
+    return tmax >= smax(tmin, 0.0f);
+}
+
+// BVH traversal over the whole scene (top-level tree over mesh roots, then
+// the per-mesh trees).  ANY = false: closest hit, ties in t resolve to the
+// larger BVH-order position = the triangle the reference tests last.
+// ANY = true: returns at the first triangle with tmin <= t <= tmax (the
+// reference's closest-hit shadow query reports a hit exactly when one
+// exists).  stk: this lane's LDS stack (stride kBlock), depth `sd`.
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk,
+                                         int sd, int32_t* ovf, uint32_t& nn, uint32_t& nt, TraceHit& h) {
     h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-    v3f id = inv_dir(d);
+    const v3f id = inv_dir(d);
     int sp = 0;
     int32_t node = S.root;
     for (;;) {
-        const GNode* n = S.nodes + node;
+        const float4* n4 = reinterpret_cast<const float4*>(S.nodes + node);
+        const float4 b0 = n4[0], b1 = n4[1], b2 = n4[2];
+        const int2 cc = *reinterpret_cast<const int2*>(n4 + 3);
         ++nn;
-        float4 b0 = ((const float4*)n->b)[0];
-        float4 b1 = ((const float4*)n->b)[1];
-        float4 b2 = ((const float4*)n->b)[2];
-        int2 ch = *(const int2*)n->child;
-        float bx0[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
-        float bx1[6] = {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
+        const float lim = prune_bound(h.t);
         float te0 = 0.0f, te1 = 0.0f;
-        float lim = prune_bound(h.t);
-        bool h0 = ch.x != kEmptyChild && aabb(o, d, id, bx0, te0) && te0 <= lim;
-        bool h1 = ch.y != kEmptyChild && aabb(o, d, id, bx1, te1) && te1 <= lim;
-        int32_t c[2] = {ch.x, ch.y};
-        bool hh[2] = {h0, h1};
-        int first = (h1 && (!h0 || te1 < te0)) ? 1 : 0;
+        bool h0 = cc.x != kEmptyChild && box_hit(o, d, id, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, te0) && te0 <= lim;
+        bool h1 = cc.y != kEmptyChild && box_hit(o, d, id, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w, te1) && te1 <= lim;
+        const bool sw = h1 && (!h0 || te1 < te0);
+        int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
+        bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
+        const float tf = sw ? te0 : te1;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            int j = k == 0 ? first : 1 - first;
-            if (hh[j] && c[j] < 0) {
-                int32_t enc = ~c[j];
-                int32_t start = enc >> 3, count = enc & 7;
+            const int32_t c = k == 0 ? cn : cf;
+            const bool hit = k == 0 ? hn : (hf && tf <= prune_bound(h.t));
+            if (hit && c < 0) {
+                const int32_t enc = ~c;
+                const int32_t start = enc >> 3, count = enc & 7;
                 nt += count;
                 for (int i = 0; i < count; ++i) {
-                    int32_t prim = start + i;
+                    const int32_t prim = start + i;
                     float t, u, v;
                     if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
+                        if (ANY) {
+                            h.t = t; h.u = u; h.v = v; h.prim = prim;
+                            return true;
+                        }
                         if (t < h.t || prim > h.prim) {
                             h.t = t; h.u = u; h.v = v; h.prim = prim;
                         }
                     }
                 }
-                hh[j] = false;
+                if (k == 0) hn = false; else hf = false;
             }
         }
-        if (hh[0] && hh[1]) {
-            int32_t nearc = c[first], farc = c[1 - first];
-            if (sp < kStack) {
-                stk[sp * kBlock] = farc;
+        if (hn && hf) {
+            if (sp < sd) {
+                stk[sp * kBlock] = cf;
                 ++sp;
             } else {
                 atomicAdd(ovf, 1);
             }
-            node = nearc;
+            node = cn;
             continue;
         }
-        if (hh[0]) { node = c[0]; continue; }
-        if (hh[1]) { node = c[1]; continue; }
+        if (hn) { node = cn; continue; }
+        if (hf) { node = cf; continue; }
         if (sp == 0) break;
         --sp;
         node = stk[sp * kBlock];
     }
-    return h;
-}
-
-// Occlusion: any triangle with tmin <= t <= tmax (the reference's closest-hit
-// shadow query reports a hit exactly when one exists).
-__device__ __noinline__ bool trace_any(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk,
-                                       int32_t* ovf, uint32_t& nn, uint32_t& nt) {
-    v3f id = inv_dir(d);
-    int sp = 0;
-    int32_t node = S.root;
-    float lim = prune_bound(tmax);
-    for (;;) {
-        const GNode* n = S.nodes + node;
-        ++nn;
-        float4 b0 = ((const float4*)n->b)[0];
-        float4 b1 = ((const float4*)n->b)[1];
-        float4 b2 = ((const float4*)n->b)[2];
-        int2 ch = *(const int2*)n->child;
-        float bx0[6] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y};
-        float bx1[6] = {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w};
-        float te0 = 0.0f, te1 = 0.0f;
-        bool h0 = ch.x != kEmptyChild && aabb(o, d, id, bx0, te0) && te0 <= lim;
-        bool h1 = ch.y != kEmptyChild && aabb(o, d, id, bx1, te1) && te1 <= lim;
-        int32_t c[2] = {ch.x, ch.y};
-        bool hh[2] = {h0, h1};
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (hh[j] && c[j] < 0) {
-                int32_t enc = ~c[j];
-                int32_t start = enc >> 3, count = enc & 7;
-                for (int i = 0; i < count; ++i) {
-                    int32_t prim = start + i;
-                    float t, u, v;
-                    ++nt;
-                    if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, tmax, t, u, v)) return true;
-                }
-                hh[j] = false;
-            }
-        }
-        if (hh[0] && hh[1]) {
-            int first = te1 < te0 ? 1 : 0;
-            if (sp < kStack) {
-                stk[sp * kBlock] = c[1 - first];
-                ++sp;
-            } else {
-                atomicAdd(ovf, 1);
-            }
-            node = c[first];
-            continue;
-        }
-        if (hh[0]) { node = c[0]; continue; }
-        if (hh[1]) { node = c[1]; continue; }
-        if (sp == 0) break;
-        --sp;
-        node = stk[sp * kBlock];
-    }
-    return false;
+    return h.prim >= 0;
 }
 
 __device__ __forceinline__ int32_t prim_mesh(const SceneDev& S, int32_t prim) { return __float_as_int(S.v0[prim].w); }
@@ -278,29 +239,31 @@ __device__ __forceinline__ int32_t prim_mesh(const SceneDev& S, int32_t prim) { 
 // intersect_skipping_volume_boundaries (path_tracer.hpp:82-95): the closest
 // real surface; boundary-only meshes are skipped by re-querying past them.
 __device__ __forceinline__ TraceHit closest_real(const SceneDev& S, v3f o, v3f d, float tmin, int32_t* stk,
-                                                 int32_t* ovf, uint32_t& nn, uint32_t& nt) {
-    TraceHit h = trace_closest(S, o, d, tmin, RAY_T_MAX, stk, ovf, nn, nt);
+                                                 int sd, int32_t* ovf, uint32_t& nn, uint32_t& nt) {
+    TraceHit h;
+    traverse<false>(S, o, d, tmin, RAY_T_MAX, stk, sd, ovf, nn, nt, h);
     if (!S.any_vbo) return h;
     float smin_ = tmin;
     for (int peek = 0; peek < 8; ++peek) {
         if (h.prim < 0) return h;
         if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return h;
         smin_ = h.t + 1e-3f;
-        h = trace_closest(S, o, d, smin_, RAY_T_MAX, stk, ovf, nn, nt);
+        traverse<false>(S, o, d, smin_, RAY_T_MAX, stk, sd, ovf, nn, nt, h);
     }
     h.prim = -1;
     return h;
 }
 // is_shadowed_skipping_volume_boundaries (path_tracer.hpp:103-119)
-__device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float tmax, int32_t* stk, int32_t* ovf,
-                                         uint32_t& nn, uint32_t& nt) {
-    if (!S.any_vbo) return trace_any(S, o, d, RAY_T_MIN, tmax, stk, ovf, nn, nt);
+__device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float tmax, int32_t* stk, int sd,
+                                         int32_t* ovf, uint32_t& nn, uint32_t& nt) {
+    TraceHit h;
+    if (!S.any_vbo) return traverse<true>(S, o, d, RAY_T_MIN, tmax, stk, sd, ovf, nn, nt, h);
     float toff = 0.0f;
     v3f po = o;
     for (int peek = 0; peek < 8; ++peek) {
         float remaining = tmax - toff;
         if (remaining <= 0.0f) return false;
-        TraceHit h = trace_closest(S, po, d, RAY_T_MIN, remaining, stk, ovf, nn, nt);
+        traverse<false>(S, po, d, RAY_T_MIN, remaining, stk, sd, ovf, nn, nt, h);
         if (h.prim < 0) return false;
         if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return true;
         float skip = h.t + 1e-3f;
@@ -348,22 +311,22 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
 }
 
 // ------------------------------------------------------------- intersect --
-__global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int qin) {
-    __shared__ int32_t stack[kStack * kBlock];
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    int n = W.cnt[qin];
-    if (i == 0) {
+__global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int qin, int sd) {
+    extern __shared__ int32_t lds_stack[];
+    const int n = W.cnt[qin];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         W.cnt[1 - qin] = 0;
         W.cnt[2] = 0;
         atomicAdd(&W.rays[0], (unsigned long long)n);
     }
-    if (i >= n) return;
-    int p = W.q[qin][i];
-    float4 o = W.ray_o[p], d = W.ray_d[p];
     uint32_t nn = 0, nt = 0;
-    TraceHit h = closest_real(S, mk(o), mk(d), o.w, stack + threadIdx.x, &W.cnt[3], nn, nt);
-    W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
-    if (W.count_work) {
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const int p = W.q[qin][i];
+        const float4 o = W.ray_o[p], d = W.ray_d[p];
+        TraceHit h = closest_real(S, mk(o), mk(d), o.w, lds_stack + threadIdx.x, sd, &W.cnt[3], nn, nt);
+        W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+    }
+    if (W.count_work && (nn | nt)) {
         atomicAdd(&W.rays[2], (unsigned long long)nn);
         atomicAdd(&W.rays[3], (unsigned long long)nt);
     }
@@ -448,9 +411,8 @@ __device__ __forceinline__ bool finite3(v3f a) { return isfinite(a.x) && isfinit
 
 // One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570).
 template <int ALGO>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, int qin, int32_t max_depth) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    int n = W.cnt[qin];
+__device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, int qin, int32_t max_depth, int i,
+                                          int n) {
     bool live = i < n;
     bool cont = false, nee = false;
     int p = live ? W.q[qin][i] : 0;
@@ -601,21 +563,30 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, int qin
     if (nee) W.qs[sslot] = p;
 }
 
+// Grid-stride over the live queue; the trip count is uniform per workgroup so
+// every lane reaches the wavefront ballots of shade_one.
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, int qin, int32_t max_depth) {
+    const int n = W.cnt[qin];
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
+        shade_one<ALGO>(S, W, qin, max_depth, base + (int)threadIdx.x, n);
+}
+
 // ---------------------------------------------------------------- shadow --
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W) {
-    __shared__ int32_t stack[kStack * kBlock];
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    int n = W.cnt[2];
-    if (i == 0) atomicAdd(&W.rays[1], (unsigned long long)n);
-    if (i >= n) return;
-    int p = W.qs[i];
-    float4 o = W.sh_o[p], d = W.sh_d[p];
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W, int sd) {
+    extern __shared__ int32_t lds_stack[];
+    const int n = W.cnt[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&W.rays[1], (unsigned long long)n);
     uint32_t nn = 0, nt = 0;
-    if (!shadowed(S, mk(o), mk(d), o.w, stack + threadIdx.x, &W.cnt[3], nn, nt)) {
-        float4 L = W.rad[p], c = W.sh_c[p];
-        W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const int p = W.qs[i];
+        const float4 o = W.sh_o[p], d = W.sh_d[p];
+        if (!shadowed(S, mk(o), mk(d), o.w, lds_stack + threadIdx.x, sd, &W.cnt[3], nn, nt)) {
+            const float4 L = W.rad[p], c = W.sh_c[p];
+            W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+        }
     }
-    if (W.count_work) {
+    if (W.count_work && (nn | nt)) {
         atomicAdd(&W.rays[4], (unsigned long long)nn);
         atomicAdd(&W.rays[5], (unsigned long long)nt);
     }
@@ -697,6 +668,7 @@ struct lucid_scene {
 
 struct lucid_ctx {
     int device = 0;
+    int num_cu = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t wave_ev[2] = {nullptr, nullptr};  // pacing: at most two waves in flight
@@ -846,6 +818,11 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     if (rc) return rc;
     const SceneDev& SD = scene->dev;
     WaveDev W = ctx->W;
+    // LDS stack sized to the scene's BVH depth; persistent grids of a few
+    // workgroups per CU walk the live queues (grid-stride).
+    const int sd = std::min(kStackMax, scene->host.max_depth + 2);
+    const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
+    const int64_t grid_cap = (int64_t)ctx->num_cu * 8;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     ctx->kev_used = 0;
     k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
@@ -887,24 +864,25 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         int q = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
             int ks = kev_begin(ctx, 0, st);
-            k_intersect<<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q);
+            const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), grid_cap);
+            k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
             ks = kev_begin(ctx, 1, st);
             switch (algo) {
                 case LUCID_ALGO_SIMPLE:
-                    k_shade<LUCID_ALGO_SIMPLE><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_SIMPLE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 case LUCID_ALGO_MIS:
-                    k_shade<LUCID_ALGO_MIS><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_MIS><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 default:
-                    k_shade<LUCID_ALGO_NEE><<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_NEE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
             }
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
-            k_shadow<<<blocks(n, kBlock), kBlock, 0, st>>>(SD, W);
+            k_shadow<<<g, kBlock, lds, st>>>(SD, W, sd);
             kev_end(ctx, ks, st);
             q = 1 - q;
         }
@@ -970,6 +948,9 @@ int lucid_create(int32_t device, lucid_ctx** out) {
     lucid_ctx* ctx = new (std::nothrow) lucid_ctx();
     if (!ctx) return LUCID_ERR_NOMEM;
     ctx->device = device;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        ctx->num_cu = ncu;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->wave_ev[0], hipEventDisableTiming) != hipSuccess ||
@@ -1015,7 +996,7 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
         return fail(ctx, LUCID_ERR_INVALID, err);
     }
     const PackedScene& ps = s->host;
-    if (ps.max_depth + 2 > kStack) {
+    if (ps.max_depth + 2 > kStackMax) {
         delete s;
         return fail(ctx, LUCID_ERR_INVALID, "BVH deeper than the traversal stack");
     }
