@@ -273,6 +273,83 @@ __device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float 
     return false;
 }
 
+
+// ------------------------------------------------------------------ media --
+// transmittance_along_segment (absorption, transmittance.hpp:77-128) and
+// extinction_transmittance_along_segment (:145-209): midpoint ray march at
+// half a voxel over every gridded medium, T = exp(-tau) per channel.
+__device__ __noinline__ v3f vol_transmittance(const SceneDev& S, v3f o, v3f d, float tmax, bool extinction) {
+    float tr = 0.0f, tg = 0.0f, tb = 0.0f;
+    for (int vi = 0; vi < S.n_vols; ++vi) {
+        const GVolume& v = S.vols[vi];
+        if (!v.has_grid) continue;
+        float tin = 0.0f, tout = 0.0f;
+        if (!ray_aabb_segment(o, d, tmax, v.wmin, v.wmax, tin, tout)) continue;
+        const float seg = tout - tin;
+        if (seg <= 0.0f) continue;
+        const float ext = max3(v.wmax[0] - v.wmin[0], v.wmax[1] - v.wmin[1], v.wmax[2] - v.wmin[2]);
+        const int md = max(max(v.dims[0], v.dims[1]), v.dims[2]);
+        const float vox = ext / (float)max(1, md);
+        const float step = smax(1e-4f, vox * 0.5f);
+        const int n = min(2048, (int)(seg / step) + 1);
+        const float er = 1.0f + v.absorption[0], eg = 1.0f + v.absorption[1], eb = 1.0f + v.absorption[2];
+        for (int i = 0; i < n; ++i) {
+            const float t = tin + ((float)i + 0.5f) * step;
+            if (t > tout) break;
+            const float dd = grid_trilinear(v, S.grid, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+            if (dd <= 0.0f) continue;
+            const float base = v.density * dd * step;
+            if (extinction) {
+                tr += base * er; tg += base * eg; tb += base * eb;
+            } else {
+                tr += base * (1.0f - v.color[0]); tg += base * (1.0f - v.color[1]); tb += base * (1.0f - v.color[2]);
+            }
+        }
+    }
+    return mk(expf(-tr), expf(-tg), expf(-tb));
+}
+
+// sample_volume_distance (transmittance.hpp:220-289): Woodcock / delta
+// tracking against the max-density majorant of each gridded medium, in mesh
+// order, at most 4096 steps; draws from the path's PCG stream in the
+// reference's order.  Returns the medium index of the nearest real collision.
+__device__ __noinline__ int vol_sample_distance(const SceneDev& S, v3f o, v3f d, float tmax, Pcg& rng, float& t_out) {
+    float best = tmax;
+    int hit = -1;
+    for (int vi = 0; vi < S.n_vols; ++vi) {
+        const GVolume& v = S.vols[vi];
+        if (!v.has_grid) continue;
+        if (v.density <= 0.0f || v.grid_max <= 0.0f) continue;
+        float tin = 0.0f, tout = 0.0f;
+        if (!ray_aabb_segment(o, d, tmax, v.wmin, v.wmax, tin, tout)) continue;
+        if (tin >= best) continue;
+        const float seg_end = smin(tout, best);
+        if (seg_end <= tin) continue;
+        const float emax = max3(1.0f + v.absorption[0], 1.0f + v.absorption[1], 1.0f + v.absorption[2]);
+        const float sig = v.density * v.grid_max * emax;
+        if (sig <= 0.0f) continue;
+        float t = tin;
+        bool sc = false;
+        for (int it = 0; it < 4096; ++it) {
+            const float us = smax(1e-12f, 1.0f - randf(rng));
+            t += -logf(us) / sig;
+            if (t >= seg_end) break;
+            const float dd = grid_trilinear(v, S.grid, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+            const float st = v.density * dd * emax;
+            if (randf(rng) * sig < st) {
+                sc = true;
+                break;
+            }
+        }
+        if (sc && t < best) {
+            best = t;
+            hit = vi;
+        }
+    }
+    t_out = best;
+    return hit;
+}
+
 // ---------------------------------------------------------------- raygen --
 // Camera ray of render_tile (pybind_renderer.hpp:253-293).
 __device__ __forceinline__ v3f cam_dir(const CamDev& c, float nx, float ny) {
@@ -434,9 +511,74 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
         lh.hit = false;
         if (depth > 0 && S.n_native > 0) lh = native_lights(S, o, d);
         bool done = false;
-        if (lh.hit && (!has_hit || lh.t < hv.x)) {
+        constexpr bool VOLUME = ALGO == LUCID_ALGO_VOLUME_MIS || ALGO == LUCID_ALGO_VOLUME_SIMPLE;
+        if (VOLUME) {
+            float ts = 1e6f;
+            if (has_hit) ts = smin(ts, hv.x);
+            if (lh.hit) ts = smin(ts, lh.t);
+            if (ALGO == LUCID_ALGO_VOLUME_SIMPLE) {
+                // traceVolumeSimple (path_tracer.hpp:226-230): absorption only
+                if (S.n_vols > 0) thr = cmul(thr, vol_transmittance(S, o, d, ts, false));
+            } else if (S.n_vols > 0) {
+                float vt;
+                const int vi = vol_sample_distance(S, o, d, ts, rng, vt);
+                if (vi >= 0) {
+                    // scatter event (path_tracer.hpp:617-699)
+                    done = true;
+                    const GVolume& vm = S.vols[vi];
+                    thr = cmul(thr, mk(vm.color[0] / (1.0f + vm.absorption[0]), vm.color[1] / (1.0f + vm.absorption[1]),
+                                       vm.color[2] / (1.0f + vm.absorption[2])));
+                    const v3f x = o + d * vt;
+                    const v3f wo = -d;
+                    const float g = vm.anisotropy;
+                    if (S.n_lights > 0) {
+                        float sp;
+                        const int li = select_light(S, randf(rng), sp);
+                        if (sp > 1e-6f) {
+                            const float a1 = randf(rng), a2 = randf(rng);
+                            const LSample ls = sample_light(S.lights[li], x, a1, a2);
+                            if (ls.pdf > MIN_PDF) {
+                                const v3f T = vol_transmittance(S, x, ls.dir, ls.dist, true);
+                                const float ct = dot(wo, ls.dir);
+                                const float pe = hg_eval(g, ct);
+                                const float pp = hg_eval(g, ct);
+                                const float pl = ls.pdf * sp;
+                                const float mw = power_heuristic(pl, pp);
+                                const v3f pw = bsdf_weight(mk(pe, pe, pe), 1.0f, pl);
+                                const v3f cc = cmul(cmul(pw * mw, cmul(ls.emission, T)), thr);
+                                W.sh_o[p] = make_float4(x.x, x.y, x.z, ls.dist);
+                                W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                                W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                                nee = true;
+                            }
+                        }
+                    }
+                    const float a1 = randf(rng), a2 = randf(rng);
+                    float ppdf;
+                    const v3f wi = hg_sample(g, wo, a1, a2, ppdf);
+                    last_pdf = ppdf;
+                    bool ok = true;
+                    if (depth >= 3) {
+                        const float mx = max3(thr.x, thr.y, thr.z);
+                        const float rr = smin(mx, 0.95f);
+                        if (randf(rng) > rr) ok = false;
+                        else thr = thr * (1.0f / rr);
+                    }
+                    if (ok && finite3(thr) && depth + 1 < max_depth) {
+                        cont = true;
+                        W.ray_o[p] = make_float4(x.x, x.y, x.z, RAY_T_MIN);
+                        W.ray_d[p] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+                        W.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+                        W.rng[p] = make_ulonglong2(rng.state, rng.inc);
+                        W.meta[p] = make_int4(meta.x, meta.y, depth + 1, 0);
+                    }
+                }
+            }
+        }
+        if (done) {
+        } else if (lh.hit && (!has_hit || lh.t < hv.x)) {
             v3f c = cmul(lh.emission, thr);
-            if (ALGO == LUCID_ALGO_MIS) {
+            if (ALGO == LUCID_ALGO_MIS || ALGO == LUCID_ALGO_VOLUME_MIS) {
                 float lp = 0.0f;
                 if (S.n_lights > 0 && lh.index >= 0) {
                     int sli = S.native_ids[lh.index];
@@ -475,11 +617,11 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             bool front = dot(wo, nrm) > 0.0f;
             v3f sn = (mat.transmission < 0.5f && !front) ? -nrm : nrm;
             bool has_em = (gm.flags & MF_EMISSIVE) != 0;
-            if (ALGO == LUCID_ALGO_SIMPLE) {
+            if (ALGO == LUCID_ALGO_SIMPLE || ALGO == LUCID_ALGO_VOLUME_SIMPLE) {
                 L = L + cmul(em, thr);
             } else if (ALGO == LUCID_ALGO_NEE) {
                 if (has_em && depth == 0) L = L + cmul(em, thr);
-            } else if (has_em) {
+            } else if (has_em) {  // traceMIS / traceVolumeMIS
                 float mw = 1.0f;
                 if (S.n_lights > 0 && last_pdf > MIN_PDF) {
                     int32_t li = __float_as_int(S.e2[prim].w);
@@ -494,7 +636,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                 L = L + cmul(em, thr) * mw;
             }
             // next event estimation
-            if (ALGO != LUCID_ALGO_SIMPLE && S.n_lights > 0 && !has_em && !(mat.transmission > 0.5f)) {
+            if (ALGO != LUCID_ALGO_SIMPLE && ALGO != LUCID_ALGO_VOLUME_SIMPLE && S.n_lights > 0 && !has_em &&
+                !(mat.transmission > 0.5f)) {
                 float sp;
                 int li = select_light(S, randf(rng), sp);
                 if (sp > 1e-6f) {
@@ -510,6 +653,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
+                            } else if (ALGO == LUCID_ALGO_VOLUME_MIS) {
+                                // surface NEE through media (path_tracer.hpp:783-797)
+                                const v3f T = S.n_vols > 0 ? vol_transmittance(S, point, ls.dir, ls.dist, true)
+                                                           : mk(1.0f, 1.0f, 1.0f);
+                                float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                                float mw = power_heuristic(pl, pb);
+                                cc = cmul(bsdf_weight(f, nl, pl) * mw, cmul(ls.emission, T));
                             } else {
                                 cc = cmul(bsdf_weight(f, nl, pl), ls.emission);
                             }
@@ -537,7 +687,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             }
             if (ok) {
                 thr = cmul(thr, w);
-                if (ALGO == LUCID_ALGO_MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
+                if (ALGO == LUCID_ALGO_MIS || ALGO == LUCID_ALGO_VOLUME_MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
                 if (depth >= 3) {
                     float mx = max3(thr.x, thr.y, thr.z);
                     float rr = smin(mx, 0.95f);
@@ -843,11 +993,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     // traceMIS on a scene without participating media (no distance-sampling
     // draws, unit transmittance).
     int algo = pr->algorithm;
-    if (algo == LUCID_ALGO_VOLUME_SIMPLE || algo == LUCID_ALGO_VOLUME_MIS) {
-        if (scene->host.any_volume)
-            return fail(ctx, LUCID_ERR_INVALID, "volume integrators are not available in this build");
+    if ((algo == LUCID_ALGO_VOLUME_SIMPLE || algo == LUCID_ALGO_VOLUME_MIS) && !scene->host.any_volume)
         algo = algo == LUCID_ALGO_VOLUME_SIMPLE ? LUCID_ALGO_SIMPLE : LUCID_ALGO_MIS;
-    }
     for (int s0 = 0; s0 < S; s0 += (int)B) {
         // Pacing: wait for the wave before the previous one, so at most two
         // waves are queued and a cancel takes effect within one wave.
@@ -875,6 +1022,12 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
                     break;
                 case LUCID_ALGO_MIS:
                     k_shade<LUCID_ALGO_MIS><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                case LUCID_ALGO_VOLUME_SIMPLE:
+                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                case LUCID_ALGO_VOLUME_MIS:
+                    k_shade<LUCID_ALGO_VOLUME_MIS><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 default:
                     k_shade<LUCID_ALGO_NEE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);

@@ -49,24 +49,36 @@ struct HostScene {
 float f(const Value& v) { return (float)lucid::json::num(v); }
 float f_at(const Value& v, size_t i) { return (float)lucid::json::at(v, i); }
 
-// Material Output "Surface" socket linked? (pybind_renderer.hpp:1386-1398).
-// Node trees are otherwise not evaluated.
+// Material Output "Surface" socket linked? (pybind_renderer.hpp:1386-1398
+// with parseNodeTree, node_evaluator.cpp:82-149: a socket is linked when a
+// "links" entry targets (node name, socket name); a tree without nodes is
+// invalid and leaves the surface linked).  Node trees are otherwise not
+// evaluated.
 bool surface_linked(const Value& mat) {
     const Value* nt = mat.get("node_tree");
-    if (!nt || nt->is_null() || !nt->is_object()) return true;
+    if (!nt || !nt->is_object()) return true;
     const Value* nodes = nt->get("nodes");
-    if (!nodes || nodes->kind != Value::Arr) return true;
+    if (!nodes || nodes->kind != Value::Arr || nodes->arr.empty()) return true;
+    const Value* links = nt->get("links");
     for (const Value& n : nodes->arr) {
         const Value* ty = n.get("type");
         if (!ty || ty->kind != Value::Str || ty->s != "ShaderNodeOutputMaterial") continue;
+        std::string name;
+        if (const Value* nm = n.get("name"); nm && nm->kind == Value::Str) name = nm->s;
         const Value* ins = n.get("inputs");
         if (ins && ins->kind == Value::Arr) {
-            for (const Value& s : ins->arr) {
-                const Value* nm = s.get("name");
-                if (nm && nm->kind == Value::Str && nm->s == "Surface") {
-                    const Value* l = s.get("is_linked");
-                    return l && l->kind == Value::Bool && l->b;
+            for (const Value& sock : ins->arr) {
+                const Value* sn = sock.get("name");
+                if (!(sn && sn->kind == Value::Str && sn->s == "Surface")) continue;
+                if (!links || links->kind != Value::Arr) return false;
+                for (const Value& l : links->arr) {
+                    const Value* tn = l.get("to_node");
+                    const Value* ts = l.get("to_socket");
+                    if (tn && ts && tn->kind == Value::Str && ts->kind == Value::Str && tn->s == name &&
+                        ts->s == "Surface")
+                        return true;
                 }
+                return false;
             }
         }
         return false;

@@ -117,6 +117,25 @@ def _f3(v, default=(0.0, 0.0, 0.0)):
     return F3(*(float(x) for x in v))
 
 
+def surface_linked(material: dict) -> bool:
+    """Material Output "Surface" socket linked? (pybind_renderer.hpp:1386-1398
+    with parseNodeTree, node_evaluator.cpp:82-149: a socket is linked when a
+    "links" entry targets (node name, socket name); a tree without nodes is
+    invalid and leaves the surface linked)."""
+    nt = material.get("node_tree")
+    if not isinstance(nt, dict) or not isinstance(nt.get("nodes"), list) or not nt["nodes"]:
+        return True
+    links = nt.get("links") or []
+    for n in nt["nodes"]:
+        if n.get("type", "") == "ShaderNodeOutputMaterial":
+            name = n.get("name", "")
+            for s in n.get("inputs", []) or []:
+                if s.get("name", "") == "Surface":
+                    return any(l.get("to_node") == name and l.get("to_socket") == "Surface" for l in links)
+            return False
+    return True
+
+
 class SceneDescs:
     """POD descriptors of a reference-schema scene dict, owning the arrays."""
 
@@ -145,7 +164,7 @@ class SceneDescs:
             d.emission = _f3(mat.get("emission"))
             d.transmission = float(mat.get("transmission", 0.0))
             d.ior = float(mat.get("ior", 1.45))
-            d.surface_linked = 1
+            d.surface_linked = 1 if surface_linked(mat) else 0
             vol = mat.get("volume")
             if isinstance(vol, dict):
                 d.has_volume = 1

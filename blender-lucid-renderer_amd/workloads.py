@@ -177,6 +177,68 @@ def icospheres(seed: int = 2, count: int = 200, level: int = 4, smooth: bool = T
 ICOSPHERES_CAMERA = {"pos": (0.0, 6.0, 20.0), "dir": (0.0, -0.25, -1.0), "up": (0.0, 1.0, 0.0), "fov": 55.0}
 
 
+def fbm_grid(n: int, seed: int = 3, octaves: int = 4) -> np.ndarray:
+    """Seeded fractal value noise on an n^3 grid (x fastest in memory), shaped
+    into a soft blob, values in [0, 1]."""
+    from scipy.ndimage import zoom
+
+    rng = np.random.default_rng(seed)
+    acc = np.zeros((n, n, n), np.float64)
+    amp, norm = 1.0, 0.0
+    for k in range(octaves):
+        r = 2 ** (k + 2) + 1
+        lat = rng.random((r, r, r))
+        acc += amp * zoom(lat, n / r, order=1, mode="nearest")[:n, :n, :n]
+        norm += amp
+        amp *= 0.5
+    acc /= norm
+    c = (np.arange(n) + 0.5) / n - 0.5
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    blob = np.clip(1.0 - np.sqrt(xx * xx + yy * yy + zz * zz) / 0.5, 0.0, 1.0)
+    g = np.clip((acc - 0.35) * 2.5, 0.0, 1.0) * blob
+    return np.ascontiguousarray(g, np.float32)      # index [z, y, x] -> x fastest
+
+
+def boundary_node_tree():
+    """Material Output with Volume linked and Surface unlinked: the mesh only
+    bounds a medium (pybind_renderer.hpp:1386-1398, 1487-1503)."""
+    return {"nodes": [{"name": "Material Output", "type": "ShaderNodeOutputMaterial",
+                       "inputs": [{"name": "Surface"}, {"name": "Volume"}]},
+                      {"name": "Principled Volume", "type": "ShaderNodePrincipledVolume", "inputs": []}],
+            "links": [{"from_node": "Principled Volume", "from_socket": "Volume",
+                       "to_node": "Material Output", "to_socket": "Volume"}]}
+
+
+def smoke_box(sidecar_path: str, grid_n: int = 32, seed: int = 4, density: float = 6.0, anisotropy: float = 0.3,
+              albedo: float = 0.9) -> dict:
+    """The closed box with a gridded smoke medium (fBm density, albedo 0.9)
+    inside a boundary-only cube, plus an emissive-mesh light and a spot light.
+    Writes the float32 density grid to `sidecar_path`."""
+    scene = closed_box(seed=1)
+    g = fbm_grid(grid_n, seed)
+    g.tofile(sidecar_path)
+    lo, hi = [-0.8, 0.15, -0.8], [0.8, 1.75, 0.8]
+    v, t = _box([(lo[i] + hi[i]) / 2 for i in range(3)], [(hi[i] - lo[i]) / 2 for i in range(3)], 0.0)
+    vol = {"color": [albedo, albedo, albedo], "absorption_color": [0.0, 0.0, 0.0], "density": density,
+           "anisotropy": anisotropy,
+           "smoke_grid": {"dims": [grid_n, grid_n, grid_n], "world_min": lo, "world_max": hi,
+                          "sidecar_path": sidecar_path}}
+    scene["meshes"].append(_mesh(v, t, {"base_color": [0.8, 0.8, 0.8], "metallic": 0.0, "roughness": 0.5,
+                                        "emission": [0.0, 0.0, 0.0], "node_tree": boundary_node_tree(),
+                                        "volume": vol}))
+    # emissive mesh light: a small quad on the left wall
+    ev, et = _quad([-0.99, 1.2, -0.3], [-0.99, 1.2, 0.1], [-0.99, 1.6, 0.1], [-0.99, 1.6, -0.3])
+    scene["meshes"].append(_mesh(ev, et, {"base_color": [0.0, 0.0, 0.0], "metallic": 0.0, "roughness": 1.0,
+                                          "emission": [6.0, 4.0, 2.0], "transmission": 0.0, "ior": 1.45}))
+    scene["native_lights"].append({"type": "SPOT", "name": "Spot", "position": [0.6, 1.9, 0.6],
+                                   "direction": [-0.4, -1.0, -0.4], "spot_size": 0.9, "spot_blend": 0.2,
+                                   "color": [1.0, 1.0, 1.0], "energy": 40.0})
+    return scene
+
+
+SMOKE_BOX_CAMERA = CLOSED_BOX_CAMERA
+
+
 def to_json(scene: dict) -> str:
     return json.dumps(scene, separators=(",", ":"))
 

@@ -177,16 +177,20 @@ def _f3(v, default=(0.0, 0.0, 0.0)):
 
 
 def surface_linked(material: dict) -> bool:
-    """Material Output 'Surface' socket linked? (pybind_renderer.hpp:1386-1398)."""
+    """Material Output "Surface" socket linked? (pybind_renderer.hpp:1386-1398
+    with parseNodeTree, node_evaluator.cpp:82-149: a socket is linked when a
+    "links" entry targets (node name, socket name); a tree without nodes is
+    invalid and leaves the surface linked)."""
     nt = material.get("node_tree")
-    if not nt:
+    if not isinstance(nt, dict) or not isinstance(nt.get("nodes"), list) or not nt["nodes"]:
         return True
-    nodes = nt.get("nodes", []) if isinstance(nt, dict) else []
-    for n in nodes:
-        if n.get("type") == "ShaderNodeOutputMaterial":
-            for s in n.get("inputs", []):
-                if s.get("name") == "Surface":
-                    return bool(s.get("is_linked", False))
+    links = nt.get("links") or []
+    for n in nt["nodes"]:
+        if n.get("type", "") == "ShaderNodeOutputMaterial":
+            name = n.get("name", "")
+            for s in n.get("inputs", []) or []:
+                if s.get("name", "") == "Surface":
+                    return any(l.get("to_node") == name and l.get("to_socket") == "Surface" for l in links)
             return False
     return True
 
