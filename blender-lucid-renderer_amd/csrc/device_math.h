@@ -12,28 +12,32 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Functions used by the BVH traversal are also compiled for the host, so the
+// exact traversal code can run on the CPU in tests (lucid_debug_host_trace).
+#define LUCID_HD __host__ __device__ __forceinline__
+
 namespace lucid {
 
 struct v3f {
     float x, y, z;
 };
-__device__ __forceinline__ v3f mk(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }
-__device__ __forceinline__ v3f mk(float4 a) { return mk(a.x, a.y, a.z); }
-__device__ __forceinline__ v3f operator+(v3f a, v3f b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ v3f operator-(v3f a, v3f b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ v3f operator-(v3f a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ v3f operator*(v3f a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ v3f operator/(v3f a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ v3f cmul(v3f a, v3f b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ float dot(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len2(v3f a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
-__device__ __forceinline__ v3f cross(v3f a, v3f b) {
+LUCID_HD v3f mk(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }
+LUCID_HD v3f mk(float4 a) { return mk(a.x, a.y, a.z); }
+LUCID_HD v3f operator+(v3f a, v3f b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+LUCID_HD v3f operator-(v3f a, v3f b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+LUCID_HD v3f operator-(v3f a) { return mk(-a.x, -a.y, -a.z); }
+LUCID_HD v3f operator*(v3f a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+LUCID_HD v3f operator/(v3f a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+LUCID_HD v3f cmul(v3f a, v3f b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+LUCID_HD float dot(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LUCID_HD float len2(v3f a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+LUCID_HD v3f cross(v3f a, v3f b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 // std::min / std::max semantics: first argument on ties and NaN
-__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float max3(float a, float b, float c) { return smax(smax(a, b), c); }
+LUCID_HD float smin(float a, float b) { return (b < a) ? b : a; }
+LUCID_HD float smax(float a, float b) { return (a < b) ? b : a; }
+LUCID_HD float max3(float a, float b, float c) { return smax(smax(a, b), c); }
 
 constexpr float kDirEps2 = 1e-6f * 1e-6f;  // render_units.hpp:461,485-486
 constexpr float PI_F = 3.14159265358979323846f;

@@ -102,13 +102,13 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
     return base + rank;
 }
 
-__device__ __forceinline__ v3f inv_dir(v3f d) {  // intersection.hpp:412-419
+LUCID_HD v3f inv_dir(v3f d) {  // intersection.hpp:412-419
     return mk(fabsf(d.x) > 1e-8f ? 1.0f / d.x : 0.0f, fabsf(d.y) > 1e-8f ? 1.0f / d.y : 0.0f,
               fabsf(d.z) > 1e-8f ? 1.0f / d.z : 0.0f);
 }
 // intersectAABB slab test (intersection.hpp:257-302), with the entry
 // distance returned for ordering and pruning.
-__device__ __forceinline__ bool slab1(float dc, float ic, float dmin, float dmax, float& tmin, float& tmax) {
+LUCID_HD bool slab1(float dc, float ic, float dmin, float dmax, float& tmin, float& tmax) {
     if (fabsf(dc) < 1e-8f) return !((dmin > 0.0f && dmax > 0.0f) || (dmin < 0.0f && dmax < 0.0f));
     float t1 = dmin * ic, t2 = dmax * ic;
     float amin = smin(t1, t2), amax = smax(t1, t2);
@@ -125,7 +125,7 @@ __device__ __forceinline__ bool aabb(v3f o, v3f d, v3f id, const float* b, float
     return tmax >= smax(tmin, 0.0f);
 }
 // Moller-Trumbore with precomputed edges (intersection.hpp:335-396).
-__device__ __forceinline__ bool tri_test(v3f o, v3f d, float4 A, float4 E1, float4 E2, float tmin, float tmax,
+LUCID_HD bool tri_test(v3f o, v3f d, float4 A, float4 E1, float4 E2, float tmin, float tmax,
                                          float& t, float& u, float& v) {
     v3f e1 = mk(E1), e2 = mk(E2);
     v3f pvec = cross(d, e2);
@@ -146,7 +146,7 @@ __device__ __forceinline__ bool tri_test(v3f o, v3f d, float4 A, float4 E1, floa
 // closest t by more than 1 % cannot hold a triangle whose Moller-Trumbore t
 // ties or beats it; the reference never prunes, and the tie rule below
 // reproduces its "last tested wins" order.
-__device__ __forceinline__ float prune_bound(float best) { return best + smax(best * 0.01f, 1e-6f); }
+LUCID_HD float prune_bound(float best) { return best + smax(best * 0.01f, 1e-6f); }
 
 struct TraceHit {
     float t, u, v;
@@ -155,7 +155,7 @@ struct TraceHit {
 
 // Slab test of one child box given as scalars (no local arrays, so nothing
 // is spilled to scratch).
-__device__ __forceinline__ bool box_hit(v3f o, v3f d, v3f id, float x0, float y0, float z0, float x1, float y1,
+LUCID_HD bool box_hit(v3f o, v3f d, v3f id, float x0, float y0, float z0, float x1, float y1,
                                         float z1, float& tentry) {
     float tmin = 0.0f, tmax = RAY_T_MAX;
     if (!slab1(d.x, id.x, x0 - o.x, x1 - o.x, tmin, tmax)) return false;
@@ -171,14 +171,41 @@ __device__ __forceinline__ bool box_hit(v3f o, v3f d, v3f id, float x0, float y0
 // ANY = true: returns at the first triangle with tmin <= t <= tmax (the
 // reference's closest-hit shadow query reports a hit exactly when one
 // exists).  stk: this lane's LDS stack (stride kBlock), depth `sd`.
+// Host execution (tests only) checks every node and triangle index against
+// the array sizes before it is used; device code does not carry the checks.
+struct HostBounds {
+    int32_t n_nodes, n_tris;
+    bool violated;
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+#define LUCID_HOST_CHECK(cond) ((void)0)
+#else
+#define LUCID_HOST_CHECK(cond)                               \
+    do {                                                      \
+        if (hb && !(cond)) {                                  \
+            hb->violated = true;                              \
+            return false;                                     \
+        }                                                     \
+    } while (0)
+#endif
+LUCID_HD void note_overflow(int32_t* ovf) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(ovf, 1);
+#else
+    ++*ovf;
+#endif
+}
+
 template <bool ANY>
-__device__ __forceinline__ bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk,
-                                         int sd, int32_t* ovf, uint32_t& nn, uint32_t& nt, TraceHit& h) {
+LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk, int sd, int32_t* ovf,
+                       uint32_t& nn, uint32_t& nt, TraceHit& h, HostBounds* hb = nullptr) {
+    (void)hb;
     h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
     const v3f id = inv_dir(d);
     int sp = 0;
     int32_t node = S.root;
     for (;;) {
+        LUCID_HOST_CHECK(node >= 0 && node < hb->n_nodes);
         const float4* n4 = reinterpret_cast<const float4*>(S.nodes + node);
         const float4 b0 = n4[0], b1 = n4[1], b2 = n4[2];
         const int2 cc = *reinterpret_cast<const int2*>(n4 + 3);
@@ -191,16 +218,21 @@ __device__ __forceinline__ bool traverse(const SceneDev& S, v3f o, v3f d, float 
         int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
         bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
         const float tf = sw ? te0 : te1;
+        // Leaf children are consumed here, whether tested or pruned, so only
+        // internal nodes (index >= 0) are ever descended into or pushed.
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int32_t c = k == 0 ? cn : cf;
-            const bool hit = k == 0 ? hn : (hf && tf <= prune_bound(h.t));
-            if (hit && c < 0) {
+            const bool take = k == 0 ? hn : hf;
+            if (take && c < 0) {
+                if (k == 0) hn = false; else hf = false;
+                if (k == 1 && !(tf <= prune_bound(h.t))) continue;  // pruned by the near leaf's hit
                 const int32_t enc = ~c;
                 const int32_t start = enc >> 3, count = enc & 7;
                 nt += count;
                 for (int i = 0; i < count; ++i) {
                     const int32_t prim = start + i;
+                    LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
                     float t, u, v;
                     if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
                         if (ANY) {
@@ -212,7 +244,6 @@ __device__ __forceinline__ bool traverse(const SceneDev& S, v3f o, v3f d, float 
                         }
                     }
                 }
-                if (k == 0) hn = false; else hf = false;
             }
         }
         if (hn && hf) {
@@ -220,7 +251,7 @@ __device__ __forceinline__ bool traverse(const SceneDev& S, v3f o, v3f d, float 
                 stk[sp * kBlock] = cf;
                 ++sp;
             } else {
-                atomicAdd(ovf, 1);
+                note_overflow(ovf);
             }
             node = cn;
             continue;
@@ -1327,6 +1358,45 @@ void lucid_reset_cancel(lucid_ctx* ctx) {
 int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out) {
     if (!ctx || !out) return LUCID_ERR_INVALID;
     *out = ctx->stats;
+    return LUCID_OK;
+}
+
+int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, int32_t n_rays, const float* o,
+                           const float* d, const float* tmin, const float* tmax, int32_t any_hit, int32_t* mesh,
+                           int32_t* tri, float* u, float* v, float* t) {
+    if (n_rays < 0 || (n_rays > 0 && (!o || !d || !tmin || !tmax || !mesh || !tri || !u || !v || !t)))
+        return LUCID_ERR_INVALID;
+    PackedScene ps;
+    std::string err;
+    const float env[3] = {0.0f, 0.0f, 0.0f};
+    if (!pack_scene(meshes, num_meshes, nullptr, 0, env, 1.0f, ps, err)) return LUCID_ERR_INVALID;
+    SceneDev S;
+    std::memset(&S, 0, sizeof(S));
+    S.nodes = ps.nodes.data();
+    S.v0 = reinterpret_cast<const float4*>(ps.tri_v0.data());
+    S.e1 = reinterpret_cast<const float4*>(ps.tri_e1.data());
+    S.e2 = reinterpret_cast<const float4*>(ps.tri_e2.data());
+    S.root = ps.root;
+    const int sd = std::min(kStackMax, ps.max_depth + 2);
+    std::vector<int32_t> stack((size_t)sd * kBlock);
+    HostBounds hb{(int32_t)ps.nodes.size(), ps.num_triangles, false};
+    int32_t ovf = 0;
+    for (int32_t i = 0; i < n_rays; ++i) {
+        uint32_t nn = 0, nt = 0;
+        TraceHit h;
+        const v3f ro = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]), rd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        bool hit = any_hit ? traverse<true>(S, ro, rd, tmin[i], tmax[i], stack.data(), sd, &ovf, nn, nt, h, &hb)
+                           : traverse<false>(S, ro, rd, tmin[i], tmax[i], stack.data(), sd, &ovf, nn, nt, h, &hb);
+        if (hb.violated || ovf) return LUCID_ERR_DEVICE;
+        if (hit) {
+            int32_t mi, ti;
+            std::memcpy(&mi, &ps.tri_v0[4 * (size_t)h.prim + 3], 4);
+            std::memcpy(&ti, &ps.tri_e1[4 * (size_t)h.prim + 3], 4);
+            mesh[i] = mi; tri[i] = ti; u[i] = h.u; v[i] = h.v; t[i] = h.t;
+        } else {
+            mesh[i] = -1; tri[i] = -1; u[i] = 0.0f; v[i] = 0.0f; t[i] = 0.0f;
+        }
+    }
     return LUCID_OK;
 }
 

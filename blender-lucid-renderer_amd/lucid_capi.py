@@ -90,6 +90,10 @@ SIGNATURES = {
     "lucid_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
+                                         C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -237,6 +241,26 @@ class Context:
         if getattr(self, "h", None):
             self.L.lucid_destroy(self.h)
             self.h = None
+
+
+def host_trace(scene: dict, o, d, tmin, tmax, any_hit=False):
+    """The device traversal code executed on the CPU (lucid_debug_host_trace)."""
+    L = load()
+    sd = SceneDescs(scene)
+    o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    n = o.shape[0]
+    tmin = np.ascontiguousarray(np.broadcast_to(np.float32(tmin), (n,)) if np.isscalar(tmin) else tmin, np.float32)
+    tmax = np.ascontiguousarray(np.broadcast_to(np.float32(tmax), (n,)) if np.isscalar(tmax) else tmax, np.float32)
+    out = {k: np.zeros(n, np.int32) for k in ("mesh", "tri")}
+    out.update({k: np.zeros(n, np.float32) for k in ("u", "v", "t")})
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    rc = L.lucid_debug_host_trace(sd.meshes, sd.n_meshes, n, fp(o), fp(d), fp(tmin), fp(tmax), 1 if any_hit else 0,
+                                  ip(out["mesh"]), ip(out["tri"]), fp(out["u"]), fp(out["v"]), fp(out["t"]))
+    if rc != 0:
+        raise RuntimeError(f"lucid_debug_host_trace failed: {ERRORS.get(rc, rc)}")
+    return out
 
 
 def camera(cam: dict, exposure_ev: float = 0.0) -> Camera:

@@ -190,6 +190,16 @@ int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out);
 /* Paths per wavefront batch (0 = automatic). */
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
 
+/* Runs the device BVH traversal code on the CPU over a scene packed from
+   `meshes` (no device needed), checking every node and triangle index before
+   use: closest hit (any_hit = 0) or occlusion (any_hit = 1) of n_rays rays.
+   Outputs as lucid_primary_hits.  Returns LUCID_ERR_DEVICE if an index would
+   leave its array or the stack would overflow.  For tests of the traversal;
+   it replaces intersectScene (include/geometry/bvh.hpp:155-177) on the host. */
+int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, int32_t n_rays, const float* o,
+                           const float* d, const float* tmin, const float* tmax, int32_t any_hit, int32_t* mesh,
+                           int32_t* tri, float* u, float* v, float* t);
+
 /* Instrumentation flags (LUCID_PROFILE_*), 0 = off (default). */
 int lucid_set_profiling(lucid_ctx* ctx, int32_t flags);
 
