@@ -38,6 +38,7 @@ namespace lucid {
 
 constexpr int kBlock = 128;      // 2 wavefronts per workgroup
 constexpr int kStackMax = 64;    // LDS traversal stack entries per lane, upper bound
+constexpr int64_t kWaveBytesPerPath = 10 * 16 + 3 * 4;  // WaveDev SoA bytes per path slot
 
 // ------------------------------------------------------------- device scene --
 struct SceneDev {
@@ -857,6 +858,10 @@ struct lucid_ctx {
     std::atomic<int> cancel{0};
     std::mutex mu;
     int64_t wave_paths = 0;
+    // Launch shape: workgroups per CU of the persistent grid-stride kernels,
+    // or 0 for one lane per queued ray.  Traversal latency is best hidden by
+    // the hardware dispatcher (0); shading runs best persistent.
+    int32_t traverse_groups = 0, shade_groups = 16;
     // wave buffers
     int64_t cap_paths = 0;
     std::vector<void*> wave_allocs;
@@ -991,7 +996,19 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     *cancelled = false;
     const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
     const int S = pr->samples;
-    int64_t target = ctx->wave_paths > 0 ? ctx->wave_paths : (int64_t)1 << 21;
+    // Wavefront batch: as many paths as fit a quarter of free device memory,
+    // at most 2^26.  Each depth iteration costs one launch per kernel whose
+    // duration has a floor set by the slowest wavefront's traversal, so few
+    // large batches beat many small ones.
+    int64_t target = ctx->wave_paths;
+    if (target <= 0) {
+        size_t free_b = 0, total_b = 0;
+        target = (int64_t)1 << 26;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+            const int64_t fit = (int64_t)(free_b / 4 / kWaveBytesPerPath) + ctx->cap_paths;
+            target = std::max<int64_t>(std::min<int64_t>(target, fit), (int64_t)1 << 16);
+        }
+    }
     int64_t B = std::max<int64_t>(1, std::min<int64_t>(S, target / P));
     int rc = ensure_wave(ctx, P * B);
     if (rc) return rc;
@@ -1003,7 +1020,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     // workgroups per CU walk the live queues (grid-stride).
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
     const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
-    const int64_t grid_cap = (int64_t)ctx->num_cu * 8;
+    const int64_t cap_trav = ctx->traverse_groups > 0 ? (int64_t)ctx->num_cu * ctx->traverse_groups : INT64_MAX;
+    const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     ctx->kev_used = 0;
     k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
@@ -1042,26 +1060,27 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         int q = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
             int ks = kev_begin(ctx, 0, st);
-            const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), grid_cap);
+            const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_trav);
+            const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
             k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
             ks = kev_begin(ctx, 1, st);
             switch (algo) {
                 case LUCID_ALGO_SIMPLE:
-                    k_shade<LUCID_ALGO_SIMPLE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 case LUCID_ALGO_MIS:
-                    k_shade<LUCID_ALGO_MIS><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_MIS><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_SIMPLE:
-                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_MIS:
-                    k_shade<LUCID_ALGO_VOLUME_MIS><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
                 default:
-                    k_shade<LUCID_ALGO_NEE><<<g, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
             }
             kev_end(ctx, ks, st);
@@ -1409,6 +1428,15 @@ int lucid_set_profiling(lucid_ctx* ctx, int32_t flags) {
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;
     ctx->wave_paths = paths;
+    return LUCID_OK;
+}
+
+int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu) {
+    if (!ctx || traverse_groups_per_cu < 0 || traverse_groups_per_cu > 64 || shade_groups_per_cu < 0 ||
+        shade_groups_per_cu > 64)
+        return LUCID_ERR_INVALID;
+    ctx->traverse_groups = traverse_groups_per_cu;
+    ctx->shade_groups = shade_groups_per_cu;
     return LUCID_OK;
 }
 

@@ -357,6 +357,13 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
             v.density = d.vol_density;
             v.anisotropy = d.vol_anisotropy;
             bool grid = d.grid && d.grid_dims[0] > 0 && d.grid_dims[1] > 0 && d.grid_dims[2] > 0;
+            if (grid && (d.grid_dims[0] < 2 || d.grid_dims[1] < 2 || d.grid_dims[2] < 2)) {
+                // The trilinear lookup (transmittance.hpp:298-345) clamps the
+                // cell index to [0, n-2], an empty range for a 1-voxel axis
+                // that the reference leaves undefined; refuse such grids.
+                err = "mesh " + std::to_string(m) + ": density grid needs at least 2 voxels on every axis";
+                return false;
+            }
             if (grid) {
                 size_t n = (size_t)d.grid_dims[0] * d.grid_dims[1] * d.grid_dims[2];
                 v.has_grid = 1;

@@ -89,6 +89,7 @@ SIGNATURES = {
     "lucid_reset_cancel": (None, [C.c_void_p]),
     "lucid_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
+    "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,

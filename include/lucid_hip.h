@@ -187,8 +187,16 @@ void lucid_reset_cancel(lucid_ctx* ctx);
 
 int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out);
 
-/* Paths per wavefront batch (0 = automatic). */
+/* Paths per wavefront batch (0 = automatic: up to 2^26 paths, bounded by a
+   quarter of free device memory at 172 B per path). */
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
+
+/* Launch shape of the traversal (closest and shadow) and shading kernels:
+   persistent grids of N workgroups per compute unit walk the ray queues
+   (1..64), or 0 for one lane per queued ray.  Defaults: traversal 0,
+   shading 16.  A tuning knob with no counterpart in the reference (its Dawn
+   backend fixes workgroup counts per dispatch). */
+int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
 
 /* Runs the device BVH traversal code on the CPU over a scene packed from
    `meshes` (no device needed), checking every node and triangle index before
