@@ -33,6 +33,7 @@
 #include "lucid_hip.h"
 #include "scene_pack.h"
 #include "shading.h"
+#include "diag_film.h"
 
 namespace lucid {
 
@@ -58,8 +59,16 @@ struct SceneDev {
     float total_area;
     float env[3];
     int32_t any_vbo;
+    int32_t native_start;  // index of the first native light in `lights`
 };
 
+// Diagnostic path storage (only in diagnostic runs), indexed [k * dstride + p]:
+//   dv0/dv1  vertex k: position + object id, normal + packed
+//            (material & 0xffff) | bsdf_type << 16 | flags << 24
+//   dr0/dr1  record slot k (= bounce): contribution + packed
+//            valid | strategy << 8 | light type << 16 | prefix vertices << 24,
+//            light vertex position + light object id
+//   dcnt     vertices | slots << 8
 struct WaveDev {
     float4* ray_o;   // origin, tmin
     float4* ray_d;   // direction
@@ -76,6 +85,9 @@ struct WaveDev {
     int32_t* cnt;    // [0],[1] queue counts, [2] shadow count, [3] stack overflows
     unsigned long long* rays;  // [0] closest, [1] shadow, [2..3] closest nodes/tris, [4..5] shadow nodes/tris
     int32_t count_work;        // accumulate node / triangle fetch counts
+    float4 *dv0, *dv1, *dr0, *dr1;
+    int32_t* dcnt;
+    int64_t dstride;
 };
 
 struct CamDev {
@@ -417,6 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
     W.rng[p] = make_ulonglong2(rng.state, rng.inc);
     W.meta[p] = make_int4(pix, s, 0, 0);
     W.q[0][p] = p;
+    if (W.dcnt) W.dcnt[p] = 0;
 }
 
 // ------------------------------------------------------------- intersect --
@@ -518,10 +531,57 @@ __device__ __forceinline__ int select_light(const SceneDev& S, float u, float& p
 
 __device__ __forceinline__ bool finite3(v3f a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
 
-// One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570).
+// Internal algorithm ids of the diagnostic tracers (traceSimpleWithDiagnostics,
+// traceNEEWithDiagnostics, traceMISWithDiagnostics; path_tracer.hpp:880-1514).
+constexpr int kAlgoDiagBase = 16;
+constexpr int kAlgoDiagSimple = kAlgoDiagBase + LUCID_ALGO_SIMPLE;
+constexpr int kAlgoDiagNee = kAlgoDiagBase + LUCID_ALGO_NEE;
+constexpr int kAlgoDiagMis = kAlgoDiagBase + LUCID_ALGO_MIS;
+constexpr int32_t kRecValid = 1;  // bit 0 of a record's packed meta
+
+// classify_bsdf / is_delta_bsdf (diagnostic_integrator.hpp:24-60) on the
+// material parameters after getMaterialParams.
+__device__ __forceinline__ uint8_t classify_bsdf(const Mat& m) {
+    if (m.transmission > 0.5f) return diag::kGlass;
+    if (m.roughness < 0.01f && m.metallic > 0.5f) return diag::kMirror;
+    if (m.metallic > 0.5f || m.roughness < 0.5f) return diag::kGlossy;
+    return diag::kDiffuse;
+}
+__device__ __forceinline__ bool is_delta_bsdf(const Mat& m) {
+    return (m.transmission > 0.5f && m.roughness < 0.01f) || (m.metallic > 0.5f && m.roughness < 0.01f);
+}
+// to_light_source_type (path_types.hpp:160-169)
+__device__ __forceinline__ uint8_t light_source_type(int32_t t) {
+    switch (t) {
+        case 0: return diag::kLightArea;
+        case 1: return diag::kLightPoint;
+        case 2: return diag::kLightDirectional;
+        case 3: return diag::kLightSpot;
+        case 4: return diag::kLightArea;
+        default: return diag::kLightUnknown;
+    }
+}
+// Record slot `slot` of path p: a completed path (record_completed_path_with_
+// geometry, diagnostic_integrator.hpp:338-368) = the path's first `prefix`
+// vertices plus a light vertex {light_obj, -1, Emission, 0} at lpos.
+__device__ __forceinline__ void put_record(const WaveDev& W, int p, int slot, v3f c, uint8_t strategy,
+                                           uint8_t light_type, int prefix, int32_t light_obj, v3f lpos, bool valid) {
+    const int32_t meta = (valid ? kRecValid : 0) | ((int32_t)strategy << 8) | ((int32_t)light_type << 16) |
+                         ((int32_t)prefix << 24);
+    const size_t k = (size_t)slot * (size_t)W.dstride + (size_t)p;
+    W.dr0[k] = make_float4(c.x, c.y, c.z, __int_as_float(meta));
+    W.dr1[k] = make_float4(lpos.x, lpos.y, lpos.z, __int_as_float(light_obj));
+}
+
+// One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570),
+// their volume variants (:209-283, 586-835), or the diagnostic tracers.
 template <int ALGO>
 __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, int qin, int32_t max_depth, int i,
                                           int n) {
+    constexpr bool DIAG = ALGO >= kAlgoDiagBase;
+    constexpr int A = DIAG ? ALGO - kAlgoDiagBase : ALGO;
+    constexpr bool MIS = A == LUCID_ALGO_MIS || A == LUCID_ALGO_VOLUME_MIS;
+    constexpr bool SIMPLE = A == LUCID_ALGO_SIMPLE || A == LUCID_ALGO_VOLUME_SIMPLE;
     bool live = i < n;
     bool cont = false, nee = false;
     int p = live ? W.q[qin][i] : 0;
@@ -538,17 +598,24 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
         rng.state = rs.x; rng.inc = rs.y;
         float4 Lv = W.rad[p];
         v3f L = mk(Lv);
+        // diagnostic path: vertex count so far; this bounce's record slot
+        // starts empty (it may hold a record of an earlier batch)
+        int nvert = 0;
+        if (DIAG) {
+            nvert = W.dcnt[p] & 0xff;
+            W.dr0[(size_t)depth * (size_t)W.dstride + (size_t)p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         bool has_hit = prim >= 0;
         NativeHit lh;
         lh.hit = false;
         if (depth > 0 && S.n_native > 0) lh = native_lights(S, o, d);
         bool done = false;
-        constexpr bool VOLUME = ALGO == LUCID_ALGO_VOLUME_MIS || ALGO == LUCID_ALGO_VOLUME_SIMPLE;
+        constexpr bool VOLUME = A == LUCID_ALGO_VOLUME_MIS || A == LUCID_ALGO_VOLUME_SIMPLE;
         if (VOLUME) {
             float ts = 1e6f;
             if (has_hit) ts = smin(ts, hv.x);
             if (lh.hit) ts = smin(ts, lh.t);
-            if (ALGO == LUCID_ALGO_VOLUME_SIMPLE) {
+            if (A == LUCID_ALGO_VOLUME_SIMPLE) {
                 // traceVolumeSimple (path_tracer.hpp:226-230): absorption only
                 if (S.n_vols > 0) thr = cmul(thr, vol_transmittance(S, o, d, ts, false));
             } else if (S.n_vols > 0) {
@@ -609,21 +676,34 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
         }
         if (done) {
         } else if (lh.hit && (!has_hit || lh.t < hv.x)) {
-            v3f c = cmul(lh.emission, thr);
-            if (ALGO == LUCID_ALGO_MIS || ALGO == LUCID_ALGO_VOLUME_MIS) {
-                float lp = 0.0f;
-                if (S.n_lights > 0 && lh.index >= 0) {
-                    int sli = S.native_ids[lh.index];
-                    float sp = area_ratio(S.lights[sli].area, S.total_area);
-                    if (sp > 0.0f) lp = pdf_light(S.lights[sli].area, o, lh.point, lh.normal) * sp;
+            // the NEE diagnostic tracer counts light hits of camera rays only
+            if (!(DIAG && A == LUCID_ALGO_NEE && depth != 0)) {
+                v3f c = cmul(lh.emission, thr);
+                if (MIS) {
+                    float lp = 0.0f;
+                    if (S.n_lights > 0 && lh.index >= 0) {
+                        int sli = S.native_ids[lh.index];
+                        float sp = area_ratio(S.lights[sli].area, S.total_area);
+                        if (sp > 0.0f) lp = pdf_light(S.lights[sli].area, o, lh.point, lh.normal) * sp;
+                    }
+                    float mw = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
+                    c = c * mw;
                 }
-                float mw = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
-                c = c * mw;
+                L = L + c;
+                if (DIAG)
+                    put_record(W, p, depth, c, MIS ? diag::kMisBSDF : diag::kBSDF,
+                               light_source_type(S.lights[S.native_ids[lh.index]].type), nvert, -(lh.index + 2),
+                               o + d * lh.t, true);
             }
-            L = L + c;
             done = true;
         } else if (!has_hit) {
-            L = L + cmul(mk(S.env[0], S.env[1], S.env[2]), thr);
+            if (!(DIAG && A == LUCID_ALGO_NEE && depth != 0)) {
+                const v3f c = cmul(mk(S.env[0], S.env[1], S.env[2]), thr);
+                L = L + c;
+                if (DIAG)
+                    put_record(W, p, depth, c, MIS ? diag::kMisBSDF : diag::kBSDF, diag::kLightEnvironment, nvert,
+                               -1, o + d * 1000.0f, true);
+            }
             done = true;
         }
         if (!done) {
@@ -649,10 +729,25 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             bool front = dot(wo, nrm) > 0.0f;
             v3f sn = (mat.transmission < 0.5f && !front) ? -nrm : nrm;
             bool has_em = (gm.flags & MF_EMISSIVE) != 0;
-            if (ALGO == LUCID_ALGO_SIMPLE || ALGO == LUCID_ALGO_VOLUME_SIMPLE) {
-                L = L + cmul(em, thr);
-            } else if (ALGO == LUCID_ALGO_NEE) {
-                if (has_em && depth == 0) L = L + cmul(em, thr);
+            bool stop = false;  // traceSimpleWithDiagnostics ends at an emitter
+            if (SIMPLE) {
+                if (DIAG) {
+                    if (has_em) {
+                        const v3f c = cmul(em, thr);
+                        L = L + c;
+                        put_record(W, p, depth, c, diag::kBSDF, diag::kLightEmissive, nvert, -(mesh + 2), point, true);
+                        stop = true;
+                    }
+                } else {
+                    L = L + cmul(em, thr);
+                }
+            } else if (A == LUCID_ALGO_NEE) {
+                if (has_em && depth == 0) {
+                    const v3f c = cmul(em, thr);
+                    L = L + c;
+                    if (DIAG)
+                        put_record(W, p, depth, c, diag::kBSDF, diag::kLightEmissive, nvert, -(mesh + 2), point, true);
+                }
             } else if (has_em) {  // traceMIS / traceVolumeMIS
                 float mw = 1.0f;
                 if (S.n_lights > 0 && last_pdf > MIN_PDF) {
@@ -665,11 +760,22 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                         }
                     }
                 }
-                L = L + cmul(em, thr) * mw;
+                const v3f c = cmul(em, thr) * mw;
+                L = L + c;
+                if (DIAG)
+                    put_record(W, p, depth, c, diag::kMisBSDF, diag::kLightEmissive, nvert, -(mesh + 2), point, true);
+            }
+            // diagnostic vertex of a non-emissive surface (record_vertex_with_geometry)
+            if (DIAG && !has_em && nvert < diag::kMaxDepth) {
+                const int32_t packed = (0 & 0xffff) | ((int32_t)classify_bsdf(mat) << 16) |
+                                       ((int32_t)(is_delta_bsdf(mat) ? diag::kFlagDelta : 0) << 24);
+                const size_t k = (size_t)nvert * (size_t)W.dstride + (size_t)p;
+                W.dv0[k] = make_float4(point.x, point.y, point.z, __int_as_float(mesh));
+                W.dv1[k] = make_float4(nrm.x, nrm.y, nrm.z, __int_as_float(packed));
+                ++nvert;
             }
             // next event estimation
-            if (ALGO != LUCID_ALGO_SIMPLE && ALGO != LUCID_ALGO_VOLUME_SIMPLE && S.n_lights > 0 && !has_em &&
-                !(mat.transmission > 0.5f)) {
+            if (!SIMPLE && S.n_lights > 0 && !has_em && !(mat.transmission > 0.5f)) {
                 float sp;
                 int li = select_light(S, randf(rng), sp);
                 if (sp > 1e-6f) {
@@ -681,11 +787,11 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                             v3f f = eval_bsdf(mat, wo, ls.dir, sn);
                             float pl = ls.pdf * sp;
                             v3f cc;
-                            if (ALGO == LUCID_ALGO_MIS) {
+                            if (A == LUCID_ALGO_MIS) {
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
-                            } else if (ALGO == LUCID_ALGO_VOLUME_MIS) {
+                            } else if (A == LUCID_ALGO_VOLUME_MIS) {
                                 // surface NEE through media (path_tracer.hpp:783-797)
                                 const v3f T = S.n_vols > 0 ? vol_transmittance(S, point, ls.dir, ls.dist, true)
                                                            : mk(1.0f, 1.0f, 1.0f);
@@ -696,8 +802,16 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                                 cc = cmul(bsdf_weight(f, nl, pl), ls.emission);
                             }
                             cc = cmul(cc, thr);
+                            float tag = 0.0f;
+                            if (DIAG) {
+                                // pending until the shadow ray finds the light unoccluded
+                                put_record(W, p, depth, cc, MIS ? diag::kMisNEE : diag::kNEE,
+                                           light_source_type(S.lights[li].type), nvert,
+                                           -((li - S.native_start) + 2), ls.pos, false);
+                                tag = __int_as_float(depth + 1);
+                            }
                             W.sh_o[p] = make_float4(point.x, point.y, point.z, ls.dist);
-                            W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                            W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tag);
                             W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
                             nee = true;
                         }
@@ -706,7 +820,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             }
             // BSDF sampling for the next bounce (at the last bounce the reference
             // still samples, but nothing it draws reaches the result)
-            if (depth + 1 < max_depth) {
+            if (!stop && depth + 1 < max_depth) {
             v3f smp_n = (mat.transmission > 0.5f) ? nrm : sn;
             float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
             BSample bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
@@ -719,7 +833,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             }
             if (ok) {
                 thr = cmul(thr, w);
-                if (ALGO == LUCID_ALGO_MIS || ALGO == LUCID_ALGO_VOLUME_MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
+                if (MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
                 if (depth >= 3) {
                     float mx = max3(thr.x, thr.y, thr.z);
                     float rr = smin(mx, 0.95f);
@@ -738,6 +852,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             }
         }
         W.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
+        if (DIAG) W.dcnt[p] = nvert | ((depth + 1) << 8);
     }
     int slot = wave_append(&W.cnt[1 - qin], cont);
     if (cont) W.q[1 - qin][slot] = p;
@@ -766,6 +881,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W, int sd
         if (!shadowed(S, mk(o), mk(d), o.w, lds_stack + threadIdx.x, sd, &W.cnt[3], nn, nt)) {
             const float4 L = W.rad[p], c = W.sh_c[p];
             W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+            // diagnostic runs: the pending NEE record of this bounce is kept
+            const int32_t tag = __float_as_int(d.w);
+            if (tag > 0) {
+                float4* r = W.dr0 + (size_t)(tag - 1) * (size_t)W.dstride + (size_t)p;
+                r->w = __int_as_float(__float_as_int(r->w) | kRecValid);
+            }
         }
     }
     if (W.count_work && (nn | nt)) {
@@ -830,6 +951,105 @@ __global__ __launch_bounds__(256) void k_hits_out(SceneDev S, WaveDev W, int32_t
     }
 }
 
+
+// ------------------------------------------------------------ diag film --
+// A recorded completed path, read from the wave's diagnostic storage: the
+// first `prefix` path vertices, then the light vertex when the path had room
+// (record_completed_path_with_geometry, diagnostic_integrator.hpp:338-368).
+struct DevTrace {
+    const float4 *dv0, *dv1;
+    int64_t stride;
+    int p, prefix, n;
+    float4 r0, r1;
+    __device__ int depth() const { return n; }
+    __device__ diag::Vertex vertex(int k) const {
+        diag::Vertex v;
+        if (k < prefix) {
+            const size_t i = (size_t)k * (size_t)stride + (size_t)p;
+            const int32_t packed = __float_as_int(dv1[i].w);
+            v.object_id = __float_as_int(dv0[i].w);
+            v.material_id = (int16_t)(packed & 0xffff);
+            v.bsdf_type = (uint8_t)((packed >> 16) & 0xff);
+            v.flags = (uint8_t)((packed >> 24) & 0xff);
+        } else {
+            v.object_id = __float_as_int(r1.w);
+            v.material_id = -1;
+            v.bsdf_type = diag::kEmission;
+            v.flags = 0;
+        }
+        return v;
+    }
+    __device__ diag::F3 position(int k) const {
+        if (k < prefix) {
+            const float4 a = dv0[(size_t)k * (size_t)stride + (size_t)p];
+            return diag::F3{a.x, a.y, a.z};
+        }
+        return diag::F3{r1.x, r1.y, r1.z};
+    }
+    __device__ diag::F3 normal(int k) const {
+        if (k < prefix) {
+            const float4 b = dv1[(size_t)k * (size_t)stride + (size_t)p];
+            return diag::F3{b.x, b.y, b.z};
+        }
+        return diag::F3{0.0f, 0.0f, 0.0f};
+    }
+    __device__ diag::F3 contribution() const { return diag::F3{r0.x, r0.y, r0.z}; }
+    __device__ uint8_t light_type() const { return (uint8_t)((__float_as_int(r0.w) >> 16) & 0xff); }
+    __device__ uint8_t strategy() const { return (uint8_t)((__float_as_int(r0.w) >> 8) & 0xff); }
+};
+
+// DiagnosticFilm::record_path for one wavefront batch: one thread per film
+// pixel applies, sample by sample and within a sample for the tile pixels
+// (fx*sub+dx, fy*sub+dy) in row-major order, each path's valid records in
+// bounce order; and adds each path's per-strategy sums to the FP64 Welford.
+__global__ __launch_bounds__(64) void k_diag_film(WaveDev W, diag::PixelData* film, diag::StrategyWelford* strat,
+                                                  unsigned long long* total, int32_t tw, int32_t sub, int32_t sw,
+                                                  int32_t sh, int32_t P, int32_t B) {
+    const int fi = blockIdx.x * 64 + threadIdx.x;
+    if (fi >= sw * sh) return;
+    const int fx = fi % sw, fy = fi / sw;
+    diag::PixelData& px = film[fi];
+    diag::StrategyWelford* w = strat + (size_t)fi * 4;
+    unsigned long long count = 0;
+    for (int b = 0; b < B; ++b) {
+        for (int dy = 0; dy < sub; ++dy) {
+            for (int dx = 0; dx < sub; ++dx) {
+                const int pix = (fy * sub + dy) * tw + fx * sub + dx;
+                const int p = b * P + pix;
+                const int slots = W.dcnt[p] >> 8;
+                double sum[4][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+                for (int sl = 0; sl < slots; ++sl) {
+                    const size_t k = (size_t)sl * (size_t)W.dstride + (size_t)p;
+                    const float4 r0 = W.dr0[k];
+                    const int32_t meta = __float_as_int(r0.w);
+                    if (!(meta & kRecValid)) continue;
+                    DevTrace t;
+                    t.dv0 = W.dv0; t.dv1 = W.dv1; t.stride = W.dstride; t.p = p;
+                    t.prefix = (meta >> 24) & 0xff;
+                    t.n = t.prefix < diag::kMaxDepth ? t.prefix + 1 : t.prefix;
+                    t.r0 = r0; t.r1 = W.dr1[k];
+                    diag::record_path(px, t);
+                    ++count;
+                    const int sg = (meta >> 8) & 3;
+                    sum[sg][0] += (double)r0.x; sum[sg][1] += (double)r0.y; sum[sg][2] += (double)r0.z;
+                }
+                for (int g = 0; g < 4; ++g) diag::welford_add(w[g], sum[g][0], sum[g][1], sum[g][2]);
+            }
+        }
+    }
+    if (count) atomicAdd(total, count);
+}
+
+// Film reset: every pixel takes the default PixelPathData pattern (1197
+// 8-byte words), every Welford accumulator zero.
+__global__ __launch_bounds__(256) void k_diag_init(uint64_t* film, const uint64_t* pattern, int64_t words,
+                                                   double* strat, int64_t n_strat) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    constexpr int64_t kWords = (int64_t)sizeof(diag::PixelData) / 8;
+    if (i < words) film[i] = pattern[i % kWords];
+    if (i < n_strat) strat[i] = 0.0;
+}
+
 }  // namespace lucid
 
 // ============================================================ host side ==
@@ -872,6 +1092,19 @@ struct lucid_ctx {
     int64_t out_cap = 0;
     lucid_stats stats{};
     int32_t profiling = 0;
+    // DiagnosticFilm (rebuilt by every diagnostic render, sized to the tile)
+    int32_t diag_on = 0, diag_sub = 1;
+    int32_t diag_tw = 0, diag_th = 0;
+    bool diag_valid = false;
+    diag::PixelData* dfilm = nullptr;
+    int64_t dfilm_cap = 0;
+    double* dstrat = nullptr;
+    int64_t dstrat_cap = 0;
+    unsigned long long* dtotal = nullptr;
+    uint64_t* dpattern = nullptr;
+    std::vector<void*> diag_allocs;  // per-path diagnostic storage
+    int64_t diag_cap_paths = 0;
+    int32_t diag_slots = 0;
     // LUCID_PROFILE_KERNELS: one event pair per launch, kind 0/1/2
     std::vector<hipEvent_t> kev;
     std::vector<int> kev_kind;
@@ -934,6 +1167,46 @@ int ensure_wave(lucid_ctx* ctx, int64_t n) {
     return LUCID_OK;
 }
 
+void free_diag_wave(lucid_ctx* ctx) {
+    for (void* p : ctx->diag_allocs) (void)hipFree(p);
+    ctx->diag_allocs.clear();
+    ctx->diag_cap_paths = 0;
+    ctx->diag_slots = 0;
+}
+// Per-path diagnostic storage: 16 vertices x 32 B, `slots` records x 32 B,
+// one count word.
+int ensure_diag_wave(lucid_ctx* ctx, int64_t n, int32_t slots) {
+    if (n <= ctx->diag_cap_paths && slots <= ctx->diag_slots) return LUCID_OK;
+    free_diag_wave(ctx);
+    n = std::max(n, ctx->diag_cap_paths);
+    slots = std::max(slots, ctx->diag_slots);
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        ctx->diag_allocs.push_back(p);
+        return p;
+    };
+    void* a = alloc((size_t)n * diag::kMaxDepth * 16);
+    void* b = alloc((size_t)n * diag::kMaxDepth * 16);
+    void* c = alloc((size_t)n * (size_t)slots * 16);
+    void* d = alloc((size_t)n * (size_t)slots * 16);
+    void* e = alloc((size_t)n * 4);
+    if (!a || !b || !c || !d || !e) {
+        free_diag_wave(ctx);
+        return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (diagnostic path storage)");
+    }
+    ctx->diag_cap_paths = n;
+    ctx->diag_slots = slots;
+    return LUCID_OK;
+}
+void free_diag_film(lucid_ctx* ctx) {
+    if (ctx->dfilm) (void)hipFree(ctx->dfilm);
+    if (ctx->dstrat) (void)hipFree(ctx->dstrat);
+    ctx->dfilm = nullptr; ctx->dstrat = nullptr;
+    ctx->dfilm_cap = ctx->dstrat_cap = 0;
+    ctx->diag_valid = false;
+}
+
 template <class T>
 int ensure_buf(lucid_ctx* ctx, T** p, int64_t* cap, int64_t n) {
     if (n <= *cap) return LUCID_OK;
@@ -943,6 +1216,10 @@ int ensure_buf(lucid_ctx* ctx, T** p, int64_t* cap, int64_t n) {
     if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed");
     *cap = n;
     return LUCID_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long* p) {
+    if (threadIdx.x == 0) *p = 0ull;
 }
 
 __global__ void k_zero_counters(int32_t* cnt, unsigned long long* rays) {
@@ -1009,13 +1286,62 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             target = std::max<int64_t>(std::min<int64_t>(target, fit), (int64_t)1 << 16);
         }
     }
+    // Diagnostic renders (render_tile with diagnostics enabled) also keep
+    // per-path vertices and records: smaller batches, at most ~2 GB of them.
+    const bool diag = mode == 0 && ctx->diag_on;
+    const int32_t slots = std::max(1, pr->max_depth);
+    if (diag) {
+        const int64_t per_path = (int64_t)diag::kMaxDepth * 32 + (int64_t)slots * 32 + 4;
+        target = std::min<int64_t>(target, std::max<int64_t>((int64_t)1 << 16, ((int64_t)2 << 30) / per_path));
+    }
     int64_t B = std::max<int64_t>(1, std::min<int64_t>(S, target / P));
     int rc = ensure_wave(ctx, P * B);
     if (rc) return rc;
     rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
     if (rc) return rc;
-    const SceneDev& SD = scene->dev;
+    SceneDev SD = scene->dev;
     WaveDev W = ctx->W;
+    const int32_t dsub = ctx->diag_sub;
+    const int32_t dsw = pr->tile_w / dsub, dsh = pr->tile_h / dsub;
+    if (diag) {
+        rc = ensure_diag_wave(ctx, P * B, slots);
+        if (rc) return rc;
+        W.dv0 = static_cast<float4*>(ctx->diag_allocs[0]);
+        W.dv1 = static_cast<float4*>(ctx->diag_allocs[1]);
+        W.dr0 = static_cast<float4*>(ctx->diag_allocs[2]);
+        W.dr1 = static_cast<float4*>(ctx->diag_allocs[3]);
+        W.dcnt = static_cast<int32_t*>(ctx->diag_allocs[4]);
+        W.dstride = ctx->diag_cap_paths;
+        // the film is rebuilt for this tile (pybind_renderer.hpp:206-208)
+        const int64_t fp = std::max<int64_t>(1, (int64_t)dsw * dsh);
+        rc = ensure_buf(ctx, &ctx->dfilm, &ctx->dfilm_cap, fp);
+        if (rc) return rc;
+        rc = ensure_buf(ctx, &ctx->dstrat, &ctx->dstrat_cap, fp * 28);
+        if (rc) return rc;
+        if (!ctx->dpattern) {
+            diag::PixelData pat;
+            diag::pixel_init(pat);
+            HIP_OK(hipMalloc((void**)&ctx->dpattern, sizeof(pat)));
+            HIP_OK(hipMemcpy(ctx->dpattern, &pat, sizeof(pat), hipMemcpyHostToDevice));
+            HIP_OK(hipMalloc((void**)&ctx->dtotal, sizeof(unsigned long long)));
+        }
+        const int64_t words = (int64_t)dsw * dsh * (int64_t)(sizeof(diag::PixelData) / 8);
+        const int64_t nst = (int64_t)dsw * dsh * 28;
+        const int64_t nin = std::max<int64_t>(std::max(words, nst), 1);
+        k_diag_init<<<blocks(nin, 256), 256, 0, st>>>(reinterpret_cast<uint64_t*>(ctx->dfilm), ctx->dpattern, words,
+                                                    ctx->dstrat, nst);
+        k_zero_u64<<<1, 64, 0, st>>>(ctx->dtotal);
+        ctx->diag_tw = pr->tile_w;
+        ctx->diag_th = pr->tile_h;
+        ctx->diag_valid = true;
+        // the diagnostic tracers use intersectScene directly, without
+        // volume-boundary skipping (path_tracer.hpp:893, 1314)
+        SD.any_vbo = 0;
+    } else {
+        W.dv0 = W.dv1 = W.dr0 = W.dr1 = nullptr;
+        W.dcnt = nullptr;
+        W.dstride = 0;
+    }
     // LDS stack sized to the scene's BVH depth; persistent grids of a few
     // workgroups per CU walk the live queues (grid-stride).
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
@@ -1044,6 +1370,13 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     int algo = pr->algorithm;
     if ((algo == LUCID_ALGO_VOLUME_SIMPLE || algo == LUCID_ALGO_VOLUME_MIS) && !scene->host.any_volume)
         algo = algo == LUCID_ALGO_VOLUME_SIMPLE ? LUCID_ALGO_SIMPLE : LUCID_ALGO_MIS;
+    // with diagnostics on, "simple" and "mis" record through their
+    // diagnostic tracers and every other algorithm through the NEE one
+    // (pybind_renderer.hpp:296-307)
+    if (diag)
+        algo = pr->algorithm == LUCID_ALGO_SIMPLE ? kAlgoDiagSimple
+               : pr->algorithm == LUCID_ALGO_MIS  ? kAlgoDiagMis
+                                                  : kAlgoDiagNee;
     for (int s0 = 0; s0 < S; s0 += (int)B) {
         // Pacing: wait for the wave before the previous one, so at most two
         // waves are queued and a cancel takes effect within one wave.
@@ -1079,6 +1412,15 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
                 case LUCID_ALGO_VOLUME_MIS:
                     k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
+                case kAlgoDiagSimple:
+                    k_shade<kAlgoDiagSimple><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                case kAlgoDiagNee:
+                    k_shade<kAlgoDiagNee><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
+                case kAlgoDiagMis:
+                    k_shade<kAlgoDiagMis><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    break;
                 default:
                     k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
                     break;
@@ -1091,6 +1433,10 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         }
         if (mode == 0) {
             k_film<<<blocks(P, 256), 256, 0, st>>>(W, ctx->film, (int32_t)P, b);
+            if (diag && dsw > 0 && dsh > 0)
+                k_diag_film<<<blocks((int64_t)dsw * dsh, 64), 64, 0, st>>>(
+                    W, ctx->dfilm, reinterpret_cast<diag::StrategyWelford*>(ctx->dstrat), ctx->dtotal, pr->tile_w,
+                    dsub, dsw, dsh, (int32_t)P, b);
         } else if (mode == 1) {
             k_samples_out<<<blocks(n, 256), 256, 0, st>>>(W, ro->samples_dev, (int32_t)P, b, s0, S);
         } else {
@@ -1170,6 +1516,10 @@ void lucid_destroy(lucid_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_wave(ctx);
+    free_diag_wave(ctx);
+    free_diag_film(ctx);
+    if (ctx->dpattern) (void)hipFree(ctx->dpattern);
+    if (ctx->dtotal) (void)hipFree(ctx->dtotal);
     if (ctx->film) (void)hipFree(ctx->film);
     if (ctx->out_dev) (void)hipFree(ctx->out_dev);
     (void)hipEventDestroy(ctx->ev0);
@@ -1233,6 +1583,7 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     D.root = ps.root;
     D.total_area = ps.total_area;
     std::memcpy(D.env, ps.env, sizeof(D.env));
+    D.native_start = ps.native_start;
     D.any_vbo = 0;
     for (const GMat& m : ps.mats)
         if (m.flags & MF_VOLUME_BOUNDARY) D.any_vbo = 1;
@@ -1437,6 +1788,103 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
         return LUCID_ERR_INVALID;
     ctx->traverse_groups = traverse_groups_per_cu;
     ctx->shade_groups = shade_groups_per_cu;
+    return LUCID_OK;
+}
+
+
+int lucid_set_diagnostics(lucid_ctx* ctx, const lucid_diag_config* cfg) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    if (cfg && cfg->enabled && cfg->subsample < 1) return fail(ctx, LUCID_ERR_INVALID, "subsample factor must be >= 1");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->diag_on = (cfg && cfg->enabled) ? 1 : 0;
+    ctx->diag_sub = (cfg && cfg->enabled) ? cfg->subsample : 1;
+    // enable_diagnostics / disable_diagnostics drop the film
+    // (pybind_renderer.hpp:893-908)
+    free_diag_film(ctx);
+    if (!ctx->diag_on) free_diag_wave(ctx);
+    return LUCID_OK;
+}
+
+int lucid_diag_get_info(lucid_ctx* ctx, lucid_diag_info* out) {
+    if (!ctx || !out) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->pixel_bytes = (int64_t)sizeof(diag::PixelData);
+    out->subsample = ctx->diag_sub;
+    if (!ctx->diag_valid) return LUCID_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    unsigned long long tot = 0;
+    HIP_OK(hipMemcpyAsync(&tot, ctx->dtotal, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    out->valid = 1;
+    out->width = ctx->diag_tw;
+    out->height = ctx->diag_th;
+    out->sampled_width = ctx->diag_tw / ctx->diag_sub;
+    out->sampled_height = ctx->diag_th / ctx->diag_sub;
+    out->total_recorded = (int64_t)tot;
+    return LUCID_OK;
+}
+
+namespace {
+int diag_range(lucid_ctx* ctx, int64_t first, int64_t count) {
+    if (!ctx->diag_valid) return fail(ctx, LUCID_ERR_INVALID, "no diagnostic film (render a tile with diagnostics on)");
+    const int64_t n = (int64_t)(ctx->diag_tw / ctx->diag_sub) * (ctx->diag_th / ctx->diag_sub);
+    if (first < 0 || count < 0 || first + count > n) return fail(ctx, LUCID_ERR_INVALID, "film pixel range out of bounds");
+    return LUCID_OK;
+}
+}  // namespace
+
+int lucid_diag_read_pixels(lucid_ctx* ctx, int64_t first, int64_t count, void* dst) {
+    if (!ctx || (count > 0 && !dst)) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = diag_range(ctx, first, count);
+    if (rc || count == 0) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemcpyAsync(dst, ctx->dfilm + first, (size_t)count * sizeof(diag::PixelData), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+int lucid_diag_write_pixels(lucid_ctx* ctx, int64_t first, int64_t count, const void* src) {
+    if (!ctx || (count > 0 && !src)) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = diag_range(ctx, first, count);
+    if (rc || count == 0) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemcpyAsync(ctx->dfilm + first, src, (size_t)count * sizeof(diag::PixelData), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+int lucid_diag_read_strategy(lucid_ctx* ctx, int64_t first, int64_t count, double* dst) {
+    if (!ctx || (count > 0 && !dst)) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = diag_range(ctx, first, count);
+    if (rc || count == 0) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemcpyAsync(dst, ctx->dstrat + first * 28, (size_t)count * 28 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+int lucid_diag_clear(lucid_ctx* ctx) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->diag_valid) return LUCID_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    const int64_t px = (int64_t)(ctx->diag_tw / ctx->diag_sub) * (ctx->diag_th / ctx->diag_sub);
+    const int64_t words = px * (int64_t)(sizeof(diag::PixelData) / 8), nst = px * 28;
+    const int64_t nin = std::max<int64_t>(std::max(words, nst), 1);
+    k_diag_init<<<blocks(nin, 256), 256, 0, ctx->stream>>>(reinterpret_cast<uint64_t*>(ctx->dfilm), ctx->dpattern,
+                                                         words, ctx->dstrat, nst);
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(ctx->dtotal);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
 }
 

@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "diag_host.h"
 #include "json_lite.h"
 #include "lucid_hip.h"
 
@@ -287,6 +288,9 @@ int algo_code(const std::string& a) {  // pybind_renderer.hpp:301-327
     return LUCID_ALGO_NEE;
 }
 
+// PyRenderer::TopGroupSortBy (pybind_renderer.hpp:1003-1006)
+enum class TopGroupSortBy { Variance, Mean };
+
 class Renderer {
 public:
     Renderer() { std::cerr << "[PyRenderer] HIP wavefront path tracer (liblucid_hip)\n"; }
@@ -353,6 +357,7 @@ public:
         if (tw == 0 || th == 0) return px;
         std::lock_guard<std::mutex> l(render_mu_);
         ensure_scene();
+        apply_diagnostics();
         lucid_render_params pr = params(tx, ty, tw, th, fw, fh, samples, offset, max_depth);
         int rc = lucid_render_tile(ctx_, scene_, &cam_, &pr, px.data());
         if (rc == LUCID_ERR_CANCELLED) {
@@ -428,11 +433,29 @@ public:
         return "Object_" + std::to_string(id);
     }
     std::string object_path_string(const std::vector<int32_t>& ids) const {
+        return object_path(ids, lucid::diag::kLightUnknown);
+    }
+    // object_path_string with the group's light type (pybind_renderer.hpp:1160-1253)
+    std::string object_path(const std::vector<int32_t>& ids, uint8_t light_type) const {
+        using namespace lucid::diag;
         if (ids.empty()) return "Camera";
         std::string path;
         size_t start = ids.size();
         int32_t last = ids[start - 1];
-        if (last <= -2) {
+        if (light_type == kLightEnvironment) {
+            path = "Environment";
+            if (last == -1) start--;
+        } else if (light_type == kLightEmissive) {
+            path = get_object_name(last);
+            start--;
+        } else if (light_type == kLightPoint || light_type == kLightArea || light_type == kLightDirectional ||
+                   light_type == kLightSpot) {
+            path = light_type == kLightPoint ? "PointLight"
+                   : light_type == kLightArea ? "AreaLight"
+                   : light_type == kLightDirectional ? "SunLight"
+                                                     : "SpotLight";
+            if (last <= -2) start--;
+        } else if (last <= -2) {
             path = "Light(?)";
             start--;
         } else if (last == -1) {
@@ -447,6 +470,161 @@ public:
         }
         path += " \xE2\x86\x92 Camera";
         return path;
+    }
+
+    // ---- diagnostics (pybind_renderer.hpp:885-1135) ----
+    struct PixelInfo {  // PyRenderer::PixelDiagnosticInfo
+        bool valid = false;
+        size_t sample_count = 0;
+        float variance = 0.0f;
+        size_t group_count = 0, outlier_count = 0;
+        std::vector<float> mean_rgb{0.0f, 0.0f, 0.0f};
+        std::vector<lucid::diag::GroupInfo> top_groups;
+    };
+    void enable_diagnostics(const lucid::diag::RecordingConfig& c) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        diag_cfg_ = c;
+        diag_on_ = true;
+        drop_film();
+    }
+    void disable_diagnostics() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        diag_on_ = false;
+        drop_film();
+    }
+    bool is_diagnostics_enabled() const { return diag_on_; }
+    void clear_diagnostics() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        if (ctx_) check(lucid_diag_clear(ctx_));
+    }
+    lucid::diag::GlobalStats get_diagnostic_stats() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return lucid::diag::GlobalStats{};
+        return lucid::diag::global_stats(f);
+    }
+    py::array_t<float> get_diagnostic_variance_map() {
+        std::vector<float> map;
+        {
+            std::lock_guard<std::mutex> l(render_mu_);
+            lucid::diag::FilmSource f;
+            if (film(f)) {
+                std::vector<float> var(f.sampled_w * f.sampled_h, 0.0f);
+                f.for_each([&](size_t i, const lucid::diag::PixelData& p) {
+                    var[i] = lucid::diag::total_variance(p.total);
+                });
+                map.assign(f.width * f.height, 0.0f);
+                for (size_t y = 0; y < f.height; ++y)
+                    for (size_t x = 0; x < f.width; ++x) {
+                        long long k = f.index_of(x, y);
+                        if (k >= 0) map[y * f.width + x] = var[(size_t)k];
+                    }
+            }
+        }
+        py::array_t<float> a((py::ssize_t)map.size());
+        if (!map.empty()) std::memcpy(a.mutable_data(), map.data(), map.size() * sizeof(float));
+        return a;
+    }
+    std::string export_diagnostic_json() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return "{}";
+        return lucid::diag::metadata_json(f);
+    }
+    std::vector<lucid::diag::GroupInfo> get_top_variance_groups(size_t count) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return {};
+        return lucid::diag::convert_refs(lucid::diag::top_groups(f, count, false));
+    }
+    PixelInfo get_pixel_diagnostic(size_t x, size_t y, const std::string& sort_by) {
+        using namespace lucid::diag;
+        PixelInfo info;
+        std::lock_guard<std::mutex> l(render_mu_);
+        FilmSource f;
+        if (!film(f)) return info;
+        const long long k = f.index_of(x, y);
+        if (k < 0) return info;
+        PixelData p;
+        check(lucid_diag_read_pixels(ctx_, k, 1, &p));
+        const size_t ng = (size_t)std::min<uint64_t>(p.num_groups, kMaxGroups);
+        // sort_by_mean / sort_by_variance reorder the pixel's groups in place
+        // (pixel_path_data.hpp:129-150)
+        if (sort_by == "mean")
+            std::sort(p.groups, p.groups + ng,
+                      [](const Group& a, const Group& b) { return mean_luminance(a.stats) > mean_luminance(b.stats); });
+        else
+            std::sort(p.groups, p.groups + ng,
+                      [](const Group& a, const Group& b) { return total_variance(a.stats) > total_variance(b.stats); });
+        check(lucid_diag_write_pixels(ctx_, k, 1, &p));
+        info.valid = true;
+        info.sample_count = (size_t)p.total_samples;
+        info.variance = total_variance(p.total);
+        info.group_count = ng;
+        info.outlier_count = (size_t)p.num_outliers;
+        info.mean_rgb = {p.total.mean.x, p.total.mean.y, p.total.mean.z};
+        for (size_t i = 0; i < ng; ++i) {
+            const Group& g = p.groups[i];
+            GroupInfo gi;
+            gi.pixel_x = x; gi.pixel_y = y;
+            gi.signature = signature(g);
+            gi.signature_heckbert = signature_heckbert(g);
+            for (int v = 0; v < g.depth; ++v) gi.object_ids.push_back(g.vertices[v].object_id);
+            gi.sample_count = g.stats.count;
+            gi.mean_luminance = mean_luminance(g.stats);
+            gi.variance_luminance = total_variance(g.stats);
+            gi.mean_rgb = {g.stats.mean.x, g.stats.mean.y, g.stats.mean.z};
+            gi.variance_rgb = {g.stats.variance.x, g.stats.variance.y, g.stats.variance.z};
+            gi.depth = g.depth;
+            gi.coarse_type = g.coarse_type;
+            gi.coarse_type_name = coarse_name(g.coarse_type);
+            gi.strategy = g.strategy;
+            gi.strategy_name = strategy_name(g.strategy);
+            gi.object_path = object_path(gi.object_ids, g.light_type);
+            gi.positions.push_back({cam_.pos[0], cam_.pos[1], cam_.pos[2]});
+            for (int v = 0; v < g.depth; ++v) {
+                gi.positions.push_back({g.positions[v].x, g.positions[v].y, g.positions[v].z});
+                gi.normals.push_back({g.normals[v].x, g.normals[v].y, g.normals[v].z});
+            }
+            info.top_groups.push_back(gi);
+        }
+        return info;
+    }
+    // Extensions: the device film as raw PixelPathData<16,4> bytes
+    // (sampled_h, sampled_w, 9576) and the FP64 per-strategy Welford
+    // (sampled_h, sampled_w, 4, 7) = {n, mean rgb, M2 rgb} per strategy
+    // (BSDF, NEE, MIS_BSDF, MIS_NEE).
+    py::array_t<uint8_t> get_diagnostic_film_bytes() {
+        std::vector<uint8_t> buf;
+        size_t sw = 0, sh = 0;
+        {
+            std::lock_guard<std::mutex> l(render_mu_);
+            lucid::diag::FilmSource f;
+            if (film(f)) {
+                sw = f.sampled_w; sh = f.sampled_h;
+                buf.resize(sw * sh * sizeof(lucid::diag::PixelData));
+                if (!buf.empty()) check(lucid_diag_read_pixels(ctx_, 0, (int64_t)(sw * sh), buf.data()));
+            }
+        }
+        py::array_t<uint8_t> a({(py::ssize_t)sh, (py::ssize_t)sw, (py::ssize_t)sizeof(lucid::diag::PixelData)});
+        if (!buf.empty()) std::memcpy(a.mutable_data(), buf.data(), buf.size());
+        return a;
+    }
+    py::array_t<double> get_diagnostic_strategy_stats() {
+        std::vector<double> buf;
+        size_t sw = 0, sh = 0;
+        {
+            std::lock_guard<std::mutex> l(render_mu_);
+            lucid::diag::FilmSource f;
+            if (film(f)) {
+                sw = f.sampled_w; sh = f.sampled_h;
+                buf.resize(sw * sh * 28);
+                if (!buf.empty()) check(lucid_diag_read_strategy(ctx_, 0, (int64_t)(sw * sh), buf.data()));
+            }
+        }
+        py::array_t<double> a({(py::ssize_t)sh, (py::ssize_t)sw, (py::ssize_t)4, (py::ssize_t)7});
+        if (!buf.empty()) std::memcpy(a.mutable_data(), buf.data(), buf.size() * sizeof(double));
+        return a;
     }
 
     std::string repr() const {
@@ -464,6 +642,48 @@ private:
     lucid_camera cam_{};
     std::string algorithm_ = "nee";
     std::vector<std::string> names_;
+    bool diag_on_ = false;
+    lucid::diag::RecordingConfig diag_cfg_;
+    bool diag_applied_ = false;
+    int diag_applied_sub_ = 0;
+
+    // Device diagnostics state follows diag_on_ / diag_cfg_ at render time.
+    void apply_diagnostics() {
+        const int sub = diag_cfg_.subsample_factor;
+        if (diag_on_ == diag_applied_ && (!diag_on_ || sub == diag_applied_sub_)) return;
+        lucid_diag_config c{diag_on_ ? 1 : 0, sub};
+        check(lucid_set_diagnostics(ctx_, &c));
+        diag_applied_ = diag_on_;
+        diag_applied_sub_ = sub;
+    }
+    void drop_film() {
+        if (!ctx_) return;
+        lucid_diag_config c{diag_on_ ? 1 : 0, diag_cfg_.subsample_factor ? diag_cfg_.subsample_factor : 1};
+        check(lucid_set_diagnostics(ctx_, &c));
+        diag_applied_ = diag_on_;
+        diag_applied_sub_ = c.subsample;
+    }
+    // The device film as a FilmSource, or false when there is none.
+    bool film(lucid::diag::FilmSource& f) {
+        if (!ctx_) return false;
+        lucid_diag_info inf{};
+        check(lucid_diag_get_info(ctx_, &inf));
+        if (!inf.valid) return false;
+        f.width = (size_t)inf.width; f.height = (size_t)inf.height;
+        f.sampled_w = (size_t)inf.sampled_width; f.sampled_h = (size_t)inf.sampled_height;
+        f.total_recorded = (size_t)inf.total_recorded;
+        f.config = diag_cfg_;
+        f.config.subsample_factor = (uint8_t)inf.subsample;
+        lucid_ctx* c = ctx_;
+        f.visit = [c](size_t first, size_t count,
+                      const std::function<void(size_t, const lucid::diag::PixelData&)>& fn) {
+            std::vector<lucid::diag::PixelData> buf(count);
+            int rc = lucid_diag_read_pixels(c, (int64_t)first, (int64_t)count, buf.data());
+            if (rc != LUCID_OK) throw std::runtime_error(std::string("lucidrenderer: ") + lucid_last_error(c));
+            for (size_t i = 0; i < count; ++i) fn(first + i, buf[i]);
+        };
+        return true;
+    }
 
     void require_ready() {
         if (!scene_loaded_ || !camera_set_) throw std::runtime_error("scene or camera not set");
@@ -557,7 +777,198 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("is_scene_loaded", &Renderer::is_scene_loaded)
         .def("is_camera_set", &Renderer::is_camera_set)
         .def("get_mesh_count", &Renderer::get_mesh_count)
+        // diagnostics (pybind_module.cpp:231-273)
+        .def("enable_diagnostics", &Renderer::enable_diagnostics, py::arg("config"),
+             "Enable diagnostic path recording with given configuration")
+        .def("disable_diagnostics", &Renderer::disable_diagnostics, "Disable diagnostic recording")
+        .def("is_diagnostics_enabled", &Renderer::is_diagnostics_enabled)
+        .def("clear_diagnostics", &Renderer::clear_diagnostics, "Clear all recorded diagnostic data")
+        .def("get_diagnostic_stats", &Renderer::get_diagnostic_stats, "Get global diagnostic statistics")
+        .def("get_diagnostic_variance_map", &Renderer::get_diagnostic_variance_map,
+             "Get per-pixel variance as 1D NumPy array (width * height)")
+        .def("export_diagnostic_json", &Renderer::export_diagnostic_json, "Export diagnostic data as JSON string")
+        .def("get_top_variance_groups", &Renderer::get_top_variance_groups, py::arg("count") = 10,
+             "Get top N path groups by variance")
+        .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
+             py::arg("sort_by") = "variance",
+             "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
+        .def("get_diagnostic_film_bytes", &Renderer::get_diagnostic_film_bytes,
+             "Extension: the film as PixelPathData<16,4> bytes, shape (sampled_h, sampled_w, 9576)")
+        .def("get_diagnostic_strategy_stats", &Renderer::get_diagnostic_strategy_stats,
+             "Extension: FP64 Welford per film pixel and strategy, shape (sampled_h, sampled_w, 4, 7)")
         .def("__repr__", &Renderer::repr);
+
+    using namespace lucid::diag;
+    py::class_<RecordingConfig>(m, "PathRecordingConfig")
+        .def(py::init<>(), "Create default recording configuration")
+        .def_readwrite("subsample_factor", &RecordingConfig::subsample_factor)
+        .def_readwrite("max_groups_per_pixel", &RecordingConfig::max_groups_per_pixel)
+        .def_readwrite("max_outliers_per_pixel", &RecordingConfig::max_outliers_per_pixel)
+        .def_readwrite("max_depth", &RecordingConfig::max_depth)
+        .def_readwrite("variance_threshold", &RecordingConfig::variance_threshold)
+        .def_readwrite("outlier_threshold", &RecordingConfig::outlier_threshold)
+        .def_static("minimal", &RecordingConfig::minimal)
+        .def_static("standard", &RecordingConfig::standard)
+        .def_static("detailed", &RecordingConfig::detailed)
+        .def("estimate_memory_bytes", &RecordingConfig::estimate_memory_bytes, py::arg("width"), py::arg("height"))
+        .def("__repr__", [](const RecordingConfig& c) {
+            return "<PathRecordingConfig subsample=" + std::to_string(c.subsample_factor) +
+                   " max_groups=" + std::to_string(c.max_groups_per_pixel) + ">";
+        });
+    py::class_<GroupInfo>(m, "ExportedGroupInfo")
+        .def(py::init<>())
+        .def_readonly("pixel_x", &GroupInfo::pixel_x)
+        .def_readonly("pixel_y", &GroupInfo::pixel_y)
+        .def_readonly("signature", &GroupInfo::signature)
+        .def_readonly("signature_heckbert", &GroupInfo::signature_heckbert)
+        .def_readonly("object_ids", &GroupInfo::object_ids)
+        .def_readonly("object_path", &GroupInfo::object_path)
+        .def_readonly("sample_count", &GroupInfo::sample_count)
+        .def_readonly("mean_luminance", &GroupInfo::mean_luminance)
+        .def_readonly("variance_luminance", &GroupInfo::variance_luminance)
+        .def_readonly("mean_rgb", &GroupInfo::mean_rgb)
+        .def_readonly("variance_rgb", &GroupInfo::variance_rgb)
+        .def_readonly("depth", &GroupInfo::depth)
+        .def_readonly("coarse_type", &GroupInfo::coarse_type)
+        .def_readonly("coarse_type_name", &GroupInfo::coarse_type_name)
+        .def_readonly("strategy_name", &GroupInfo::strategy_name)
+        .def_readonly("positions", &GroupInfo::positions)
+        .def_readonly("normals", &GroupInfo::normals)
+        .def("__repr__", [](const GroupInfo& g) {
+            return "<ExportedGroupInfo signature='" + g.signature + "' heckbert='" + g.signature_heckbert + "'" +
+                   " path='" + g.object_path + "'" + " strategy='" + g.strategy_name + "'" +
+                   " mean=" + std::to_string(g.mean_luminance) + " var=" + std::to_string(g.variance_luminance) +
+                   " positions=" + std::to_string(g.positions.size()) + ">";
+        });
+    py::enum_<TopGroupSortBy>(m, "TopGroupSortBy")
+        .value("Variance", TopGroupSortBy::Variance, "Sort by variance (noisy paths first)")
+        .value("Mean", TopGroupSortBy::Mean, "Sort by mean contribution (bright paths first)")
+        .export_values();
+    py::class_<Renderer::PixelInfo>(m, "PixelDiagnosticInfo")
+        .def(py::init<>())
+        .def_readonly("valid", &Renderer::PixelInfo::valid)
+        .def_readonly("sample_count", &Renderer::PixelInfo::sample_count)
+        .def_readonly("variance", &Renderer::PixelInfo::variance)
+        .def_readonly("group_count", &Renderer::PixelInfo::group_count)
+        .def_readonly("outlier_count", &Renderer::PixelInfo::outlier_count)
+        .def_readonly("mean_rgb", &Renderer::PixelInfo::mean_rgb)
+        .def_readonly("top_groups", &Renderer::PixelInfo::top_groups)
+        .def("__repr__", [](const Renderer::PixelInfo& p) {
+            if (!p.valid) return std::string("<PixelDiagnosticInfo invalid>");
+            return "<PixelDiagnosticInfo samples=" + std::to_string(p.sample_count) +
+                   " variance=" + std::to_string(p.variance) + " groups=" + std::to_string(p.group_count) + ">";
+        });
+    py::class_<Suggestion>(m, "DiagnosticSuggestion")
+        .def(py::init<>())
+        .def_readonly("category", &Suggestion::category)
+        .def_readonly("severity", &Suggestion::severity)
+        .def_readonly("message", &Suggestion::message)
+        .def_readonly("action", &Suggestion::action)
+        .def("__repr__", [](const Suggestion& s) {
+            return "<DiagnosticSuggestion [" + s.severity + "] " + s.message + ">";
+        });
+    py::class_<GlobalStats>(m, "GlobalDiagnosticStats")
+        .def(py::init<>())
+        .def_readonly("total_samples", &GlobalStats::total_samples)
+        .def_readonly("active_pixels", &GlobalStats::active_pixels)
+        .def_readonly("total_groups", &GlobalStats::total_groups)
+        .def_readonly("total_overflow", &GlobalStats::total_overflow)
+        .def_readonly("total_variance", &GlobalStats::total_variance);
+    py::class_<CoarseInfo>(m, "ExportedCoarseStats")
+        .def(py::init<>())
+        .def_readonly("type_id", &CoarseInfo::type_id)
+        .def_readonly("name", &CoarseInfo::name)
+        .def_readonly("sample_count", &CoarseInfo::sample_count)
+        .def_readonly("mean_luminance", &CoarseInfo::mean_luminance)
+        .def_readonly("variance_luminance", &CoarseInfo::variance_luminance);
+    py::class_<HostFilm>(m, "DiagnosticFilm")
+        .def(py::init<size_t, size_t, const RecordingConfig&>(), py::arg("width"), py::arg("height"),
+             py::arg("config"))
+        .def("width", &HostFilm::width)
+        .def("height", &HostFilm::height)
+        .def("sampled_width", &HostFilm::sampled_width)
+        .def("sampled_height", &HostFilm::sampled_height)
+        .def("total_paths_recorded", &HostFilm::total_paths_recorded)
+        .def("global_stats", [](const HostFilm& f) { return global_stats(f.source()); })
+        .def("metadata_json", [](const HostFilm& f) { return metadata_json(f.source()); })
+        .def("record_path", &HostFilm::record_path, py::arg("x"), py::arg("y"), py::arg("trace"))
+        .def("clear", &HostFilm::clear)
+        .def("pixel_bytes",
+             [](const HostFilm& f, size_t x, size_t y) {
+                 FilmSource src = f.source();
+                 long long k = src.index_of(x, y);
+                 if (k < 0) throw py::index_error("pixel outside the film");
+                 std::string b;
+                 src.visit((size_t)k, 1, [&](size_t, const PixelData& p) {
+                     b.assign(reinterpret_cast<const char*>(&p), sizeof(p));
+                 });
+                 return py::bytes(b);
+             },
+             py::arg("x"), py::arg("y"), "Extension: PixelPathData<16,4> bytes of the film pixel holding (x, y)");
+    struct HostTracer {
+        HostFilm film;
+        bool enabled = true;
+    };
+    py::class_<HostTracer>(m, "DiagnosticPathTracer")
+        .def(py::init([](size_t w, size_t h, const RecordingConfig& c) { return HostTracer{HostFilm(w, h, c), true}; }),
+             py::arg("width"), py::arg("height"), py::arg("config"))
+        .def("is_enabled", [](const HostTracer& t) { return t.enabled; })
+        .def("set_enabled", [](HostTracer& t, bool e) { t.enabled = e; })
+        .def("film", [](HostTracer& t) -> HostFilm& { return t.film; }, py::return_value_policy::reference_internal);
+    struct Exporter {
+        const HostFilm* film;
+    };
+    py::class_<Exporter>(m, "DiagnosticExporter")
+        .def(py::init([](const HostFilm& f) { return Exporter{&f}; }), py::arg("film"), py::keep_alive<1, 2>())
+        .def("get_global_stats", [](const Exporter& e) { return global_stats(e.film->source()); })
+        .def("get_top_variance_groups",
+             [](const Exporter& e, size_t n) { return convert_refs(top_groups(e.film->source(), n, false)); },
+             py::arg("n") = 10)
+        .def("get_top_mean_groups",
+             [](const Exporter& e, size_t n) { return convert_refs(top_groups(e.film->source(), n, true)); },
+             py::arg("n") = 10)
+        .def("get_coarse_stats", [](const Exporter& e) { return coarse_stats(e.film->source()); })
+        .def("generate_suggestions", [](const Exporter& e) { return suggestions(e.film->source()); })
+        .def("export_metadata_json", [](const Exporter& e) { return metadata_json(e.film->source()); })
+        .def("export_binary", [](const Exporter& e) { return py::bytes(export_binary(e.film->source())); });
+    py::enum_<BsdfType>(m, "BsdfType")
+        .value("Diffuse", kDiffuse)
+        .value("Glossy", kGlossy)
+        .value("Mirror", kMirror)
+        .value("Glass", kGlass)
+        .value("Emission", kEmission)
+        .value("Environment", kEnvironment)
+        .export_values();
+    py::enum_<LightSource>(m, "LightSourceType")
+        .value("Unknown", kLightUnknown)
+        .value("Point", kLightPoint)
+        .value("Area", kLightArea)
+        .value("Directional", kLightDirectional)
+        .value("Spot", kLightSpot)
+        .value("Environment", kLightEnvironment)
+        .value("Emissive", kLightEmissive)
+        .export_values();
+    py::class_<HostTrace>(m, "PathTrace")
+        .def(py::init<>(), "Create empty path trace")
+        .def_property_readonly("depth", [](const HostTrace& t) { return (size_t)t.n; })
+        .def("get_contribution", [](const HostTrace& t) { return std::vector<float>{t.c.x, t.c.y, t.c.z}; });
+    py::class_<HostRecorder>(m, "PathDiagnosticRecorder")
+        .def(py::init<>())
+        .def("begin_path", &HostRecorder::begin_path)
+        .def("record_vertex",
+             [](HostRecorder& r, int32_t obj, int16_t mat, BsdfType t, bool delta, bool ls) {
+                 r.record_vertex(obj, mat, (uint8_t)t, delta, ls);
+             },
+             py::arg("object_id"), py::arg("material_id"), py::arg("bsdf_type"), py::arg("is_delta"),
+             py::arg("is_light_sampled"))
+        .def("record_environment_hit", &HostRecorder::record_environment_hit)
+        .def("record_light_hit", [](HostRecorder& r, int32_t id, LightSource t) { r.record_light_hit(id, (uint8_t)t); },
+             py::arg("light_id"), py::arg("light_source_type") = kLightUnknown)
+        .def("record_emissive_hit", &HostRecorder::record_emissive_hit, py::arg("object_id"), py::arg("material_id"))
+        .def("current_depth", &HostRecorder::current_depth)
+        .def("end_path", &HostRecorder::end_path, py::arg("r"), py::arg("g"), py::arg("b"))
+        .def("end_path_lum", [](HostRecorder& r, float l) { return r.end_path(l, l, l); }, py::arg("luminance"));
+
     m.attr("__version__") = "1.0.0";
     m.attr("openmp_enabled") = false;
     m.attr("dawn_enabled") = false;

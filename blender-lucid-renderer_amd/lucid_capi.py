@@ -65,6 +65,16 @@ class Stats(C.Structure):
 
 
 # name -> (restype, argtypes); every symbol include/lucid_hip.h declares
+class DiagConfig(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("subsample", C.c_int32)]
+
+
+class DiagInfo(C.Structure):
+    _fields_ = [("valid", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("sampled_width", C.c_int32),
+                ("sampled_height", C.c_int32), ("subsample", C.c_int32), ("total_recorded", C.c_int64),
+                ("pixel_bytes", C.c_int64)]
+
+
 SIGNATURES = {
     "lucid_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "lucid_destroy": (None, [C.c_void_p]),
@@ -90,6 +100,12 @@ SIGNATURES = {
     "lucid_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
     "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "lucid_set_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(DiagConfig)]),
+    "lucid_diag_get_info": (C.c_int, [C.c_void_p, C.POINTER(DiagInfo)]),
+    "lucid_diag_read_pixels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "lucid_diag_write_pixels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "lucid_diag_read_strategy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
+    "lucid_diag_clear": (C.c_int, [C.c_void_p]),
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,

@@ -198,6 +198,51 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
    backend fixes workgroup counts per dispatch). */
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
 
+/* ------------------------------------------------------------------------
+   DiagnosticFilm (diagnostics/diagnostic_film.hpp, pixel_path_data.hpp).
+   With diagnostics enabled, lucid_render_tile / lucid_render_tile_device
+   trace through the diagnostic tracers (traceSimpleWithDiagnostics for
+   "simple", traceMISWithDiagnostics for "mis", traceNEEWithDiagnostics for
+   every other algorithm; path_tracer.hpp:880-1514) and rebuild a film sized
+   to the tile (pybind_renderer.hpp:206-208, 296-317).  Film pixel (fx, fy)
+   holds PixelPathData<16, 4> in the reference's byte layout (9576 bytes) and
+   an FP64 Welford {n, mean[3], M2[3]} per sampling strategy (BSDF, NEE,
+   MIS_BSDF, MIS_NEE) over each path's per-strategy contribution.  Records of
+   tile pixels (fx*sub+dx, fy*sub+dy) are applied sample by sample, the
+   block's pixels in row-major order within a sample.
+   ------------------------------------------------------------------------ */
+typedef struct lucid_diag_config {
+    int32_t enabled;    /* 0: render_tile records nothing */
+    int32_t subsample;  /* PathRecordingConfig::subsample_factor, >= 1 */
+} lucid_diag_config;
+
+typedef struct lucid_diag_info {
+    int32_t valid;                         /* a diagnostic render built the film */
+    int32_t width, height;                 /* tile the film was built for */
+    int32_t sampled_width, sampled_height; /* width / subsample, height / subsample */
+    int32_t subsample;
+    int64_t total_recorded;                /* DiagnosticFilm::total_paths_recorded */
+    int64_t pixel_bytes;                   /* bytes per film pixel (9576) */
+} lucid_diag_info;
+
+/* PyRenderer::enable_diagnostics / disable_diagnostics
+   (pybind_renderer.hpp:893-908): cfg NULL or enabled = 0 disables; both drop
+   the current film. */
+int lucid_set_diagnostics(lucid_ctx* ctx, const lucid_diag_config* cfg);
+int lucid_diag_get_info(lucid_ctx* ctx, lucid_diag_info* out);
+/* Film pixels [first, first + count) in row-major sampled order, as
+   PixelPathData<16, 4> bytes (DiagnosticFilm::pixel_data, :118-127).  Write
+   stores host-side edits back (PyRenderer::get_pixel_diagnostic sorts a
+   pixel's groups in place, pybind_renderer.hpp:1022-1031). */
+int lucid_diag_read_pixels(lucid_ctx* ctx, int64_t first, int64_t count, void* dst);
+int lucid_diag_write_pixels(lucid_ctx* ctx, int64_t first, int64_t count, const void* src);
+/* Per-strategy FP64 Welford of film pixels [first, first + count): 28
+   doubles per pixel (4 strategies x {n, mean[3], M2[3]}).  No counterpart in
+   the reference (its film keeps float statistics only). */
+int lucid_diag_read_strategy(lucid_ctx* ctx, int64_t first, int64_t count, double* dst);
+/* DiagnosticFilm::clear (diagnostic_film.hpp:305-312). */
+int lucid_diag_clear(lucid_ctx* ctx);
+
 /* Runs the device BVH traversal code on the CPU over a scene packed from
    `meshes` (no device needed), checking every node and triangle index before
    use: closest hit (any_hit = 0) or occlusion (any_hit = 1) of n_rays rays.

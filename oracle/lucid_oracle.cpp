@@ -1566,6 +1566,360 @@ static Ray camera_ray(const RayGen& g, int gx, int gy, uint32_t sidx, int sample
     return r;
 }
 
+// ------------------------------------------------------------ diagnostics --
+// Path records of the diagnostic tracers (traceSimpleWithDiagnostics,
+// traceNEEWithDiagnostics, traceMISWithDiagnostics; path_tracer.hpp:880-1514)
+// and the per-pixel statistics of PixelPathData
+// (diagnostics/pixel_path_data.hpp), restated over std::vector and written
+// out at the end in the reference's byte layout for comparison.
+enum DBsdf : uint8_t { DB_DIFFUSE = 0, DB_GLOSSY = 1, DB_MIRROR = 2, DB_GLASS = 3, DB_EMISSION = 4, DB_ENV = 5 };
+enum DLight : uint8_t { DL_UNKNOWN = 0, DL_POINT = 1, DL_AREA = 2, DL_DIRECTIONAL = 3, DL_SPOT = 4, DL_ENV = 5,
+                        DL_EMISSIVE = 6 };
+enum DStrat : uint8_t { DS_BSDF = 0, DS_NEE = 1, DS_MIS_BSDF = 2, DS_MIS_NEE = 3 };
+static const size_t D_MAX_DEPTH = 16;  // path_types.hpp:22
+
+struct DVert {  // PathVertex, path_types.hpp:178-243, plus its geometry
+    int32_t obj = -1;
+    int16_t mat = -1;
+    uint8_t type = DB_DIFFUSE, flags = 0;
+    V3 pos, nrm;
+};
+struct DRec {  // one completed path (PathTrace)
+    std::vector<DVert> v;
+    V3 contrib;
+    uint8_t light_type = DL_UNKNOWN, strategy = DS_BSDF;
+};
+// PathDiagnosticRecorder, diagnostic_integrator.hpp:87-389
+struct DRecorder {
+    std::vector<DVert> cur;
+    std::vector<DRec> done;
+    void vertex(int32_t obj, uint8_t type, bool delta, V3 pos, V3 nrm) {  // :112-128
+        if (cur.size() >= D_MAX_DEPTH) return;
+        DVert x;
+        x.obj = obj; x.mat = 0; x.type = type; x.flags = delta ? 1 : 0; x.pos = pos; x.nrm = nrm;
+        cur.push_back(x);
+    }
+    void complete(uint8_t strat, int32_t light_id, uint8_t ltype, V3 c, V3 lpos) {  // :338-368
+        DRec r;
+        r.v = cur;
+        if (r.v.size() < D_MAX_DEPTH) {
+            DVert x;
+            x.obj = ltype == DL_ENV ? -1 : -(light_id + 2);
+            x.mat = -1; x.type = DB_EMISSION; x.flags = 0; x.pos = lpos; x.nrm = v3(0, 0, 0);
+            r.v.push_back(x);
+        }
+        r.contrib = c; r.light_type = ltype; r.strategy = strat;
+        done.push_back(r);
+    }
+};
+// classify_bsdf / is_delta_bsdf, diagnostic_integrator.hpp:24-60
+static uint8_t d_classify(const MatParams& m) {
+    if (m.transmission > 0.5f) return DB_GLASS;
+    if (m.roughness < 0.01f && m.metallic > 0.5f) return DB_MIRROR;
+    if (m.metallic > 0.5f || m.roughness < 0.5f) return DB_GLOSSY;
+    return DB_DIFFUSE;
+}
+static bool d_is_delta(const MatParams& m) {
+    if (m.transmission > 0.5f && m.roughness < 0.01f) return true;
+    if (m.metallic > 0.5f && m.roughness < 0.01f) return true;
+    return false;
+}
+// to_light_source_type, path_types.hpp:160-169
+static uint8_t d_light_type(int t) {
+    switch (t) {
+        case L_EMISSIVE: return DL_AREA;
+        case L_POINT: return DL_POINT;
+        case L_SUN: return DL_DIRECTIONAL;
+        case L_SPOT: return DL_SPOT;
+        case L_AREA: return DL_AREA;
+        default: return DL_UNKNOWN;
+    }
+}
+static V3 d_far(const Ray& r) { return v3(r.o.x + r.d.x * 1000.0f, r.o.y + r.d.y * 1000.0f, r.o.z + r.d.z * 1000.0f); }
+
+// The three diagnostic tracers.  kind: 0 simple, 1 NEE, 2 MIS.  They use
+// intersectScene directly (no volume-boundary skipping) for camera,
+// extension and shadow rays.
+static V3 trace_diag(const Scene& s, int kind, Ray ray, int max_depth, Rng& rng, DRecorder& rec) {
+    const SceneLights& sl = s.sl;
+    V3 result, thr = v3(1, 1, 1);
+    float last_pdf = 0.0f, tmin = 0.0f;
+    for (int depth = 0; depth < max_depth; ++depth) {
+        Hit hit = intersect_scene(s, ray, tmin);
+        LightHit lh = intersect_native_lights(s, ray, depth);
+        if (lh.hit && (!hit.hit || lh.t < hit.t)) {
+            if (kind == 1 && depth != 0) break;  // :1060-1091
+            uint8_t lt = DL_UNKNOWN;
+            if (lh.light_index >= 0 && lh.light_index < (int)s.native.size()) lt = d_light_type(s.native[lh.light_index].type);
+            V3 lpos = v3(ray.o.x + ray.d.x * lh.t, ray.o.y + ray.d.y * lh.t, ray.o.z + ray.d.z * lh.t);
+            V3 c = cmul(lh.emission, thr);
+            if (kind == 2) {  // :1282-1322
+                float lp = 0.0f;
+                if (!sl.lights.empty() && lh.light_index >= 0) {
+                    int sli = find_native_light_index(sl, lh.light_index);
+                    if (sli >= 0) {
+                        float spb = light_select_pdf(sl, sli);
+                        if (spb > 0.0f) lp = pdf_light(sl.lights[sli], ray.o, lh.point, lh.normal) * spb;
+                    }
+                }
+                float mw = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
+                c = mul(c, mw);
+            }
+            rec.complete(kind == 2 ? DS_MIS_BSDF : DS_BSDF, lh.light_index, lt, c, lpos);
+            acc(result, c);
+            break;
+        }
+        if (!hit.hit) {
+            if (kind == 1 && depth != 0) break;  // :1094-1116
+            V3 c = cmul(env_color(s), thr);
+            rec.complete(kind == 2 ? DS_MIS_BSDF : DS_BSDF, -1, DL_ENV, c, d_far(ray));
+            acc(result, c);
+            break;
+        }
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
+        V3 em = s.meshes[hit.mesh].mat.emission;
+        uint8_t bt = d_classify(mat);
+        bool dl = d_is_delta(mat);
+        bool has_em = is_emissive(em);
+        V3 n = hit.normal, wo = neg(ray.d);
+        bool front = dot(wo, n) > 0;
+        V3 sn = n;
+        if (mat.transmission < 0.5f && !front) sn = neg(n);
+        if (kind == 0) {  // :949-985
+            if (has_em) {
+                V3 c = cmul(em, thr);
+                rec.complete(DS_BSDF, hit.mesh, DL_EMISSIVE, c, hit.point);
+                acc(result, c);
+                break;
+            }
+            rec.vertex(hit.mesh, bt, dl, hit.point, hit.normal);
+        } else {
+            if (has_em) {
+                if (kind == 1 && depth == 0) {  // :1131-1146
+                    V3 c = cmul(em, thr);
+                    rec.complete(DS_BSDF, hit.mesh, DL_EMISSIVE, c, hit.point);
+                    acc(result, c);
+                } else if (kind == 2) {  // :1386-1406
+                    float mw = 1.0f;
+                    if (!sl.lights.empty() && last_pdf > MIN_PDF) {
+                        int li = s.tri_light[hit.mesh][hit.tri];
+                        if (li >= 0) {
+                            float spb = light_select_pdf(sl, li);
+                            if (spb > 0.0f) {
+                                float lp = pdf_light(sl.lights[li], ray.o, hit.point, hit.normal) * spb;
+                                if (lp > MIN_PDF) mw = power_heuristic(last_pdf, lp);
+                            }
+                        }
+                    }
+                    V3 c = mul(cmul(em, thr), mw);
+                    rec.complete(DS_MIS_BSDF, hit.mesh, DL_EMISSIVE, c, hit.point);
+                    acc(result, c);
+                }
+            } else {
+                rec.vertex(hit.mesh, bt, dl, hit.point, hit.normal);
+                bool transm = mat.transmission > 0.5f;
+                if (!sl.lights.empty() && !transm) {
+                    float sp;
+                    int li = select_light(sl, randf(rng), sp);
+                    if (li >= 0 && sp > 1e-6f) {
+                        const Light& l = sl.lights[li];
+                        float a1 = randf(rng), a2 = randf(rng);
+                        LSample ls = sample_light(l, hit.point, a1, a2);
+                        if (ls.pdf > MIN_PDF) {
+                            float nl = dot(sn, ls.dir);
+                            if (nl > 1e-6f) {
+                                Ray sr; sr.o = hit.point; sr.d = ls.dir;
+                                if (!intersect_scene(s, sr, RAY_T_MIN, ls.dist).hit) {
+                                    V3 f = eval_bsdf(mat, wo, ls.dir, sn);
+                                    float pl = ls.pdf * sp;
+                                    V3 nee;
+                                    if (kind == 2) {
+                                        float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                                        float mw = power_heuristic(pl, pb);
+                                        nee = cmul(mul(bsdf_weight(f, nl, pl), mw), ls.emission);
+                                    } else {
+                                        nee = cmul(bsdf_weight(f, nl, pl), ls.emission);
+                                    }
+                                    V3 c = cmul(thr, nee);
+                                    rec.complete(kind == 2 ? DS_MIS_NEE : DS_NEE, li - sl.native_start,
+                                                 d_light_type(l.type), c, ls.pos);
+                                    acc(result, c);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        V3 smp_n = (mat.transmission > 0.5f) ? n : sn;
+        float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
+        BSample bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+        if (!bs.valid()) break;
+        V3 w;
+        if (!throughput_update(bs, smp_n, w)) break;
+        thr = cmul(thr, w);
+        if (kind == 2) last_pdf = mis_pdf_next(bs);
+        if (depth >= 3) {
+            float mx = std::max({thr.x, thr.y, thr.z});
+            float rr = fmin_std(mx, 0.95f);
+            if (randf(rng) > rr) break;
+            thr = mul(thr, 1.0f / rr);
+        }
+        if (!std::isfinite(thr.x) || !std::isfinite(thr.y) || !std::isfinite(thr.z)) break;
+        ray.o = hit.point; ray.d = bs.wi;
+        tmin = RAY_T_MIN;
+    }
+    return result;
+}
+
+// PathTrace::hash (FNV-1a over depth and every vertex field, each widened
+// to uint64) and coarse_type, path_types.hpp:321-372
+static uint64_t d_hash(const std::vector<DVert>& v) {
+    uint64_t h = 14695981039346656037ULL;
+    h ^= (uint64_t)v.size();
+    h *= 1099511628211ULL;
+    for (const DVert& x : v) {
+        h ^= (uint64_t)(int64_t)x.obj; h *= 1099511628211ULL;
+        h ^= (uint64_t)(int64_t)x.mat; h *= 1099511628211ULL;
+        h ^= (uint64_t)x.type; h *= 1099511628211ULL;
+        h ^= (uint64_t)x.flags; h *= 1099511628211ULL;
+    }
+    return h;
+}
+static uint8_t d_coarse(const std::vector<DVert>& v) {
+    uint8_t t = 0;
+    for (const DVert& x : v) {
+        if (x.flags & 2) t |= 1;
+        if (x.flags & 1) t |= 2;
+    }
+    if (v.size() > 2) t |= 4;
+    return t;
+}
+// PathStatistics::add_sample, path_stats.hpp:63-98 (float Welford, n-1)
+struct DStats {
+    uint32_t n = 0;
+    float mean[3] = {0, 0, 0}, m2[3] = {0, 0, 0}, var[3] = {0, 0, 0};
+    void add(V3 c) {
+        const float x[3] = {c.x, c.y, c.z};
+        ++n;
+        if (n == 1) {
+            for (int k = 0; k < 3; ++k) { mean[k] = x[k]; m2[k] = 0.0f; var[k] = 0.0f; }
+            return;
+        }
+        float d1[3], d2[3];
+        for (int k = 0; k < 3; ++k) d1[k] = x[k] - mean[k];
+        for (int k = 0; k < 3; ++k) mean[k] += d1[k] / (float)n;
+        for (int k = 0; k < 3; ++k) d2[k] = x[k] - mean[k];
+        for (int k = 0; k < 3; ++k) m2[k] += d1[k] * d2[k];
+        float nm1 = (float)(n - 1);
+        for (int k = 0; k < 3; ++k) var[k] = m2[k] / nm1;
+    }
+    float total_var() const { return var[0] + var[1] + var[2]; }
+};
+// PathGroup, path_stats.hpp:133-268: fixed arrays, init_from copies only
+// the first `depth` vertices (later ones keep what was there).
+struct DGroup {
+    uint64_t hash = 0;
+    DStats st;
+    uint8_t depth = 0, coarse = 0, ltype = DL_UNKNOWN, strat = DS_BSDF;
+    DVert v[D_MAX_DEPTH];
+    void init_from(const DRec& r) {
+        hash = d_hash(r.v);
+        depth = (uint8_t)std::min(r.v.size(), D_MAX_DEPTH);
+        coarse = d_coarse(r.v);
+        ltype = r.light_type;
+        strat = r.strategy;
+        for (size_t i = 0; i < depth; ++i) v[i] = r.v[i];
+        st = DStats();
+        st.add(r.contrib);
+    }
+};
+// PixelPathData<G, O>::record_path, pixel_path_data.hpp:53-224
+struct DPixel {
+    size_t cap_g = 16, cap_o = 4;
+    std::vector<DGroup> g;
+    size_t n_g = 0, n_o = 0, total = 0, overflow = 0;
+    DStats coarse[8], tot;
+    std::vector<uint64_t> o_hash;
+    std::vector<V3> o_c;
+    DPixel(size_t cg = 16, size_t co = 4) : cap_g(cg), cap_o(co), g(cg), o_hash(co, 0), o_c(co) {}
+    void record(const DRec& r) {
+        ++total;
+        tot.add(r.contrib);
+        coarse[d_coarse(r.v)].add(r.contrib);
+        if (total >= 10) {  // is_outlier, :172-189 (3 sigma on any channel)
+            const float x[3] = {r.contrib.x, r.contrib.y, r.contrib.z};
+            bool out = false;
+            for (int k = 0; k < 3; ++k) {
+                float sd = std::sqrt(tot.var[k] + 1e-10f);
+                if (std::fabs(x[k] - tot.mean[k]) > 3.0f * sd) out = true;
+            }
+            if (out) {
+                size_t i = n_o % cap_o;
+                o_hash[i] = d_hash(r.v);
+                o_c[i] = r.contrib;
+                ++n_o;
+            }
+        }
+        uint64_t h = d_hash(r.v);
+        for (size_t i = 0; i < n_g; ++i) {
+            if (g[i].hash == h) { g[i].st.add(r.contrib); return; }
+        }
+        if (n_g < cap_g) {
+            g[n_g].init_from(r);
+            ++n_g;
+            return;
+        }
+        size_t mi = 0;  // evict_and_add, :203-224
+        float mv = g[0].st.total_var();
+        for (size_t i = 1; i < n_g; ++i) {
+            float vv = g[i].st.total_var();
+            float adj = vv * std::log2((float)(g[i].st.n + 1));
+            float madj = mv * std::log2((float)(g[mi].st.n + 1));
+            if (adj < madj) { mi = i; mv = vv; }
+        }
+        ++overflow;
+        g[mi].init_from(r);
+    }
+};
+// The film pixel in PixelPathData<16, 4>'s byte layout (9576 bytes).
+static const size_t D_PIXEL_BYTES = 9576;
+static void d_put_stats(unsigned char* p, const DStats& s) {
+    std::memcpy(p, &s.n, 4);
+    std::memcpy(p + 4, s.mean, 12);
+    std::memcpy(p + 16, s.m2, 12);
+    std::memcpy(p + 28, s.var, 12);
+}
+static void d_serialise(const DPixel& px, unsigned char* out) {
+    std::memset(out, 0, D_PIXEL_BYTES);
+    for (size_t i = 0; i < 16; ++i) {
+        unsigned char* q = out + i * 568;
+        const DGroup& G = px.g[i];
+        std::memcpy(q, &G.hash, 8);
+        d_put_stats(q + 8, G.st);
+        q[48] = G.depth; q[49] = G.coarse; q[50] = G.ltype; q[51] = G.strat;
+        for (size_t k = 0; k < D_MAX_DEPTH; ++k) {
+            std::memcpy(q + 56 + k * 8, &G.v[k].obj, 4);
+            std::memcpy(q + 60 + k * 8, &G.v[k].mat, 2);
+            q[62 + k * 8] = G.v[k].type;
+            q[63 + k * 8] = G.v[k].flags;
+            const float pp[3] = {G.v[k].pos.x, G.v[k].pos.y, G.v[k].pos.z};
+            const float nn[3] = {G.v[k].nrm.x, G.v[k].nrm.y, G.v[k].nrm.z};
+            std::memcpy(q + 184 + k * 12, pp, 12);
+            std::memcpy(q + 376 + k * 12, nn, 12);
+        }
+    }
+    for (int t = 0; t < 8; ++t) d_put_stats(out + 9088 + t * 40, px.coarse[t]);
+    for (size_t i = 0; i < 4; ++i) {
+        std::memcpy(out + 9408 + i * 24, &px.o_hash[i], 8);
+        const float c[3] = {px.o_c[i].x, px.o_c[i].y, px.o_c[i].z};
+        std::memcpy(out + 9416 + i * 24, c, 12);
+    }
+    const uint64_t cnt[4] = {px.n_g, px.n_o, px.total, px.overflow};
+    std::memcpy(out + 9504, cnt, 32);
+    d_put_stats(out + 9536, px.tot);
+}
+
 }  // namespace orc
 
 // =========================================================== C interface ==
@@ -2014,6 +2368,133 @@ int orc_select_light(void* sp, float u, float* prob) { return select_light(stati
 float orc_pdf_light_area(float area, const float* p, const float* lp, const float* ln) {
     Light l; l.area = area;
     return pdf_light(l, v3(p[0], p[1], p[2]), v3(lp[0], lp[1], lp[2]), v3(ln[0], ln[1], ln[2]));
+}
+
+// ----------------------------------------------------------- diagnostics --
+// render_tile with diagnostics enabled (pybind_renderer.hpp:206-208,
+// 296-317): the diagnostic tracer of the algorithm ("simple", "mis", any
+// other name -> NEE), radiance as render_tile, and a DiagnosticFilm sized to
+// the tile with subsample factor `sub`.  Film pixel (fx, fy) receives the
+// records of the tile pixels (fx*sub+dx, fy*sub+dy), applied sample by
+// sample, and within a sample in row-major (dy, dx) order, each path's
+// records in the order it produced them.  Alongside, an FP64 Welford
+// (n, mean[3], M2[3]) per film pixel and sampling strategy over the
+// per-path strategy sums.
+// film: sw*sh pixels of 9576 bytes; strat: sw*sh*4*7 doubles.
+int orc_render_tile_diag(void* p, const char* algorithm, int tx, int ty, int tw, int th, int fw, int fh, int samples,
+                         int sample_offset, int max_depth, int sub, int nthreads, float* out, unsigned char* film,
+                         double* strat, long long* total_recorded) {
+    Scene& s = *static_cast<Scene*>(p);
+    if (sub < 1) return 2;
+    std::string a = algorithm ? algorithm : "";
+    const int kind = a == "simple" ? 0 : (a == "mis" ? 2 : 1);
+    std::vector<float> px((size_t)tw * th * 4, 0.0f);
+    const int sw = tw / sub, sh = th / sub;
+    if (!s.cam.set) {
+        std::memcpy(out, px.data(), px.size() * sizeof(float));
+        return 1;
+    }
+    RayGen g = make_raygen(s, fw, fh);
+    const float sens = std::pow(2.0f, s.cam.exposure_ev);
+    std::vector<std::vector<DRec>> recs((size_t)tw * th * samples);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int pi = 0; pi < tw * th; ++pi) {
+        const int pxi = pi % tw, pyi = pi / tw;
+        V3 radf;
+        for (int si = 0; si < samples; ++si) {
+            Rng rng;
+            Ray r = camera_ray(g, tx + pxi, ty + pyi, (uint32_t)(sample_offset + si), samples, rng);
+            DRecorder rec;
+            V3 L = trace_diag(s, kind, r, max_depth, rng, rec);
+            radf = add(radf, L);
+            recs[(size_t)pi * samples + si] = std::move(rec.done);
+        }
+        V3 col = v3(radf.x * sens, radf.y * sens, radf.z * sens);
+        col = v3(fmax_std(col.x, 0.0f), fmax_std(col.y, 0.0f), fmax_std(col.z, 0.0f));
+        px[(size_t)pi * 4 + 0] = col.x; px[(size_t)pi * 4 + 1] = col.y; px[(size_t)pi * 4 + 2] = col.z;
+        px[(size_t)pi * 4 + 3] = 1.0f;
+    }
+    long long total = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+    for (int fi = 0; fi < sw * sh; ++fi) {
+        const int fx = fi % sw, fy = fi / sw;
+        DPixel d;
+        double w[4][7] = {{0}};
+        for (int si = 0; si < samples; ++si) {
+            for (int dy = 0; dy < sub; ++dy) {
+                for (int dx = 0; dx < sub; ++dx) {
+                    const size_t pi = (size_t)(fy * sub + dy) * tw + (fx * sub + dx);
+                    double sum[4][3] = {{0}};
+                    for (const DRec& r : recs[pi * samples + si]) {
+                        d.record(r);
+                        ++total;
+                        sum[r.strategy][0] += r.contrib.x; sum[r.strategy][1] += r.contrib.y;
+                        sum[r.strategy][2] += r.contrib.z;
+                    }
+                    for (int k = 0; k < 4; ++k) {
+                        double* W = w[k];
+                        W[0] += 1.0;
+                        for (int c = 0; c < 3; ++c) {
+                            double dl = sum[k][c] - W[1 + c];
+                            W[1 + c] += dl / W[0];
+                            W[4 + c] += dl * (sum[k][c] - W[1 + c]);
+                        }
+                    }
+                }
+            }
+        }
+        if (film) d_serialise(d, film + (size_t)fi * D_PIXEL_BYTES);
+        if (strat) std::memcpy(strat + (size_t)fi * 28, w, sizeof(w));
+    }
+    if (total_recorded) *total_recorded = total;
+    for (int y = 0; y < th; ++y)
+        std::memcpy(out + (size_t)(th - 1 - y) * tw * 4, px.data() + (size_t)y * tw * 4, (size_t)tw * 4 * sizeof(float));
+    return 0;
+}
+
+// PixelPathData<max_groups, max_outliers> driven by explicit records, for
+// the known-answer tests of the reference's diagnostics suite.  Record i has
+// depth[i] vertices (obj/mat/type/flags at i*16 + k), contribution
+// contrib[3i..], light type and strategy.  Outputs: hash[i], coarse[i] per
+// record; summary = {groups, outliers, samples, overflow}; per group g:
+// ghash, gcount, gmean[3], gvar[3], gdepth, gtype0 (bsdf type of vertex 0);
+// totals: tmean[3], tvar[3]; coarse stats: ccount[8], cmean[24]; for
+// <16, 4>, the pixel in the reference's byte layout when bytes is given.
+void orc_diag_pixel_run(int max_groups, int max_outliers, int n, const int32_t* depth, const int32_t* obj,
+                        const int16_t* mat, const uint8_t* type, const uint8_t* flags, const float* contrib,
+                        const uint8_t* ltype, const uint8_t* strat, uint64_t* hash, uint8_t* coarse, long long* summary,
+                        uint64_t* ghash, uint32_t* gcount, float* gmean, float* gvar, int32_t* gdepth, int32_t* gtype0,
+                        float* tmean, float* tvar, uint32_t* ccount, float* cmean, unsigned char* bytes) {
+    DPixel d((size_t)max_groups, (size_t)max_outliers);
+    for (int i = 0; i < n; ++i) {
+        DRec r;
+        for (int k = 0; k < depth[i] && k < (int)D_MAX_DEPTH; ++k) {
+            DVert x;
+            x.obj = obj[i * 16 + k]; x.mat = mat[i * 16 + k]; x.type = type[i * 16 + k]; x.flags = flags[i * 16 + k];
+            r.v.push_back(x);
+        }
+        r.contrib = v3(contrib[3 * i], contrib[3 * i + 1], contrib[3 * i + 2]);
+        r.light_type = ltype[i]; r.strategy = strat[i];
+        hash[i] = d_hash(r.v);
+        coarse[i] = d_coarse(r.v);
+        d.record(r);
+    }
+    summary[0] = (long long)d.n_g; summary[1] = (long long)d.n_o; summary[2] = (long long)d.total;
+    summary[3] = (long long)d.overflow;
+    for (int gi = 0; gi < max_groups; ++gi) {
+        const DGroup& G = d.g[gi];
+        ghash[gi] = G.hash; gcount[gi] = G.st.n; gdepth[gi] = G.depth; gtype0[gi] = G.v[0].type;
+        for (int c = 0; c < 3; ++c) { gmean[gi * 3 + c] = G.st.mean[c]; gvar[gi * 3 + c] = G.st.var[c]; }
+    }
+    for (int c = 0; c < 3; ++c) { tmean[c] = d.tot.mean[c]; tvar[c] = d.tot.var[c]; }
+    for (int t = 0; t < 8; ++t) {
+        ccount[t] = d.coarse[t].n;
+        for (int c = 0; c < 3; ++c) cmean[t * 3 + c] = d.coarse[t].mean[c];
+    }
+    if (bytes && max_groups == 16 && max_outliers == 4) d_serialise(d, bytes);
 }
 
 }  // extern "C"

@@ -158,6 +158,12 @@ def lib(fast: bool = False) -> C.CDLL:
     L.orc_sample_light.argtypes = [vp, ip, fp, C.c_float, C.c_float, fp]
     L.orc_select_light.argtypes = [vp, C.c_float, fp]
     L.orc_pdf_light_area.argtypes = [C.c_float, fp, fp, fp]
+    u8p = C.POINTER(C.c_uint8)
+    L.orc_render_tile_diag.argtypes = [vp, C.c_char_p] + [ip] * 11 + [fp, u8p, dp, i64p]
+    L.orc_diag_pixel_run.argtypes = [ip, ip, ip, i32p, i32p, C.POINTER(C.c_int16), u8p, u8p, fp, u8p, u8p,
+                                     C.POINTER(C.c_uint64), u8p, i64p, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint32), fp, fp, i32p, i32p, fp, fp, C.POINTER(C.c_uint32), fp,
+                                     u8p]
     L.orc_pdf_light_area.restype = C.c_float
     _libs[name] = L
     return L
@@ -318,6 +324,26 @@ class OracleScene:
                                counts.ctypes.data_as(C.POINTER(C.c_longlong)))
         return out, {"stats": st, "strat": sd, "counts": counts}
 
+    def render_tile_diag(self, algorithm, tx, ty, tw, th, fw, fh, samples=1, sample_offset=0, max_depth=8,
+                         subsample=1, threads=0):
+        """render_tile with diagnostics on: (rgba bottom-first, film as a
+        PIXEL_DT array (sh, sw), per-strategy FP64 Welford (sh, sw, 4, 7),
+        total records)."""
+        sw, sh = tw // subsample, th // subsample
+        out = np.zeros(tw * th * 4, np.float32)
+        film = np.zeros(max(sw * sh, 1) * PIXEL_DT.itemsize, np.uint8)
+        strat = np.zeros(max(sw * sh, 1) * 28, np.float64)
+        tot = np.zeros(1, np.int64)
+        rc = self.L.orc_render_tile_diag(self.h, algorithm.encode(), tx, ty, tw, th, fw, fh, samples, sample_offset,
+                                         max_depth, subsample, threads, _fptr(out),
+                                         film.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                         strat.ctypes.data_as(C.POINTER(C.c_double)),
+                                         tot.ctypes.data_as(C.POINTER(C.c_int64)))
+        if rc != 0:
+            raise RuntimeError(f"orc_render_tile_diag failed ({rc})")
+        film = film[: sw * sh * PIXEL_DT.itemsize].view(PIXEL_DT).reshape(sh, sw)
+        return out, film, strat[: sw * sh * 28].reshape(sh, sw, 4, 7), int(tot[0])
+
     def trace_pixel(self, algorithm, gx, gy, fw, fh, samples_in_call, sample_offset, nsamples, max_depth=8):
         out = np.zeros((nsamples, 3), dtype=np.float32)
         self.L.orc_trace_pixel(self.h, algorithm.encode(), gx, gy, fw, fh, samples_in_call, sample_offset,
@@ -372,6 +398,65 @@ class OracleScene:
         p = np.zeros(1, np.float32)
         i = self.L.orc_select_light(self.h, u, _fptr(p))
         return i, float(p[0])
+
+
+# --------------------------------------------------------------- diagnostics
+# PixelPathData<16, 4> byte layout (diagnostics/pixel_path_data.hpp:145-160,
+# path_stats.hpp:51-58 and 133-147, path_types.hpp:178-183).
+STATS_DT = np.dtype([("count", "<u4"), ("mean", "<f4", 3), ("m2", "<f4", 3), ("variance", "<f4", 3)])
+VERTEX_DT = np.dtype([("object_id", "<i4"), ("material_id", "<i2"), ("bsdf_type", "u1"), ("flags", "u1")])
+GROUP_DT = np.dtype([("path_hash", "<u8"), ("stats", STATS_DT), ("depth", "u1"), ("coarse_type", "u1"),
+                     ("light_type", "u1"), ("strategy", "u1"), ("pad", "u1", 4), ("vertices", VERTEX_DT, 16),
+                     ("positions", "<f4", (16, 3)), ("normals", "<f4", (16, 3))])
+OUTLIER_DT = np.dtype({"names": ["path_hash", "contribution"], "formats": ["<u8", ("<f4", 3)],
+                       "offsets": [0, 8], "itemsize": 24})
+PIXEL_DT = np.dtype([("groups", GROUP_DT, 16), ("coarse", STATS_DT, 8), ("outliers", OUTLIER_DT, 4),
+                     ("num_groups", "<u8"), ("num_outliers", "<u8"), ("total_samples", "<u8"),
+                     ("overflow_count", "<u8"), ("total", STATS_DT)])
+assert GROUP_DT.itemsize == 568 and PIXEL_DT.itemsize == 9576
+
+
+def diag_pixel_run(records, max_groups=16, max_outliers=4):
+    """Feeds records [(vertices, contribution, light_type, strategy)], each
+    vertex (object_id, material_id, bsdf_type, flags), through the oracle's
+    PixelPathData<max_groups, max_outliers> and returns its state."""
+    L = lib()
+    n = len(records)
+    depth = np.zeros(max(n, 1), np.int32)
+    obj = np.full(max(n, 1) * 16, -1, np.int32)
+    mat = np.full(max(n, 1) * 16, -1, np.int16)
+    typ = np.zeros(max(n, 1) * 16, np.uint8)
+    flg = np.zeros(max(n, 1) * 16, np.uint8)
+    con = np.zeros(max(n, 1) * 3, np.float32)
+    lty = np.zeros(max(n, 1), np.uint8)
+    stg = np.zeros(max(n, 1), np.uint8)
+    for i, (verts, c, lt, st) in enumerate(records):
+        depth[i] = len(verts)
+        for k, (o_, m_, t_, f_) in enumerate(verts[:16]):
+            obj[i * 16 + k], mat[i * 16 + k], typ[i * 16 + k], flg[i * 16 + k] = o_, m_, t_, f_
+        con[3 * i:3 * i + 3] = c
+        lty[i], stg[i] = lt, st
+    out = {"hash": np.zeros(max(n, 1), np.uint64), "coarse": np.zeros(max(n, 1), np.uint8),
+           "summary": np.zeros(4, np.int64), "ghash": np.zeros(max_groups, np.uint64),
+           "gcount": np.zeros(max_groups, np.uint32), "gmean": np.zeros(max_groups * 3, np.float32),
+           "gvar": np.zeros(max_groups * 3, np.float32), "gdepth": np.zeros(max_groups, np.int32),
+           "gtype0": np.zeros(max_groups, np.int32), "tmean": np.zeros(3, np.float32),
+           "tvar": np.zeros(3, np.float32), "ccount": np.zeros(8, np.uint32), "cmean": np.zeros(24, np.float32),
+           "bytes": np.zeros(PIXEL_DT.itemsize, np.uint8)}
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    L.orc_diag_pixel_run(max_groups, max_outliers, n, P(depth, C.c_int32), P(obj, C.c_int32), P(mat, C.c_int16),
+                         P(typ, C.c_uint8), P(flg, C.c_uint8), P(con, C.c_float), P(lty, C.c_uint8),
+                         P(stg, C.c_uint8), P(out["hash"], C.c_uint64), P(out["coarse"], C.c_uint8),
+                         P(out["summary"], C.c_int64), P(out["ghash"], C.c_uint64), P(out["gcount"], C.c_uint32),
+                         P(out["gmean"], C.c_float), P(out["gvar"], C.c_float), P(out["gdepth"], C.c_int32),
+                         P(out["gtype0"], C.c_int32), P(out["tmean"], C.c_float), P(out["tvar"], C.c_float),
+                         P(out["ccount"], C.c_uint32), P(out["cmean"], C.c_float),
+                         P(out["bytes"], C.c_uint8) if (max_groups, max_outliers) == (16, 4) else None)
+    out["hash"], out["coarse"] = out["hash"][:n], out["coarse"][:n]
+    out["gmean"] = out["gmean"].reshape(-1, 3)
+    out["gvar"] = out["gvar"].reshape(-1, 3)
+    out["cmean"] = out["cmean"].reshape(-1, 3)
+    return out
 
 
 def flip_to_top_first(rgba_flat: np.ndarray, tw: int, th: int) -> np.ndarray:

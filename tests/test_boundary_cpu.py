@@ -19,6 +19,14 @@ REFERENCE_METHODS = [
     "cancel", "is_cancelled", "reset_cancel", "load_scene_json", "set_camera", "set_algorithm",
     "set_object_names", "get_object_name", "object_path_string", "render_tile", "render_tile_gpu",
     "render_tile_numpy", "is_scene_loaded", "is_camera_set", "get_algorithm", "get_mesh_count",
+    "enable_diagnostics", "disable_diagnostics", "is_diagnostics_enabled", "clear_diagnostics",
+    "get_diagnostic_stats", "get_diagnostic_variance_map", "export_diagnostic_json", "get_top_variance_groups",
+    "get_pixel_diagnostic",
+]
+REFERENCE_CLASSES = [
+    "Renderer", "PathRecordingConfig", "ExportedGroupInfo", "TopGroupSortBy", "PixelDiagnosticInfo",
+    "DiagnosticSuggestion", "GlobalDiagnosticStats", "ExportedCoarseStats", "DiagnosticFilm", "DiagnosticPathTracer",
+    "DiagnosticExporter", "BsdfType", "LightSourceType", "PathTrace", "PathDiagnosticRecorder",
 ]
 
 
@@ -54,6 +62,8 @@ def test_module_surface(lucid):
     r = lucid.Renderer()
     for m in REFERENCE_METHODS:
         assert hasattr(r, m), m
+    for c in REFERENCE_CLASSES:  # pybind_module.cpp:41-541
+        assert hasattr(lucid, c), c
     assert lucid.__version__ == "1.0.0"
     assert lucid.openmp_enabled is False and lucid.dawn_enabled is False
     assert r.get_algorithm() == "nee"                          # C++ default (pybind_renderer.hpp:68)

@@ -1,0 +1,131 @@
+"""GPU parity of the DiagnosticFilm path (traceMISWithDiagnostics and friends,
+path_tracer.hpp:880-1514; DiagnosticFilm / PixelPathData<16,4>) against the
+oracle, through the lucidrenderer module and the C ABI underneath.
+
+Bars (same seed streams, closed box):
+* the MIS diagnostic render equals the plain MIS render bit for bit on the
+  GPU (test_pyrenderer_diagnostics.cpp:353-367), and the radiance of every
+  diagnostic tracer matches the oracle to 2 % relative L1;
+* per film pixel: record counts (total_samples) equal the oracle's in >= 99 %
+  of pixels; where they are equal, the sets of group keys agree and the
+  group / total statistics agree to 1e-3 relative (float Welford over
+  contributions that differ from the oracle's in the last bits);
+* FP64 per-strategy Welford: n exact everywhere; means to 1e-4 relative;
+* independent seeds: per-strategy pixel means within 3 sigma of the oracle's.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+import workloads
+from conftest import make_oracle, make_renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def _render_diag(lucid, algo, W, H, spp, depth, sub=1, offset=0):
+    r = make_renderer(lucid, workloads.closed_box(), workloads.CLOSED_BOX_CAMERA, algo)
+    cfg = lucid.PathRecordingConfig()
+    cfg.subsample_factor = sub
+    r.enable_diagnostics(cfg)
+    img = np.asarray(r.render_tile(0, 0, W, H, W, H, spp, offset, depth), np.float32)
+    film = r.get_diagnostic_film_bytes()
+    film = np.ascontiguousarray(film).view(oracle.PIXEL_DT)[..., 0]
+    return r, img, film, r.get_diagnostic_strategy_stats()
+
+
+def _live_groups(px):
+    n = int(px["num_groups"])
+    return {int(g["path_hash"]): g for g in px["groups"][:n]}
+
+
+# every algorithm twice, so a later render runs on storage an earlier one used
+@pytest.mark.parametrize("algo", ["mis", "nee", "simple", "mis", "simple"])
+def test_film_matches_oracle(lucid, algo):
+    W = H = 16
+    spp, depth = 8, 6
+    r, img, film, strat = _render_diag(lucid, algo, W, H, spp, depth)
+    o = make_oracle(workloads.closed_box(), workloads.CLOSED_BOX_CAMERA)
+    ref_img, ref_film, ref_strat, ref_tot = o.render_tile_diag(algo, 0, 0, W, H, W, H, samples=spp, max_depth=depth)
+    rel = np.abs(img - ref_img).sum() / max(np.abs(ref_img).sum(), 1e-12)
+    assert rel < 2e-2, f"radiance rel L1 {rel:.3g}"
+    assert film.shape == ref_film.shape == (H, W)
+    same = film["total_samples"] == ref_film["total_samples"]
+    assert same.mean() >= 0.99, f"record counts agree in {same.mean():.3f} of pixels"
+    st = r.get_diagnostic_stats()
+    assert st.total_samples == int(film["total_samples"].sum()) and st.active_pixels > 0
+    bad = 0
+    for y, x in zip(*np.nonzero(same)):
+        a, b = _live_groups(film[y, x]), _live_groups(ref_film[y, x])
+        if a.keys() != b.keys():
+            bad += 1
+            continue
+        for k in a:
+            assert a[k]["stats"]["count"] == b[k]["stats"]["count"]
+            np.testing.assert_allclose(a[k]["stats"]["mean"], b[k]["stats"]["mean"], rtol=1e-3, atol=1e-6)
+            assert a[k]["depth"] == b[k]["depth"] and a[k]["strategy"] == b[k]["strategy"]
+            assert a[k]["light_type"] == b[k]["light_type"]
+        np.testing.assert_allclose(film[y, x]["total"]["mean"], ref_film[y, x]["total"]["mean"], rtol=1e-3, atol=1e-6)
+    assert bad <= 0.01 * same.sum() + 1
+    np.testing.assert_array_equal(strat[..., 0], ref_strat[..., 0])
+    np.testing.assert_allclose(strat[..., 1:4], ref_strat[..., 1:4], rtol=1e-4, atol=1e-6)
+
+
+def test_mis_diagnostic_render_equals_plain(lucid):
+    W = H = 16
+    _, img, _, _ = _render_diag(lucid, "mis", W, H, 4, 6)
+    r = make_renderer(lucid, workloads.closed_box(), workloads.CLOSED_BOX_CAMERA, "mis")
+    plain = np.asarray(r.render_tile(0, 0, W, H, W, H, 4, 0, 6), np.float32)
+    np.testing.assert_array_equal(img, plain)
+
+
+def test_subsampled_film_and_api(lucid):
+    W, H = 20, 14
+    r, img, film, strat = _render_diag(lucid, "mis", W, H, 4, 6, sub=2)
+    assert film.shape == (7, 10) and strat.shape == (7, 10, 4, 7)
+    assert (strat[..., 0] == 4 * 4).all()
+    vm = r.get_diagnostic_variance_map()
+    assert vm.shape == (W * H,) and (vm >= 0).all()
+    np.testing.assert_array_equal(vm.reshape(H, W)[::2, ::2], film["total"]["variance"].sum(-1))
+    js = r.export_diagnostic_json()
+    assert '"sampled_width": 10' in js and '"subsample_factor": 2' in js
+    top = r.get_top_variance_groups(5)
+    assert 0 < len(top) <= 5
+    # ranked by total variance (the channel sum, diagnostic_film.hpp:47-49), reported as luminance variance
+    tv = [sum(g.variance_rgb) for g in top]
+    assert tv == sorted(tv, reverse=True)
+    info = r.get_pixel_diagnostic(6, 4, "mean")
+    assert info.valid and info.sample_count == int(film[2, 3]["total_samples"])
+    means = [g.mean_luminance for g in info.top_groups]
+    assert means == sorted(means, reverse=True)
+    for g in info.top_groups:
+        assert len(g.positions) == g.depth + 1 and len(g.normals) == g.depth
+        assert g.strategy_name in ("MIS_BSDF", "MIS_NEE")
+    r.clear_diagnostics()
+    assert r.get_diagnostic_stats().total_samples == 0
+    r.disable_diagnostics()
+    assert r.export_diagnostic_json() == "{}"
+
+
+@pytest.mark.parametrize("algo", ["simple", "nee", "mis", "volume_mis", "volume_simple"])
+def test_records_with_every_algorithm(lucid, algo):
+    # test_pyrenderer_diagnostics.cpp:463-477
+    r, img, film, strat = _render_diag(lucid, algo, 4, 4, 2, 4)
+    assert r.get_diagnostic_stats().total_samples > 0
+
+
+def test_per_strategy_means_3sigma(lucid):
+    W = H = 12
+    n = 64
+    _, _, _, g = _render_diag(lucid, "mis", W, H, n, 6, offset=7000)
+    o = make_oracle(workloads.closed_box(), workloads.CLOSED_BOX_CAMERA)
+    _, _, c, _ = o.render_tile_diag("mis", 0, 0, W, H, W, H, samples=n, max_depth=6)
+    for k in (2, 3):  # MIS_BSDF, MIS_NEE
+        mg, mc = g[:, :, k, 1:4], c[:, :, k, 1:4]
+        vg = g[:, :, k, 4:7] / (n - 1)
+        vc = c[:, :, k, 4:7] / (n - 1)
+        se = np.sqrt(vg / n + vc / n)
+        z = np.abs(mg - mc) / np.maximum(se, 1e-12)
+        ok = (z <= 3.0) | ((se < 1e-9) & (np.abs(mg - mc) < 1e-6))
+        assert 1.0 - ok.mean() < 0.02, f"strategy {k}: {1 - ok.mean():.3f} beyond 3 sigma"

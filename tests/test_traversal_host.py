@@ -102,3 +102,16 @@ def test_icospheres_subset():
     rng = np.random.default_rng(3)
     o, d = _rays(rng, 10000, -8, 8)
     _compare(scene, o, d, 1e-4, 1e30)
+
+
+def test_one_voxel_grid_axis_is_refused(tmp_path):
+    # the trilinear lookup's cell clamp [0, n-2] is empty for a 1-voxel axis
+    scene = workloads.smoke_box(str(tmp_path / "density.bin"), grid_n=8)
+    for m in scene["meshes"]:
+        sg = (m.get("material") or {}).get("volume", {}).get("smoke_grid")
+        if sg:
+            sg["dims"] = [8, 8, 1]
+    o = np.zeros((1, 3), np.float32)
+    d = np.float32([[0, 0, 1]])
+    with pytest.raises(RuntimeError, match="LUCID_ERR_INVALID"):
+        lucid_capi.host_trace(scene, o, d, 1e-4, 1e30)
