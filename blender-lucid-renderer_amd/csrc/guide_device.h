@@ -1,0 +1,94 @@
+// guide_device.h -- device side of path guiding: input encoding and the
+// von Mises-Fisher mixture the network outputs (see guiding.h for the
+// network).  Used by the inference / training kernels and by the guided
+// shading bounce.
+#pragma once
+
+#include "device_math.h"
+#include "guiding.h"
+
+namespace lucid {
+namespace guide {
+
+// Input feature idx (0..63) of a surface point: pn = position in the scene
+// box mapped to [0,1]^3, n = shading normal, wo = outgoing direction.
+//   0..35   sin / cos (2^k pi pn_c), k = 0..5
+//   36..38  2 pn - 1
+//   39..41  n,  42..44 wo
+//   45..47  sin(pi n),  48..50 sin(pi wo),  51 n . wo
+//   52..63  0
+__device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f wo) {
+    if (idx < kFeatPos) {
+        const int c = idx % 3, q = idx / 3, k = q >> 1;
+        const float a = pn[c] * (PI_F * (float)(1 << k));
+        return (q & 1) ? cosf(a) : sinf(a);
+    }
+    const float nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
+    if (idx < 39) return 2.0f * pn[idx - 36] - 1.0f;
+    if (idx < 42) return nn[idx - 39];
+    if (idx < 45) return ww[idx - 42];
+    if (idx < 48) return sinf(PI_F * nn[idx - 45]);
+    if (idx < 51) return sinf(PI_F * ww[idx - 48]);
+    if (idx == 51) return dot(n, wo);
+    return 0.0f;
+}
+
+// The mixture decoded from the 40 raw outputs: lobe k has mean direction
+// normalize(o[5k..5k+2]), concentration exp(clamp(o[5k+3], -2, 7)) and
+// weight softmax(o[5k+4]).
+struct Mixture {
+    v3f mu[kLobes];
+    float kappa[kLobes], logc[kLobes], w[kLobes];
+};
+constexpr float kLogKappaMin = -2.0f, kLogKappaMax = 7.0f;
+
+// log of the vMF normalisation kappa / (2 pi (1 - exp(-2 kappa)))
+__device__ __forceinline__ float vmf_logc(float kappa) {
+    return logf(kappa) - 1.8378770664093453f - logf(-expm1f(-2.0f * kappa));
+}
+
+__device__ __forceinline__ void decode(const float* o, Mixture& m) {
+    float mx = -1e30f;
+    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
+    float sum = 0.0f;
+    for (int k = 0; k < kLobes; ++k) {
+        const v3f r = mk(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+        const float len = sqrtf(dot(r, r)) + 1e-6f;
+        m.mu[k] = r * (1.0f / len);
+        const float lk = fminf(fmaxf(o[5 * k + 3], kLogKappaMin), kLogKappaMax);
+        m.kappa[k] = expf(lk);
+        m.logc[k] = vmf_logc(m.kappa[k]);
+        m.w[k] = expf(o[5 * k + 4] - mx);
+        sum += m.w[k];
+    }
+    const float inv = 1.0f / sum;
+    for (int k = 0; k < kLobes; ++k) m.w[k] *= inv;
+}
+
+__device__ __forceinline__ float mixture_pdf(const Mixture& m, v3f w) {
+    float p = 0.0f;
+    for (int k = 0; k < kLobes; ++k) p += m.w[k] * expf(m.logc[k] + m.kappa[k] * (dot(m.mu[k], w) - 1.0f));
+    return p;
+}
+
+// Samples a lobe with u0, then the lobe with (u1, u2) (the inversion
+// w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa of the vMF cosine).
+__device__ __forceinline__ v3f mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+    int k = 0;
+    float c = m.w[0];
+    while (k < kLobes - 1 && u0 >= c) c += m.w[++k];
+    const float ka = m.kappa[k];
+    const float wz = fminf(fmaxf(1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * ka)) / ka, -1.0f), 1.0f);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
+    const float phi = 2.0f * PI_F * u2;
+    const v3f mu = m.mu[k];
+    // orthonormal basis around mu (Duff et al. 2017, "Building an orthonormal basis, revisited")
+    const float s = copysignf(1.0f, mu.z);
+    const float a = -1.0f / (s + mu.z), b = mu.x * mu.y * a;
+    const v3f t = mk(1.0f + s * mu.x * mu.x * a, s * b, -s * mu.x);
+    const v3f bb = mk(b, s + mu.y * mu.y * a, -mu.y);
+    return t * (r * cosf(phi)) + bb * (r * sinf(phi)) + mu * wz;
+}
+
+}  // namespace guide
+}  // namespace lucid

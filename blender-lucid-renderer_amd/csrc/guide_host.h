@@ -1,0 +1,37 @@
+// guide_host.h -- host interface of the guiding network state (guiding.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "guide_mlp.h"
+#include "guiding.h"
+
+namespace lucid {
+namespace guide {
+
+struct State;
+
+// Device state: FP32 master parameters with Adam moments, packed BF16 MFMA
+// operands, a ring of `capacity` training samples and batch buffers for up
+// to `batch_cap` samples.  nullptr when device memory runs out.
+State* create(int capacity, int batch_cap);
+void destroy(State* s);
+// Fresh network (He-uniform, seeded), Adam state and sample ring cleared.
+void reset(State* s, uint32_t seed, hipStream_t st);
+// Kernel view of the network for the scene box [bmin, bmax].
+GuideDev dev(const State* s, const float bmin[3], const float bmax[3], float alpha);
+Sample* ring(State* s);
+unsigned long long* ring_total(State* s);
+int ring_capacity(const State* s);
+// `steps` Adam steps on batches drawn uniformly from the ring.
+void train(State* s, const GuideDev& g, int steps, int batch, float lr, hipStream_t st);
+// One forward / backward pass over exactly `samples` with `params`:
+// raw outputs (n x 40), parameter gradient (kParams) and loss.  For tests.
+int debug_step(State* s, const float* params, const Sample* samples, int n, const float bmin[3], const float bmax[3],
+               float* out, float* grad, float* loss, hipStream_t st);
+int get_params(State* s, float* params, hipStream_t st);
+float last_loss(State* s, hipStream_t st);
+int set_params(State* s, const float* params, hipStream_t st);
+
+}  // namespace guide
+}  // namespace lucid

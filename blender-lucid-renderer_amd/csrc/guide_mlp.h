@@ -1,0 +1,81 @@
+// guide_mlp.h -- the guiding network on 16 samples per wavefront, with
+// v_mfma_f32_16x16x32_bf16 (gfx950).  Lane l works on sample l % 16 and, in
+// every operand, on the K / M slice of group g = l / 16 (layouts in
+// guiding.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "guiding.h"
+
+namespace lucid {
+namespace guide {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct GuideDev {
+    const uint4* fwd;     // packed forward A operands, 8 bf16 per lane per tile
+    const uint4* bwd;     // packed backward A operands
+    const float* params;  // FP32 master parameters (biases are read from here)
+    float bmin[3], binv[3];  // scene box: pn = (p - bmin) * binv
+    float alpha;             // guide-sampling probability of the guided bounce
+};
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) {
+    bf16x8 r;
+    __builtin_memcpy(&r, &u, 16);
+    return r;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+// B operand of the next layer from the accumulators of tiles 2kb, 2kb+1
+__device__ __forceinline__ bf16x8 pack_b(f32x4 lo, f32x4 hi) {
+    bf16x8 r;
+    for (int i = 0; i < 4; ++i) {
+        r[i] = (__bf16)lo[i];
+        r[4 + i] = (__bf16)hi[i];
+    }
+    return r;
+}
+
+// Forward pass.  `x` holds the lane's input operand for K blocks 0 and 1
+// (features perm(kb, g, i) of its sample).  Outputs: the three output tiles
+// (features 16m + 4g + j).  When `store` is given, the post-ReLU hidden
+// activations of layers 0..2 are returned in store[l][m] (D layout).
+__device__ __forceinline__ void mlp_forward(const GuideDev& G, bf16x8 x0, bf16x8 x1, f32x4 out[3],
+                                            f32x4 (*store)[4] = nullptr) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    bf16x8 b0 = x0, b1 = x1;
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        const int mt = layer_mtiles(l);
+        const float* bias = G.params + b_offset(l);
+        f32x4 acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (m >= mt) break;
+            for (int j = 0; j < 4; ++j) acc[m][j] = bias[16 * m + 4 * g + j];
+            const uint4* a = G.fwd + (fwd_offset(l) + (m * 2) * 512) / 8 + lane;
+            acc[m] = mfma(as_bf16x8(a[0]), b0, acc[m]);
+            acc[m] = mfma(as_bf16x8(a[64]), b1, acc[m]);
+        }
+        if (l < kLayers - 1) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                for (int j = 0; j < 4; ++j) acc[m][j] = fmaxf(acc[m][j], 0.0f);
+            if (store)
+                for (int m = 0; m < 4; ++m) store[l][m] = acc[m];
+            b0 = pack_b(acc[0], acc[1]);
+            b1 = pack_b(acc[2], acc[3]);
+        } else {
+            out[0] = acc[0];
+            out[1] = acc[1];
+            out[2] = acc[2];
+        }
+    }
+}
+
+}  // namespace guide
+}  // namespace lucid

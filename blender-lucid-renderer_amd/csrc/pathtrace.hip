@@ -34,6 +34,9 @@
 #include "scene_pack.h"
 #include "shading.h"
 #include "diag_film.h"
+#include "guide_device.h"
+#include "guide_host.h"
+#include "guide_mlp.h"
 
 namespace lucid {
 
@@ -88,7 +91,14 @@ struct WaveDev {
     float4 *dv0, *dv1, *dr0, *dr1;
     int32_t* dcnt;
     int64_t dstride;
+    // guided runs: network outputs (40 per path) and, for 1 path in
+    // `tstride`, training slots of kTrainSlots bounces x 5 float4:
+    // (p, T_in.x) (n, T_in.y) (wo, T_in.z) (wi, valid) (L_after, valid)
+    float* gout;
+    float4* tslot;
+    int32_t tstride;
 };
+constexpr int kTrainSlots = 6;
 
 struct CamDev {
     float pos[3], fwd[3], right[3], up[3];
@@ -430,6 +440,13 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
     W.meta[p] = make_int4(pix, s, 0, 0);
     W.q[0][p] = p;
     if (W.dcnt) W.dcnt[p] = 0;
+    if (W.tslot && p % W.tstride == 0) {
+        float4* ts = W.tslot + (size_t)(p / W.tstride) * (kTrainSlots * 5);
+        for (int k = 0; k < kTrainSlots; ++k) {
+            ts[k * 5 + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ts[k * 5 + 4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
 }
 
 // ------------------------------------------------------------- intersect --
@@ -576,10 +593,11 @@ __device__ __forceinline__ void put_record(const WaveDev& W, int p, int slot, v3
 // One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570),
 // their volume variants (:209-283, 586-835), or the diagnostic tracers.
 template <int ALGO>
-__device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, int qin, int32_t max_depth, int i,
-                                          int n) {
+__device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, const guide::GuideDev& G, int qin,
+                                          int32_t max_depth, int i, int n) {
     constexpr bool DIAG = ALGO >= kAlgoDiagBase;
-    constexpr int A = DIAG ? ALGO - kAlgoDiagBase : ALGO;
+    constexpr bool GUIDED = ALGO == LUCID_ALGO_GUIDED_MIS;
+    constexpr int A = DIAG ? ALGO - kAlgoDiagBase : (GUIDED ? LUCID_ALGO_MIS : ALGO);
     constexpr bool MIS = A == LUCID_ALGO_MIS || A == LUCID_ALGO_VOLUME_MIS;
     constexpr bool SIMPLE = A == LUCID_ALGO_SIMPLE || A == LUCID_ALGO_VOLUME_SIMPLE;
     bool live = i < n;
@@ -598,6 +616,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
         rng.state = rs.x; rng.inc = rs.y;
         float4 Lv = W.rad[p];
         v3f L = mk(Lv);
+        // guided runs: the radiance gathered up to here closes the previous
+        // bounce's training slot (everything after it came through wi)
+        float4* ts = nullptr;
+        if (GUIDED && W.tslot && p % W.tstride == 0) {
+            ts = W.tslot + (size_t)(p / W.tstride) * (kTrainSlots * 5);
+            if (depth >= 1 && depth - 1 < kTrainSlots) ts[(depth - 1) * 5 + 4] = make_float4(L.x, L.y, L.z, 1.0f);
+        }
         // diagnostic path: vertex count so far; this bounce's record slot
         // starts empty (it may hold a record of an earlier batch)
         int nvert = 0;
@@ -730,6 +755,12 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             v3f sn = (mat.transmission < 0.5f && !front) ? -nrm : nrm;
             bool has_em = (gm.flags & MF_EMISSIVE) != 0;
             bool stop = false;  // traceSimpleWithDiagnostics ends at an emitter
+            // guided bounce: opaque, non-delta surfaces (the BSDF sample is an
+            // evaluated f / pdf pair there)
+            const bool guidable = GUIDED && mat.transmission == 0.0f && !(mat.roughness < 0.05f && mat.metallic > 0.5f);
+            const bool guided = guidable && G.alpha > 0.0f;
+            guide::Mixture gmx;
+            if (GUIDED && guided) guide::decode(W.gout + (size_t)p * guide::kOut, gmx);
             if (SIMPLE) {
                 if (DIAG) {
                     if (has_em) {
@@ -789,6 +820,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
                             v3f cc;
                             if (A == LUCID_ALGO_MIS) {
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                                // guided bounce: the BSDF strategy is the one-sample mixture
+                                if (GUIDED && guided) pb = G.alpha * guide::mixture_pdf(gmx, ls.dir) + (1.0f - G.alpha) * pb;
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
@@ -822,14 +855,44 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
             // still samples, but nothing it draws reaches the result)
             if (!stop && depth + 1 < max_depth) {
             v3f smp_n = (mat.transmission > 0.5f) ? nrm : sn;
-            float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
-            BSample bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+            const v3f thr_in = thr;
+            BSample bs;
+            if (GUIDED && guided) {
+                // one-sample MIS of the guide and the BSDF: pdf = a p_guide + (1 - a) p_bsdf
+                if (randf(rng) < G.alpha) {
+                    const float g0 = randf(rng), g1 = randf(rng), g2 = randf(rng);
+                    bs.wi = guide::mixture_sample(gmx, g0, g1, g2);
+                    bs.kind = 0;
+                    bs.type = 0;
+                    bs.fw = eval_bsdf(mat, wo, bs.wi, smp_n);
+                    bs.pdf = pdf_bsdf(mat, wo, bs.wi, smp_n);
+                } else {
+                    const float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
+                    bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+                }
+                bs.pdf = G.alpha * guide::mixture_pdf(gmx, bs.wi) + (1.0f - G.alpha) * bs.pdf;
+            } else {
+                const float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
+                bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+            }
             bool ok = bs.kind == 1 || bs.pdf > MIN_PDF;
             v3f w = bs.fw;
             if (ok && bs.kind == 0) {
                 float cs = fabsf(dot(smp_n, bs.wi));
                 if (cs <= 1e-6f || bs.pdf < MIN_PDF) ok = false;
                 else w = bsdf_weight(bs.fw, cs, bs.pdf);
+            }
+            // training slot of this bounce: where, how much arrived, which way it left
+            if (GUIDED && ts && depth < kTrainSlots) {
+                const bool rec = ok && guidable && (w.x > 0.0f || w.y > 0.0f || w.z > 0.0f);
+                float4* t = ts + depth * 5;
+                if (rec) {
+                    t[0] = make_float4(point.x, point.y, point.z, thr_in.x);
+                    t[1] = make_float4(nrm.x, nrm.y, nrm.z, thr_in.y);
+                    t[2] = make_float4(wo.x, wo.y, wo.z, thr_in.z);
+                }
+                t[3] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, rec ? 1.0f : 0.0f);
+                t[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
             if (ok) {
                 thr = cmul(thr, w);
@@ -863,10 +926,101 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, i
 // Grid-stride over the live queue; the trip count is uniform per workgroup so
 // every lane reaches the wavefront ballots of shade_one.
 template <int ALGO>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, int qin, int32_t max_depth) {
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::GuideDev G, int qin,
+                                                  int32_t max_depth) {
     const int n = W.cnt[qin];
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
-        shade_one<ALGO>(S, W, qin, max_depth, base + (int)threadIdx.x, n);
+        shade_one<ALGO>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
+}
+
+// Surface point and shading normal of a closest hit (as the shade kernel
+// computes them).
+__device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float4 hv, v3f& point, v3f& nrm) {
+    const int32_t prim = __float_as_int(hv.y);
+    point = o + d * hv.x;
+    const float4 fnv = S.fn[prim];
+    nrm = mk(fnv);
+    if (__float_as_int(fnv.w)) {
+        const float u = hv.z, v = hv.w, w = 1.0f - u - v;
+        const float4* vn = S.vn + 3 * (size_t)prim;
+        nrm = make_dir_or(mk(vn[0]) * w + mk(vn[1]) * u + mk(vn[2]) * v, nrm);
+    }
+}
+
+// Guiding network on the live queue: 64 paths per wavefront, 16 per MFMA
+// tile.  Paths without a surface hit run on zero input and are not written.
+__global__ __launch_bounds__(64) void k_guide_infer(SceneDev S, WaveDev W, guide::GuideDev G, int qin) {
+    const int n = W.cnt[qin];
+    const int lane = threadIdx.x, g = lane >> 4, s = lane & 15;
+    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        for (int t = 0; t < 4; ++t) {
+            const int qi = base + 16 * t + s;
+            const bool valid = qi < n;
+            const int p = valid ? W.q[qin][qi] : 0;
+            bool hit = false;
+            float pn[3] = {0.0f, 0.0f, 0.0f};
+            v3f nrm = mk(0.0f, 0.0f, 1.0f), wo = mk(0.0f, 0.0f, 1.0f);
+            if (valid) {
+                const float4 hv = W.hit[p];
+                if (__float_as_int(hv.y) >= 0) {
+                    const v3f o = mk(W.ray_o[p]), d = mk(W.ray_d[p]);
+                    v3f point;
+                    hit_frame(S, o, d, hv, point, nrm);
+                    wo = -d;
+                    pn[0] = (point.x - G.bmin[0]) * G.binv[0];
+                    pn[1] = (point.y - G.bmin[1]) * G.binv[1];
+                    pn[2] = (point.z - G.bmin[2]) * G.binv[2];
+                    hit = true;
+                }
+            }
+            guide::bf16x8 x0, x1;
+            for (int i = 0; i < 8; ++i) {
+                x0[i] = (__bf16)guide::feature(guide::perm(0, g, i), pn, nrm, wo);
+                x1[i] = (__bf16)guide::feature(guide::perm(1, g, i), pn, nrm, wo);
+            }
+            guide::f32x4 o[3];
+            guide::mlp_forward(G, x0, x1, o);
+            if (hit) {
+                float* out = W.gout + (size_t)p * guide::kOut;
+                for (int m = 0; m < 3; ++m)
+                    for (int j = 0; j < 4; ++j) {
+                        const int f = 16 * m + 4 * g + j;
+                        if (f < guide::kOut) out[f] = o[m][j];
+                    }
+            }
+        }
+    }
+}
+
+// Training samples of the finished batch: for each recorded bounce of the
+// recording paths, weight = lum((L_end - L_after) / T_in), the radiance that
+// came back through wi in units of the throughput that reached the vertex.
+__global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_paths, guide::Sample* ring,
+                                                       unsigned long long* total, int capacity) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int p = r * W.tstride;
+    if (p >= n_paths) return;
+    const float4* ts = W.tslot + (size_t)r * (kTrainSlots * 5);
+    const float4 le = W.rad[p];
+    for (int k = 0; k < kTrainSlots; ++k) {
+        const float4 a = ts[k * 5], b = ts[k * 5 + 1], c = ts[k * 5 + 2], d = ts[k * 5 + 3], e = ts[k * 5 + 4];
+        if (d.w != 1.0f || e.w != 1.0f) continue;
+        const float tr = a.w, tg = b.w, tb = c.w;
+        const float cr = tr > 0.0f ? (le.x - e.x) / tr : 0.0f;
+        const float cg = tg > 0.0f ? (le.y - e.y) / tg : 0.0f;
+        const float cb = tb > 0.0f ? (le.z - e.z) / tb : 0.0f;
+        const float wgt = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
+        if (!(wgt > 0.0f) || !isfinite(wgt)) continue;
+        const unsigned long long slot = atomicAdd(total, 1ull);
+        guide::Sample sm;
+        sm.p[0] = a.x; sm.p[1] = a.y; sm.p[2] = a.z;
+        sm.n[0] = b.x; sm.n[1] = b.y; sm.n[2] = b.z;
+        sm.wo[0] = c.x; sm.wo[1] = c.y; sm.wo[2] = c.z;
+        sm.wi[0] = d.x; sm.wi[1] = d.y; sm.wi[2] = d.z;
+        sm.weight = wgt;
+        sm.pad[0] = sm.pad[1] = sm.pad[2] = 0.0f;
+        ring[slot % (unsigned long long)capacity] = sm;
+    }
 }
 
 // ---------------------------------------------------------------- shadow --
@@ -1066,6 +1220,7 @@ struct lucid_scene {
     PackedScene host;
     std::vector<void*> allocs;
     SceneDev dev;
+    float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // scene box
 };
 
 struct lucid_ctx {
@@ -1092,6 +1247,14 @@ struct lucid_ctx {
     int64_t out_cap = 0;
     lucid_stats stats{};
     int32_t profiling = 0;
+    // learned path guiding
+    guide::State* gstate = nullptr;
+    lucid_guiding_config gcfg{0.5f, 8, 16384, 5e-3f, 8, 1u};
+    int64_t gsteps = 0;
+    float* gout = nullptr;
+    int64_t gout_cap = 0;
+    float4* tslot = nullptr;
+    int64_t tslot_cap = 0;
     // DiagnosticFilm (rebuilt by every diagnostic render, sized to the tile)
     int32_t diag_on = 0, diag_sub = 1;
     int32_t diag_tw = 0, diag_th = 0;
@@ -1199,6 +1362,16 @@ int ensure_diag_wave(lucid_ctx* ctx, int64_t n, int32_t slots) {
     ctx->diag_slots = slots;
     return LUCID_OK;
 }
+constexpr int kGuideRing = 1 << 21, kGuideBatchCap = 65536;
+int ensure_guide(lucid_ctx* ctx) {
+    if (ctx->gstate) return LUCID_OK;
+    ctx->gstate = guide::create(kGuideRing, kGuideBatchCap);
+    if (!ctx->gstate) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (guiding network)");
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    ctx->gsteps = 0;
+    return LUCID_OK;
+}
+
 void free_diag_film(lucid_ctx* ctx) {
     if (ctx->dfilm) (void)hipFree(ctx->dfilm);
     if (ctx->dstrat) (void)hipFree(ctx->dstrat);
@@ -1255,7 +1428,7 @@ int validate(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const 
         return fail(ctx, LUCID_ERR_INVALID, "tile and image sizes must be positive");
     if ((int64_t)pr->tile_w * pr->tile_h > (1LL << 28)) return fail(ctx, LUCID_ERR_INVALID, "tile too large");
     if (pr->samples < 0 || pr->max_depth < 0) return fail(ctx, LUCID_ERR_INVALID, "negative samples or depth");
-    if (pr->algorithm < 0 || pr->algorithm > 4) return fail(ctx, LUCID_ERR_INVALID, "unknown algorithm");
+    if (pr->algorithm < 0 || pr->algorithm > LUCID_ALGO_GUIDED_MIS) return fail(ctx, LUCID_ERR_INVALID, "unknown algorithm");
     if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
     return LUCID_OK;
 }
@@ -1289,6 +1462,9 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     // Diagnostic renders (render_tile with diagnostics enabled) also keep
     // per-path vertices and records: smaller batches, at most ~2 GB of them.
     const bool diag = mode == 0 && ctx->diag_on;
+    // Guided renders keep 40 network outputs per path: batches of <= 2^23.
+    const bool guided = !diag && pr->algorithm == LUCID_ALGO_GUIDED_MIS;
+    if (guided) target = std::min<int64_t>(target, (int64_t)1 << 23);
     const int32_t slots = std::max(1, pr->max_depth);
     if (diag) {
         const int64_t per_path = (int64_t)diag::kMaxDepth * 32 + (int64_t)slots * 32 + 4;
@@ -1342,6 +1518,28 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         W.dcnt = nullptr;
         W.dstride = 0;
     }
+    guide::GuideDev G;
+    std::memset(&G, 0, sizeof(G));
+    W.gout = nullptr;
+    W.tslot = nullptr;
+    W.tstride = 1;
+    const bool train = guided && ctx->gcfg.train_steps > 0;
+    if (guided) {
+        rc = ensure_guide(ctx);
+        if (rc) return rc;
+        rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, P * B * guide::kOut);
+        if (rc) return rc;
+        W.gout = ctx->gout;
+        if (train) {
+            W.tstride = std::max(1, ctx->gcfg.record_stride);
+            const int64_t nrec = (P * B + W.tstride - 1) / W.tstride;
+            rc = ensure_buf(ctx, &ctx->tslot, &ctx->tslot_cap, nrec * kTrainSlots * 5);
+            if (rc) return rc;
+            W.tslot = ctx->tslot;
+        }
+        // the guide samples only once it has been trained
+        G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gsteps > 0 ? ctx->gcfg.alpha : 0.0f);
+    }
     // LDS stack sized to the scene's BVH depth; persistent grids of a few
     // workgroups per CU walk the live queues (grid-stride).
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
@@ -1375,8 +1573,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     // (pybind_renderer.hpp:296-307)
     if (diag)
         algo = pr->algorithm == LUCID_ALGO_SIMPLE ? kAlgoDiagSimple
-               : pr->algorithm == LUCID_ALGO_MIS  ? kAlgoDiagMis
-                                                  : kAlgoDiagNee;
+               : (pr->algorithm == LUCID_ALGO_MIS || pr->algorithm == LUCID_ALGO_GUIDED_MIS) ? kAlgoDiagMis
+                                                                                               : kAlgoDiagNee;
     for (int s0 = 0; s0 < S; s0 += (int)B) {
         // Pacing: wait for the wave before the previous one, so at most two
         // waves are queued and a cancel takes effect within one wave.
@@ -1398,31 +1596,40 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
+            if (guided && G.alpha > 0.0f) {
+                ks = kev_begin(ctx, 3, st);
+                const unsigned gg = (unsigned)std::min<int64_t>(blocks(n, 64), (int64_t)ctx->num_cu * 32);
+                k_guide_infer<<<gg, 64, 0, st>>>(SD, W, G, q);
+                kev_end(ctx, ks, st);
+            }
             ks = kev_begin(ctx, 1, st);
             switch (algo) {
                 case LUCID_ALGO_SIMPLE:
-                    k_shade<LUCID_ALGO_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_MIS:
-                    k_shade<LUCID_ALGO_MIS><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_SIMPLE:
-                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_MIS:
-                    k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    break;
+                case LUCID_ALGO_GUIDED_MIS:
+                    k_shade<LUCID_ALGO_GUIDED_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case kAlgoDiagSimple:
-                    k_shade<kAlgoDiagSimple><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<kAlgoDiagSimple><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case kAlgoDiagNee:
-                    k_shade<kAlgoDiagNee><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<kAlgoDiagNee><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 case kAlgoDiagMis:
-                    k_shade<kAlgoDiagMis><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<kAlgoDiagMis><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
                 default:
-                    k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, q, max_depth);
+                    k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
             }
             kev_end(ctx, ks, st);
@@ -1431,6 +1638,10 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             kev_end(ctx, ks, st);
             q = 1 - q;
         }
+        if (train)
+            k_guide_collect<<<blocks((n + W.tstride - 1) / W.tstride, 256), 256, 0, st>>>(
+                W, (int32_t)n, guide::ring(ctx->gstate), guide::ring_total(ctx->gstate),
+                guide::ring_capacity(ctx->gstate));
         if (mode == 0) {
             k_film<<<blocks(P, 256), 256, 0, st>>>(W, ctx->film, (int32_t)P, b);
             if (diag && dsw > 0 && dsh > 0)
@@ -1445,6 +1656,18 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         }
         (void)hipEventRecord(ctx->wave_ev[waves & 1], st);
         ++waves;
+    }
+    // online training after the render (its time is part of the render's)
+    if (train && !*cancelled) {
+        unsigned long long recorded = 0;
+        HIP_OK(hipMemcpyAsync(&recorded, guide::ring_total(ctx->gstate), 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (recorded > 0) {  // nothing to learn from before the first sample
+            const int ks = kev_begin(ctx, 4, st);
+            guide::train(ctx->gstate, G, ctx->gcfg.train_steps, ctx->gcfg.batch, ctx->gcfg.learning_rate, st);
+            kev_end(ctx, ks, st);
+            ctx->gsteps += ctx->gcfg.train_steps;
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, LUCID_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -1468,8 +1691,8 @@ int finish_stats(lucid_ctx* ctx) {
     ctx->stats.tris_shadow = rays[5];
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->stats.ms_render = ms;
-    double kms[3] = {0, 0, 0};
-    uint64_t kn[3] = {0, 0, 0};
+    double kms[5] = {0, 0, 0, 0, 0};
+    uint64_t kn[5] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < ctx->kev_used; ++i) {
         float e = 0.0f;
         if (hipEventElapsedTime(&e, ctx->kev[2 * i], ctx->kev[2 * i + 1]) == hipSuccess) {
@@ -1479,6 +1702,7 @@ int finish_stats(lucid_ctx* ctx) {
     }
     ctx->stats.ms_intersect = kms[0]; ctx->stats.ms_shade = kms[1]; ctx->stats.ms_shadow = kms[2];
     ctx->stats.launches_intersect = kn[0]; ctx->stats.launches_shade = kn[1]; ctx->stats.launches_shadow = kn[2];
+    ctx->stats.ms_guide_infer = kms[3]; ctx->stats.ms_guide_train = kms[4];
     if (cnt[3] != 0) return fail(ctx, LUCID_ERR_DEVICE, "BVH traversal stack overflow");
     return LUCID_OK;
 }
@@ -1516,6 +1740,9 @@ void lucid_destroy(lucid_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     free_wave(ctx);
+    guide::destroy(ctx->gstate);
+    if (ctx->gout) (void)hipFree(ctx->gout);
+    if (ctx->tslot) (void)hipFree(ctx->tslot);
     free_diag_wave(ctx);
     free_diag_film(ctx);
     if (ctx->dpattern) (void)hipFree(ctx->dpattern);
@@ -1584,6 +1811,18 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     D.total_area = ps.total_area;
     std::memcpy(D.env, ps.env, sizeof(D.env));
     D.native_start = ps.native_start;
+    // scene box = union of the root's child boxes
+    if (!ps.nodes.empty()) {
+        const GNode& r = ps.nodes[ps.root];
+        for (int c = 0; c < 3; ++c) {
+            s->bmin[c] = r.b[c];
+            s->bmax[c] = r.b[3 + c];
+            if (r.child[1] != kEmptyChild) {
+                s->bmin[c] = std::min(s->bmin[c], r.b[6 + c]);
+                s->bmax[c] = std::max(s->bmax[c], r.b[9 + c]);
+            }
+        }
+    }
     D.any_vbo = 0;
     for (const GMat& m : ps.mats)
         if (m.flags & MF_VOLUME_BOUNDARY) D.any_vbo = 1;
@@ -1884,6 +2123,86 @@ int lucid_diag_clear(lucid_ctx* ctx) {
                                                          words, ctx->dstrat, nst);
     k_zero_u64<<<1, 64, 0, ctx->stream>>>(ctx->dtotal);
     HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+
+int lucid_set_guiding(lucid_ctx* ctx, const lucid_guiding_config* cfg) {
+    if (!ctx || !cfg) return LUCID_ERR_INVALID;
+    if (!(cfg->alpha >= 0.0f && cfg->alpha <= 1.0f) || cfg->train_steps < 0 || cfg->batch < 16 ||
+        cfg->batch > 65536 || !(cfg->learning_rate > 0.0f) || cfg->record_stride < 1)
+        return fail(ctx, LUCID_ERR_INVALID, "guiding: alpha in [0,1], train_steps >= 0, batch in [16, 65536], "
+                                            "learning_rate > 0, record_stride >= 1");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->gcfg = *cfg;
+    int rc = ensure_guide(ctx);
+    if (rc) return rc;
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    ctx->gsteps = 0;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+int lucid_guiding_reset(lucid_ctx* ctx) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    lucid_guiding_config c = ctx->gcfg;
+    return lucid_set_guiding(ctx, &c);
+}
+
+int lucid_guiding_get_stats(lucid_ctx* ctx, lucid_guiding_stats* out) {
+    if (!ctx || !out) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->num_params = guide::kParams;
+    out->train_steps = ctx->gsteps;
+    out->active = ctx->gsteps > 0 ? 1 : 0;
+    if (!ctx->gstate) return LUCID_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    unsigned long long tot = 0;
+    HIP_OK(hipMemcpyAsync(&tot, guide::ring_total(ctx->gstate), 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    out->samples_total = (int64_t)tot;
+    out->last_loss = guide::last_loss(ctx->gstate, ctx->stream);
+    return LUCID_OK;
+}
+
+int lucid_guiding_get_params(lucid_ctx* ctx, float* params, int32_t n) {
+    if (!ctx || !params || n != guide::kParams) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = ensure_guide(ctx);
+    if (rc) return rc;
+    return guide::get_params(ctx->gstate, params, ctx->stream) ? fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed")
+                                                               : LUCID_OK;
+}
+
+int lucid_guiding_set_params(lucid_ctx* ctx, const float* params, int32_t n) {
+    if (!ctx || !params || n != guide::kParams) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = ensure_guide(ctx);
+    if (rc) return rc;
+    if (guide::set_params(ctx->gstate, params, ctx->stream)) return fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed");
+    ctx->gsteps = std::max<int64_t>(ctx->gsteps, 1);  // explicit weights count as trained
+    return LUCID_OK;
+}
+
+int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
+                             const float bmin[3], const float bmax[3], float* out, float* grad, float* loss) {
+    if (!ctx || !params || !samples || !bmin || !bmax || !out || !grad || !loss || n < 1 || n > kGuideBatchCap)
+        return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = ensure_guide(ctx);
+    if (rc) return rc;
+    rc = guide::debug_step(ctx->gstate, params, reinterpret_cast<const guide::Sample*>(samples), n, bmin, bmax, out,
+                           grad, loss, ctx->stream);
+    if (rc) return fail(ctx, rc == 1 ? LUCID_ERR_INVALID : LUCID_ERR_DEVICE, "guiding debug step failed");
+    // the network under training was overwritten: start afresh
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
 }

@@ -285,6 +285,7 @@ int algo_code(const std::string& a) {  // pybind_renderer.hpp:301-327
     if (a == "mis") return LUCID_ALGO_MIS;
     if (a == "volume_simple") return LUCID_ALGO_VOLUME_SIMPLE;
     if (a == "volume_mis") return LUCID_ALGO_VOLUME_MIS;
+    if (a == "guided" || a == "guided_mis") return LUCID_ALGO_GUIDED_MIS;  // extension
     return LUCID_ALGO_NEE;
 }
 
@@ -327,6 +328,7 @@ public:
             scene_ = nullptr;
         }
         scene_loaded_ = true;
+        if (ctx_) lucid_guiding_reset(ctx_);  // a new scene needs a new guide
         std::cerr << "[PyRenderer] Scene loaded: " << host_->meshes.size() << " meshes\n";
         return true;
     }
@@ -470,6 +472,50 @@ public:
         }
         path += " \xE2\x86\x92 Camera";
         return path;
+    }
+
+    // ---- learned path guiding (extension) ----
+    void set_guiding(float alpha, int train_steps, int batch, float lr, int stride, uint32_t seed) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        lucid_guiding_config c{alpha, train_steps, batch, lr, stride, seed};
+        check(lucid_set_guiding(ctx_, &c));
+    }
+    void reset_guiding() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        check(lucid_guiding_reset(ctx_));
+    }
+    py::dict guiding_stats() {
+        py::dict d;
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        lucid_guiding_stats st{};
+        check(lucid_guiding_get_stats(ctx_, &st));
+        d["samples_total"] = st.samples_total;
+        d["train_steps"] = st.train_steps;
+        d["last_loss"] = st.last_loss;
+        d["active"] = (bool)st.active;
+        d["num_params"] = st.num_params;
+        lucid_stats rs{};
+        lucid_get_stats(ctx_, &rs);
+        d["ms_guide_infer"] = rs.ms_guide_infer;
+        d["ms_guide_train"] = rs.ms_guide_train;
+        return d;
+    }
+    py::array_t<float> get_guiding_params() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        lucid_guiding_stats st{};
+        check(lucid_guiding_get_stats(ctx_, &st));
+        py::array_t<float> a(st.num_params);
+        check(lucid_guiding_get_params(ctx_, a.mutable_data(), st.num_params));
+        return a;
+    }
+    void set_guiding_params(py::array_t<float, py::array::c_style | py::array::forcecast> a) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_ctx();
+        check(lucid_guiding_set_params(ctx_, a.data(), (int32_t)a.size()));
     }
 
     // ---- diagnostics (pybind_renderer.hpp:885-1135) ----
@@ -792,6 +838,14 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
+        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.5f, py::arg("train_steps") = 8,
+             py::arg("batch") = 16384, py::arg("learning_rate") = 5e-3f, py::arg("record_stride") = 8,
+             py::arg("seed") = 1u,
+             "Extension: configure learned path guiding (algorithm 'guided') and start from a fresh network")
+        .def("reset_guiding", &Renderer::reset_guiding, "Extension: fresh guiding network")
+        .def("guiding_stats", &Renderer::guiding_stats, "Extension: training state of the guiding network")
+        .def("get_guiding_params", &Renderer::get_guiding_params)
+        .def("set_guiding_params", &Renderer::set_guiding_params, py::arg("params"))
         .def("get_diagnostic_film_bytes", &Renderer::get_diagnostic_film_bytes,
              "Extension: the film as PixelPathData<16,4> bytes, shape (sampled_h, sampled_w, 9576)")
         .def("get_diagnostic_strategy_stats", &Renderer::get_diagnostic_strategy_stats,

@@ -14,7 +14,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
-ALGOS = {"simple": 0, "nee": 1, "mis": 2, "volume_simple": 3, "volume_mis": 4}
+ALGOS = {"simple": 0, "nee": 1, "mis": 2, "volume_simple": 3, "volume_mis": 4, "guided": 5, "guided_mis": 5}
 LIGHT_TYPES = {"POINT": 1, "SUN": 2, "SPOT": 3, "AREA": 4}
 SHAPES = {"SQUARE": 0, "RECTANGLE": 1, "DISK": 2, "ELLIPSE": 3}
 ERRORS = {0: "LUCID_OK", 1: "LUCID_ERR_INVALID", 2: "LUCID_ERR_DEVICE", 3: "LUCID_ERR_CANCELLED",
@@ -58,7 +58,7 @@ class Stats(C.Structure):
                 ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_shadow", C.c_uint64),
                 ("tris_shadow", C.c_uint64), ("ms_intersect", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("launches_intersect", C.c_uint64), ("launches_shade", C.c_uint64),
-                ("launches_shadow", C.c_uint64)]
+                ("launches_shadow", C.c_uint64), ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +73,16 @@ class DiagInfo(C.Structure):
     _fields_ = [("valid", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("sampled_width", C.c_int32),
                 ("sampled_height", C.c_int32), ("subsample", C.c_int32), ("total_recorded", C.c_int64),
                 ("pixel_bytes", C.c_int64)]
+
+
+class GuidingConfig(C.Structure):
+    _fields_ = [("alpha", C.c_float), ("train_steps", C.c_int32), ("batch", C.c_int32),
+                ("learning_rate", C.c_float), ("record_stride", C.c_int32), ("seed", C.c_uint32)]
+
+
+class GuidingStats(C.Structure):
+    _fields_ = [("samples_total", C.c_int64), ("train_steps", C.c_int64), ("last_loss", C.c_float),
+                ("active", C.c_int32), ("num_params", C.c_int32)]
 
 
 SIGNATURES = {
@@ -106,6 +116,14 @@ SIGNATURES = {
     "lucid_diag_write_pixels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
     "lucid_diag_read_strategy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
     "lucid_diag_clear": (C.c_int, [C.c_void_p]),
+    "lucid_set_guiding": (C.c_int, [C.c_void_p, C.POINTER(GuidingConfig)]),
+    "lucid_guiding_reset": (C.c_int, [C.c_void_p]),
+    "lucid_guiding_get_stats": (C.c_int, [C.c_void_p, C.POINTER(GuidingStats)]),
+    "lucid_guiding_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32]),
+    "lucid_guiding_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32]),
+    "lucid_guiding_debug_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
+                                           C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                           C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,

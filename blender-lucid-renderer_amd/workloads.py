@@ -15,6 +15,10 @@ Workloads (all seeded; the seed is part of the returned spec):
   smooth normals; per object roughness U[0.05, 1], metallic Bernoulli(0.3),
   IOR U[1.3, 1.7]; one point, one sun and 4 area lights.  1024^2, 64 spp,
   depth 12.
+* ``occluded_room`` -- about 300k triangles: a glossy room (roughness
+  U[0.1, 0.4]) cut by a partition whose only opening is a 5 cm slit, with an
+  area light hidden behind it.  1920 x 1080, up to 1024 spp, depth 12; the
+  scene of the time-to-relMSE measurement (guided vs plain).
 """
 
 from __future__ import annotations
@@ -239,6 +243,88 @@ def smoke_box(sidecar_path: str, grid_n: int = 32, seed: int = 4, density: float
 SMOKE_BOX_CAMERA = CLOSED_BOX_CAMERA
 
 
+def _grid_quad(origin, du, dv, n):
+    """A planar quad origin + a*du + b*dv, a, b in [0, 1], as an n x n grid
+    of 2 n^2 triangles."""
+    a = np.linspace(0.0, 1.0, n + 1)
+    A, Bm = np.meshgrid(a, a, indexing="ij")
+    v = (np.asarray(origin, np.float64)[None] + A.reshape(-1, 1) * np.asarray(du, np.float64)[None]
+         + Bm.reshape(-1, 1) * np.asarray(dv, np.float64)[None])
+    i = np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]
+    i = i.reshape(-1)
+    t = np.concatenate([np.stack([i, i + n + 1, i + n + 2], 1), np.stack([i, i + n + 2, i + 1], 1)])
+    return v, t.astype(np.int64)
+
+
+def _glossy(rng, lo=0.1, hi=0.4):
+    return {"base_color": [float(x) for x in rng.uniform(0.55, 0.85, 3)], "metallic": 0.0,
+            "roughness": float(rng.uniform(lo, hi)), "emission": [0.0, 0.0, 0.0], "transmission": 0.0, "ior": 1.45}
+
+
+def occluded_room(seed: int = 5, grid: int = 150, spheres: int = 20) -> dict:
+    """An 8 m x 3 m x 8 m room cut by a partition wall at z = -3 whose only
+    opening is a 5 cm horizontal slit (y in [2.00, 2.05], x in [-3, 3]).  A
+    2 m x 1.2 m area light behind the partition faces it, hidden from the
+    camera in the front part; the front part is lit only through the slit.
+    Walls are finely tessellated glossy surfaces (roughness U[0.1, 0.4]),
+    plus `spheres` glossy icospheres on the floor: about 300k triangles at
+    the defaults."""
+    rng = np.random.default_rng(seed)
+    X0, X1, Y1, Z0, Z1 = -4.0, 4.0, 3.0, -6.0, 2.0
+    meshes = []
+    walls = [
+        ((X0, 0, Z0), (X1 - X0, 0, 0), (0, 0, Z1 - Z0)),      # floor
+        ((X0, Y1, Z0), (0, 0, Z1 - Z0), (X1 - X0, 0, 0)),     # ceiling
+        ((X0, 0, Z0), (0, Y1, 0), (X1 - X0, 0, 0)),           # back wall
+        ((X0, 0, Z1), (X1 - X0, 0, 0), (0, Y1, 0)),           # front wall (behind the camera)
+        ((X0, 0, Z0), (0, 0, Z1 - Z0), (0, Y1, 0)),           # left
+        ((X1, 0, Z0), (0, Y1, 0), (0, 0, Z1 - Z0)),           # right
+    ]
+    for o, du, dv in walls:
+        v, t = _grid_quad(o, du, dv, grid)
+        meshes.append(_mesh(v, t, _glossy(rng)))
+    # partition (0.1 m thick) with the slit
+    zc, hz = -3.0, 0.05
+    parts = [((0.0, 1.0, zc), (4.0, 1.0, hz)),            # below the slit, y in [0, 2]
+             ((0.0, 2.525, zc), (4.0, 0.475, hz)),        # above the slit, y in [2.05, 3]
+             ((-3.5, 2.025, zc), (0.5, 0.025, hz)),       # slit ends
+             ((3.5, 2.025, zc), (0.5, 0.025, hz))]
+    pmat = _glossy(rng)
+    for c, h in parts:
+        v, t = _box(c, h, 0.0)
+        meshes.append(_mesh(v, t, pmat))
+    bv, bt = _icosphere(3)
+    for _ in range(spheres):
+        r = float(rng.uniform(0.25, 0.55))
+        c = np.array([rng.uniform(-3.2, 3.2), r, rng.uniform(-2.5, 0.5)])
+        meshes.append(_mesh(bv * r + c, bt, _glossy(rng), normals=None))
+    radiance, sx, sy = 400.0, 2.0, 1.2
+    light = {"type": "AREA", "name": "HiddenLight", "position": [0.0, 1.4, -5.9], "direction": [0.0, 0.0, 1.0],
+             "right": [1.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0], "shape": "RECTANGLE", "size": sx, "size_y": sy,
+             "color": [1.0, 0.95, 0.85], "energy": radiance * math.pi * sx * sy}
+    return {"meshes": meshes, "native_lights": [light],
+            "environment": {"color": [0.0, 0.0, 0.0], "strength": 0.0}}
+
+
+OCCLUDED_ROOM_CAMERA = {"pos": (0.0, 1.5, 1.9), "dir": (0.0, 0.08, -1.0), "up": (0.0, 1.0, 0.0), "fov": 65.0}
+
+
+def relmse_pixels(width: int = 1920, height: int = 1080, nx: int = 128, ny: int = 128):
+    """Stratified pixel subset of the relMSE measurement: the centre of each
+    cell of an nx x ny grid over the frame (16,384 pixels by default)."""
+    xs = ((np.arange(nx) + 0.5) * width / nx).astype(np.int32)
+    ys = ((np.arange(ny) + 0.5) * height / ny).astype(np.int32)
+    X, Y = np.meshgrid(xs, ys)
+    return X.reshape(-1), Y.reshape(-1)
+
+
+def relmse(x: np.ndarray, r: np.ndarray, eps: float = 0.01) -> float:
+    """mean over pixels and channels of (x - r)^2 / (r^2 + eps)."""
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + eps)))
+
+
 def to_json(scene: dict) -> str:
     return json.dumps(scene, separators=(",", ":"))
 
@@ -248,4 +334,6 @@ WORKLOADS = {
                    "max_depth": 8, "algorithm": "mis"},
     "icospheres": {"make": icospheres, "camera": ICOSPHERES_CAMERA, "width": 1024, "height": 1024, "spp": 64,
                    "max_depth": 12, "algorithm": "mis"},
+    "occluded_room": {"make": occluded_room, "camera": OCCLUDED_ROOM_CAMERA, "width": 1920, "height": 1080,
+                      "spp": 1024, "max_depth": 12, "algorithm": "mis"},
 }

@@ -49,6 +49,9 @@ extern "C" {
 #define LUCID_ALGO_MIS 2
 #define LUCID_ALGO_VOLUME_SIMPLE 3
 #define LUCID_ALGO_VOLUME_MIS 4
+/* Extension, no counterpart in the reference: traceMIS with learned path
+   guiding at opaque non-delta surfaces (lucid_set_guiding). */
+#define LUCID_ALGO_GUIDED_MIS 5
 
 /* native light types (include/light/light.hpp:22-28) and area shapes (:30-35) */
 #define LUCID_LIGHT_POINT 1
@@ -136,6 +139,9 @@ typedef struct lucid_stats {
        the intersect, shade and shadow kernels */
     double ms_intersect, ms_shade, ms_shadow;
     uint64_t launches_intersect, launches_shade, launches_shadow;
+    /* with LUCID_PROFILE_KERNELS, guided renders: guiding-network inference
+       and the training after the render */
+    double ms_guide_infer, ms_guide_train;
 } lucid_stats;
 
 #define LUCID_PROFILE_WORK 1    /* count node / triangle fetches */
@@ -242,6 +248,54 @@ int lucid_diag_write_pixels(lucid_ctx* ctx, int64_t first, int64_t count, const 
 int lucid_diag_read_strategy(lucid_ctx* ctx, int64_t first, int64_t count, double* dst);
 /* DiagnosticFilm::clear (diagnostic_film.hpp:305-312). */
 int lucid_diag_clear(lucid_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+   Learned path guiding (extension; no counterpart in the reference).
+   LUCID_ALGO_GUIDED_MIS renders traceMIS where the bounce direction at
+   opaque non-delta surfaces comes from the one-sample mixture
+     pdf = alpha * p_guide + (1 - alpha) * p_bsdf
+   of the BSDF and an MLP (64 frequency-encoded inputs, 3 x 64 ReLU, 8 von
+   Mises-Fisher lobes; BF16 weights, FP32 accumulation on MFMA), with NEE
+   weighted against the same mixture.  The network is trained online: every
+   guided render records, for 1 path in record_stride, each bounce's
+   (position, normal, outgoing, incoming direction, lum(W_bsdf * L_in)), and
+   after the render runs train_steps Adam steps of batch `batch` on
+   -sum w log p_guide(wi) / sum w.  Until the first training step the guided
+   render samples the BSDF only (alpha = 0).  The state lives in the context.
+   ------------------------------------------------------------------------ */
+typedef struct lucid_guiding_config {
+    float alpha;           /* guide-sampling probability, 0..1 */
+    int32_t train_steps;   /* Adam steps after each guided render (0: frozen) */
+    int32_t batch;         /* training batch, <= 65536 */
+    float learning_rate;
+    int32_t record_stride; /* 1 path in N records training samples, >= 1 */
+    uint32_t seed;         /* network initialisation */
+} lucid_guiding_config;
+
+typedef struct lucid_guiding_stats {
+    int64_t samples_total;  /* training samples recorded so far */
+    int64_t train_steps;    /* Adam steps taken so far */
+    float last_loss;        /* loss of the last step */
+    int32_t active;         /* 1 once the network has been trained */
+    int32_t num_params;     /* FP32 master parameters */
+} lucid_guiding_stats;
+
+/* Sets the configuration and starts from a fresh network. */
+int lucid_set_guiding(lucid_ctx* ctx, const lucid_guiding_config* cfg);
+/* Fresh network with the current configuration (e.g. after a scene change). */
+int lucid_guiding_reset(lucid_ctx* ctx);
+int lucid_guiding_get_stats(lucid_ctx* ctx, lucid_guiding_stats* out);
+/* FP32 master parameters (num_params floats): per layer l, W_l [in][out]
+   row-major then b_l; layers 64x64, 64x64, 64x64, 64x48 (columns 40..47 0). */
+int lucid_guiding_get_params(lucid_ctx* ctx, float* params, int32_t n);
+int lucid_guiding_set_params(lucid_ctx* ctx, const float* params, int32_t n);
+/* One forward / backward pass of the training kernels over exactly n
+   samples (16 floats each: position, normal, outgoing, incoming direction,
+   weight, 3 unused) with `params`, positions mapped through the box
+   [bmin, bmax]: raw outputs (n x 40), gradient (num_params) and loss.  For
+   the gradient check against the CPU reference. */
+int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
+                             const float bmin[3], const float bmax[3], float* out, float* grad, float* loss);
 
 /* Runs the device BVH traversal code on the CPU over a scene packed from
    `meshes` (no device needed), checking every node and triangle index before

@@ -2497,4 +2497,34 @@ void orc_diag_pixel_run(int max_groups, int max_outliers, int n, const int32_t* 
     if (bytes && max_groups == 16 && max_outliers == 4) d_serialise(d, bytes);
 }
 
+// Per-pixel FP64 sums of the radiance of `samples` samples (render_tile's
+// camera rays with samples_in_call = `samples_in_call`, seeds
+// sample_offset + s) at the listed pixels: sum[3n], sumsq[3n].  For the
+// high-spp reference of the relMSE measurement.
+int orc_render_pixels(void* p, const char* algorithm, int fw, int fh, int n, const int32_t* xs, const int32_t* ys,
+                      int samples, int samples_in_call, long long sample_offset, int max_depth, int nthreads,
+                      double* sum, double* sumsq) {
+    Scene& s = *static_cast<Scene*>(p);
+    if (!s.cam.set) return 1;
+    const int algo = algo_of(algorithm);
+    RayGen g = make_raygen(s, fw, fh);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < n; ++i) {
+        double a[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+        for (int si = 0; si < samples; ++si) {
+            Rng rng;
+            Ray r = camera_ray(g, xs[i], ys[i], (uint32_t)(sample_offset + si), samples_in_call, rng);
+            SampleOut so;
+            V3 L = trace(s, algo, r, max_depth, rng, so);
+            const double v[3] = {L.x, L.y, L.z};
+            for (int c = 0; c < 3; ++c) { a[c] += v[c]; q[c] += v[c] * v[c]; }
+        }
+        for (int c = 0; c < 3; ++c) { sum[3 * i + c] += a[c]; sumsq[3 * i + c] += q[c]; }
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -159,6 +159,7 @@ def lib(fast: bool = False) -> C.CDLL:
     L.orc_select_light.argtypes = [vp, C.c_float, fp]
     L.orc_pdf_light_area.argtypes = [C.c_float, fp, fp, fp]
     u8p = C.POINTER(C.c_uint8)
+    L.orc_render_pixels.argtypes = [vp, C.c_char_p, ip, ip, ip, i32p, i32p, ip, ip, C.c_longlong, ip, ip, dp, dp]
     L.orc_render_tile_diag.argtypes = [vp, C.c_char_p] + [ip] * 11 + [fp, u8p, dp, i64p]
     L.orc_diag_pixel_run.argtypes = [ip, ip, ip, i32p, i32p, C.POINTER(C.c_int16), u8p, u8p, fp, u8p, u8p,
                                      C.POINTER(C.c_uint64), u8p, i64p, C.POINTER(C.c_uint64),
@@ -343,6 +344,21 @@ class OracleScene:
             raise RuntimeError(f"orc_render_tile_diag failed ({rc})")
         film = film[: sw * sh * PIXEL_DT.itemsize].view(PIXEL_DT).reshape(sh, sw)
         return out, film, strat[: sw * sh * 28].reshape(sh, sw, 4, 7), int(tot[0])
+
+    def render_pixels(self, algorithm, fw, fh, xs, ys, samples, samples_in_call, sample_offset, max_depth=8,
+                      threads=0, acc=None):
+        """FP64 radiance sums over `samples` samples at pixels (xs, ys):
+        returns (sum (n, 3), sumsq (n, 3)), added onto `acc` when given."""
+        xs = np.ascontiguousarray(xs, np.int32)
+        ys = np.ascontiguousarray(ys, np.int32)
+        n = len(xs)
+        sm, sq = acc if acc is not None else (np.zeros((n, 3)), np.zeros((n, 3)))
+        rc = self.L.orc_render_pixels(self.h, algorithm.encode(), fw, fh, n, _iptr(xs), _iptr(ys), samples,
+                                      samples_in_call, sample_offset, max_depth, threads,
+                                      sm.ctypes.data_as(C.POINTER(C.c_double)), sq.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError("camera not set")
+        return sm, sq
 
     def trace_pixel(self, algorithm, gx, gy, fw, fh, samples_in_call, sample_offset, nsamples, max_depth=8):
         out = np.zeros((nsamples, 3), dtype=np.float32)
