@@ -1,21 +1,36 @@
-"""Benchmark of the MI355X path tracer: Mrays/s on the icospheres workload.
+"""Benchmark of the MI355X path tracer on the metric's workload: Mrays/s and
+time-to-relMSE 0.01 (guided vs plain) at 1080p.
 
-Workload (one "step" = one full-frame render_tile pass): 200 displaced
-icospheres = 1,024,000 triangles, Principled materials, 1 point + 1 sun + 4
-area lights, 1024 x 1024 pixels, 64 samples per pixel, max depth 12, MIS.
-The scene is packed and uploaded before the timed region; the output stays in
-device memory (lucid_render_tile_device).  Mrays/s = (closest-hit + shadow
-queries) / wall time of the timed steps.
+Workload (default ``occluded_room``): a 295,648-triangle glossy room
+(roughness U[0.1, 0.4]) lit only through a 5 cm slit by a hidden area light,
+1920 x 1080, traceMIS at depth 12.  One "step" = one progressive pass of
+16 samples per pixel over the full frame (lucid_render_tile_device; the scene
+is packed and uploaded before the timed region, the output stays in device
+memory).  Mrays/s = (closest-hit + shadow queries) / wall time of the timed
+plain passes.
 
-Also reported:
-* roofline of the dominant kernel (BVH closest-hit traversal) from one
-  instrumented pass: algorithmic bytes per launch (ray in 32 B + hit out 16 B
-  per query, 64 B per BVH node and 48 B per triangle fetched, counted on the
-  device) over the HIP-event average launch duration, against 8 TB/s;
-* cpu_baseline: the oracle's -O3 -ffast-math build of the reference
-  algorithm on the host cores, on a bounded strip of the same frame.
+time-to-relMSE: plain MIS and guided MIS (online-trained BF16 MFMA guiding
+network, training time included) each render progressive 16-spp passes for
+a fixed time budget.  After every pass the running mean is compared with
+the FP64-accumulated oracle image on a stratified subset of 16,384 pixels
+(tests/golden/occluded_room_reference.npz, written by
+tests/golden/make_occluded_room_reference.py):
+relMSE = mean over pixels and channels of (x - r)^2 / (r^2 + 0.01), minus the
+reference's own estimated relMSE (sample variance / spp), which the
+independent noise of the reference adds on average.  The time at which the
+corrected relMSE crosses 0.01 is reported as measured when a run gets there
+within its budget, else extrapolated from relMSE * t over the last half of
+the run (relMSE proportional to 1/t), and labelled as such.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+Also reported: the roofline of the dominant kernel (BVH closest-hit
+traversal) from one instrumented pass -- algorithmic bytes per launch (ray
+32 B + hit 16 B per query, 64 B per BVH node and 48 B per triangle fetched,
+counted on the device) over the HIP-event average launch duration, against
+8 TB/s -- and cpu_baseline: the oracle's -O3 -ffast-math build of the same
+algorithm on the host cores, on a bounded strip of the same frame.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--budget SECONDS]
+       python bench.py --workload icospheres      (1M triangles, 1024^2, 64 spp: Mrays/s only)
 """
 
 from __future__ import annotations
@@ -35,27 +50,26 @@ for _p in (PKG, ORACLE):
 
 METRIC = "Mrays/s + time-to-relMSE 0.01 (guided vs plain), 1080p, 1×MI355X"
 HBM_PEAK_GBS = 8000.0
+REFERENCE = os.path.join(ROOT, "tests", "golden", "occluded_room_reference.npz")
+TARGET = 0.01
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, cam, W, H, spp, depth, seconds_target=15.0):
+def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
     """Times the oracle (fast build) on a strip of rows of the same frame."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     o = oracle.OracleScene(scene, fast=True)
     o.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"])
-    # calibrate on a small strip, then size the strip for ~seconds_target
-    rows, samples = 16, 2
+    rows, samples = 8, 2
     t0 = time.perf_counter()
-    _, ex = o.render_tile("mis", 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
+    o.render_tile("mis", 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0
-    rays = int(ex["counts"].sum())
-    scale = max(1.0, min(64.0, seconds_target / max(dt, 1e-3)))
-    rows2 = int(min(H, rows * scale))
+    rows2 = int(max(1, min(H, rows * seconds_target / max(dt, 1e-3))))
     y0 = max(0, H // 2 - rows2 // 2)
     t0 = time.perf_counter()
     _, ex = o.render_tile("mis", 0, y0, W, rows2, W, H, samples=samples, max_depth=depth, threads=threads)
@@ -63,18 +77,39 @@ def cpu_baseline(scene, cam, W, H, spp, depth, seconds_target=15.0):
     rays = int(ex["counts"].sum())
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{W}x{rows2} rows {y0}..{y0 + rows2 - 1} of the {W}x{H} frame, {samples} spp, depth {depth}, "
-                      f"mis; {rays} queries in {dt:.1f} s (oracle -O3 -ffast-math, OpenMP)",
-            "spp_full_frame_s_est": round(dt * (H / rows2) * (spp / samples), 1)}
+                      f"mis; {rays} queries in {dt:.1f} s (oracle -O3 -ffast-math, OpenMP)"}
+
+
+def time_to_target(curve):
+    """curve: [(seconds, relMSE)].  Measured crossing (linear interpolation
+    between checkpoints) or extrapolation from relMSE * t over the last half."""
+    for i, (t, e) in enumerate(curve):
+        if e <= TARGET:
+            if i == 0:
+                return t, "measured"
+            t0, e0 = curve[i - 1]
+            return t0 + (t - t0) * (e0 - TARGET) / max(e0 - e, 1e-30), "measured"
+    tail = curve[len(curve) // 2:]
+    import statistics
+    c = statistics.median([t * e for t, e in tail])
+    return c / TARGET, "extrapolated (relMSE ~ 1/t over the last half of the run)"
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="icospheres")
+    ap.add_argument("--workload", default="occluded_room")
+    ap.add_argument("--budget", type=float, default=45.0, help="seconds of progressive rendering per tracer")
+    ap.add_argument("--pass-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--wave-paths", type=int, default=0)
+    ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE leg")
+    ap.add_argument("--alpha", type=float, default=0.7)
+    ap.add_argument("--train-steps", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=32768)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--record-stride", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -90,7 +125,9 @@ def main():
     import workloads
 
     wl = workloads.WORKLOADS[args.workload]
-    W, H, spp, depth, algo = wl["width"], wl["height"], wl["spp"], wl["max_depth"], wl["algorithm"]
+    W, H, depth, algo = wl["width"], wl["height"], wl["max_depth"], wl["algorithm"]
+    relmse_leg = args.workload == "occluded_room" and not args.no_relmse
+    spp = args.pass_spp if args.workload == "occluded_room" else wl["spp"]
     cam = wl["camera"]
     t0 = time.perf_counter()
     scene = wl["make"]()
@@ -102,39 +139,37 @@ def main():
     sc = ctx.scene(scene)
     ntri = L.lucid_scene_num_triangles(sc)
     log(f"scene packed+uploaded: {ntri} triangles in {time.perf_counter() - t0:.1f} s")
-    if args.wave_paths:
-        ctx.check(L.lucid_set_wave_paths(ctx.h, args.wave_paths))
     camd = lucid_capi.camera(cam)
-    prm = lucid_capi.params(0, 0, W, H, W, H, spp, 0, depth, algo)
     out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream().cuda_stream
 
-    def step():
+    def render(algorithm, n_spp, offset):
+        prm = lucid_capi.params(0, 0, W, H, W, H, n_spp, offset, depth, algorithm)
         ctx.check(L.lucid_render_tile_device(ctx.h, sc, C.byref(camd), C.byref(prm), C.c_void_p(out.data_ptr()),
                                              C.c_void_p(stream)))
         ctx.check(L.lucid_synchronize(ctx.h))
         return ctx.stats()
 
+    # ---- timed plain passes: Mrays/s ----
     for i in range(args.warmup):
-        s = step()
+        s = render(algo, spp, 1000000 + i * spp)
         log(f"warmup {i}: {s['ms_render']:.1f} ms, {s['rays_closest'] + s['rays_shadow']} rays")
     torch.cuda.synchronize()
     rays = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        s = step()
+        s = render(algo, spp, i * spp)
         rays += s["rays_closest"] + s["rays_shadow"]
         log(f"step {i}: {s['ms_render']:.1f} ms")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     mrays = rays / dt / 1e6
     ms_step = dt / args.steps * 1e3
-    img = out.view(H, W, 4)
-    assert torch.isfinite(img).all().item(), "non-finite pixels"
+    assert torch.isfinite(out).all().item(), "non-finite pixels"
 
-    # instrumented pass: node / triangle fetch counts + per-kernel event timing
+    # ---- instrumented pass: node / triangle fetch counts + per-kernel event timing ----
     ctx.check(L.lucid_set_profiling(ctx.h, lucid_capi.PROFILE_WORK | lucid_capi.PROFILE_KERNELS))
-    p = step()
+    p = render(algo, spp, 2000000)
     ctx.check(L.lucid_set_profiling(ctx.h, 0))
     q = p["rays_closest"]
     bytes_closest = q * (32 + 16) + p["nodes_closest"] * 64 + p["tris_closest"] * 48
@@ -145,6 +180,8 @@ def main():
     log(f"intersect: {n_launch} launches, avg {avg_ms:.3f} ms, nodes/query {p['nodes_closest'] / max(q, 1):.1f}, "
         f"tris/query {p['tris_closest'] / max(q, 1):.1f}; shade {p['ms_shade']:.1f} ms, shadow {p['ms_shadow']:.1f} ms")
 
+    lights = {"occluded_room": "1 area light behind a 5 cm slit",
+              "icospheres": "1 point + 1 sun + 4 area lights"}.get(args.workload, "")
     result = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -158,8 +195,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (procedural scene, seeded)",
-        "config": {"workload": f"{args.workload}: {ntri} triangles, {W}x{H}, {spp} spp, depth {depth}, {algo}, "
-                               f"1 point + 1 sun + 4 area lights",
+        "config": {"workload": f"{args.workload}: {ntri} triangles, {W}x{H}, {spp} spp per step, depth {depth}, "
+                               f"{algo}, {lights}",
                    "rays_per_step": rays // max(args.steps, 1), "paths_per_step": W * H * spp,
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",
@@ -171,9 +208,61 @@ def main():
                                    "tris": round(p["tris_closest"] / max(q, 1), 2),
                                    "node_bytes": 64, "tri_bytes": 48, "ray_bytes": 32, "hit_bytes": 16}},
     }
+
+    # ---- time-to-relMSE 0.01, guided vs plain ----
+    if relmse_leg:
+        d = np.load(REFERENCE)
+        xs, ys, rspp = d["xs"], d["ys"], int(d["spp"])
+        R = d["sum"] / rspp
+        rvar = np.maximum(d["sumsq"] / rspp - R * R, 0.0) * rspp / max(rspp - 1, 1)
+        ref_err = float(np.mean(rvar / rspp / (R * R + 0.01)))
+        idx = torch.tensor(((H - 1 - ys) * W + xs).astype(np.int64), device="cuda")  # rows are bottom-first
+        Rt = torch.tensor(R, device="cuda", dtype=torch.float64)
+
+        def progressive(algorithm, seed_base):
+            acc = torch.zeros(len(xs), 3, dtype=torch.float64, device="cuda")
+            t_run, n, curve, i = 0.0, 0, [], 0
+            while t_run < args.budget:
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+                render(algorithm, spp, seed_base + i * spp)
+                torch.cuda.synchronize()
+                t_run += time.perf_counter() - ts
+                n += spp
+                i += 1
+                acc += out.view(-1, 4)[idx, :3].double()
+                x = acc / n
+                e = torch.mean((x - Rt) ** 2 / (Rt * Rt + 0.01)).item() - ref_err
+                curve.append((t_run, e))
+            log(f"{algorithm}: {n} spp in {t_run:.1f} s, relMSE {curve[-1][1]:.4f}")
+            return curve, n
+
+        plain_curve, plain_spp = progressive("mis", 0)
+        gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7)
+        ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
+        guided_curve, guided_spp = progressive("guided", 0)
+        gst = lucid_capi.GuidingStats()
+        ctx.check(L.lucid_guiding_get_stats(ctx.h, C.byref(gst)))
+        tp, mp = time_to_target(plain_curve)
+        tg, mg = time_to_target(guided_curve)
+        # equal-time comparison at the end of the shorter run
+        t_eq = min(plain_curve[-1][0], guided_curve[-1][0])
+        at = lambda cv: min(cv, key=lambda te: abs(te[0] - t_eq))[1]
+        result["time_to_relmse"] = {
+            "target": TARGET, "pixels": int(len(xs)), "pass_spp": spp, "budget_s": args.budget,
+            "plain_s": round(tp, 2), "plain_method": mp, "guided_s": round(tg, 2), "guided_method": mg,
+            "speedup": round(tp / tg, 3),
+            "plain_relmse_end": round(plain_curve[-1][1], 5), "plain_spp": plain_spp,
+            "guided_relmse_end": round(guided_curve[-1][1], 5), "guided_spp": guided_spp,
+            "equal_time_relmse_ratio": round(at(plain_curve) / max(at(guided_curve), 1e-12), 3),
+            "reference": {"spp": rspp, "self_relmse": round(ref_err, 5), "kind": "oracle FP64, stratified subset"},
+            "guiding": {"alpha": args.alpha, "train_steps_per_pass": args.train_steps, "batch": args.batch,
+                        "lr": args.lr, "record_stride": args.record_stride, "samples": int(gst.samples_total),
+                        "adam_steps": int(gst.train_steps), "last_loss": round(float(gst.last_loss), 4)},
+        }
     if not args.no_cpu:
         log("cpu baseline (oracle, fast build) ...")
-        result["cpu_baseline"] = cpu_baseline(scene, cam, W, H, spp, depth)
+        result["cpu_baseline"] = cpu_baseline(scene, cam, W, H, depth)
     print(json.dumps(result), flush=True)
     L.lucid_scene_release(sc)
     ctx.close()

@@ -12,11 +12,10 @@ namespace guide {
 
 // Input feature idx (0..63) of a surface point: pn = position in the scene
 // box mapped to [0,1]^3, n = shading normal, wo = outgoing direction.
-//   0..35   sin / cos (2^k pi pn_c), k = 0..5
-//   36..38  2 pn - 1
-//   39..41  n,  42..44 wo
-//   45..47  sin(pi n),  48..50 sin(pi wo),  51 n . wo
-//   52..63  0
+//   0..47   sin / cos (2^k pi pn_c), k = 0..7: feature 3 q + c, q = 2k (sin) or 2k + 1 (cos)
+//   48..50  2 pn - 1
+//   51..53  n,  54..56 wo
+//   57..59  sin(pi n),  60..62 sin(pi wo),  63 n . wo
 __device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f wo) {
     if (idx < kFeatPos) {
         const int c = idx % 3, q = idx / 3, k = q >> 1;
@@ -24,13 +23,35 @@ __device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f 
         return (q & 1) ? cosf(a) : sinf(a);
     }
     const float nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
-    if (idx < 39) return 2.0f * pn[idx - 36] - 1.0f;
-    if (idx < 42) return nn[idx - 39];
-    if (idx < 45) return ww[idx - 42];
-    if (idx < 48) return sinf(PI_F * nn[idx - 45]);
-    if (idx < 51) return sinf(PI_F * ww[idx - 48]);
-    if (idx == 51) return dot(n, wo);
-    return 0.0f;
+    if (idx < 51) return 2.0f * pn[idx - 48] - 1.0f;
+    if (idx < 54) return nn[idx - 51];
+    if (idx < 57) return ww[idx - 54];
+    if (idx < 60) return sinf(PI_F * nn[idx - 57]);
+    if (idx < 63) return sinf(PI_F * ww[idx - 60]);
+    return dot(n, wo);
+}
+
+// All 64 features at once: sin / cos (2^k pi pn_c) from one sincos per
+// coordinate and the double-angle recurrence (the values feature() defines,
+// to a few float ulps).
+__device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float f[kIn]) {
+    for (int c = 0; c < 3; ++c) {
+        float sn, cs;
+        sincosf(PI_F * pn[c], &sn, &cs);
+        for (int k = 0; k < kOctaves; ++k) {
+            f[3 * (2 * k) + c] = sn;
+            f[3 * (2 * k + 1) + c] = cs;
+            const float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
+            sn = s2;
+            cs = c2;
+        }
+    }
+    f[48] = 2.0f * pn[0] - 1.0f; f[49] = 2.0f * pn[1] - 1.0f; f[50] = 2.0f * pn[2] - 1.0f;
+    f[51] = n.x; f[52] = n.y; f[53] = n.z;
+    f[54] = wo.x; f[55] = wo.y; f[56] = wo.z;
+    f[57] = sinf(PI_F * n.x); f[58] = sinf(PI_F * n.y); f[59] = sinf(PI_F * n.z);
+    f[60] = sinf(PI_F * wo.x); f[61] = sinf(PI_F * wo.y); f[62] = sinf(PI_F * wo.z);
+    f[63] = dot(n, wo);
 }
 
 // The mixture decoded from the 40 raw outputs: lobe k has mean direction

@@ -29,6 +29,9 @@ void train(State* s, const GuideDev& g, int steps, int batch, float lr, hipStrea
 // raw outputs (n x 40), parameter gradient (kParams) and loss.  For tests.
 int debug_step(State* s, const float* params, const Sample* samples, int n, const float bmin[3], const float bmax[3],
                float* out, float* grad, float* loss, hipStream_t st);
+// The rendering-side forward pass on explicit samples: raw outputs (n x 40).
+int debug_infer(State* s, const float* params, const Sample* samples, int n, const float bmin[3], const float bmax[3],
+                float* out, hipStream_t st);
 int get_params(State* s, float* params, hipStream_t st);
 float last_loss(State* s, hipStream_t st);
 int set_params(State* s, const float* params, hipStream_t st);

@@ -77,5 +77,45 @@ __device__ __forceinline__ void mlp_forward(const GuideDev& G, bf16x8 x0, bf16x8
     }
 }
 
+// Forward pass of NT tiles (16 * NT samples) per wavefront: each weight
+// operand is loaded once and applied to every tile.  No activations kept.
+template <int NT>
+__device__ __forceinline__ void mlp_forward_tiles(const GuideDev& G, bf16x8 (&b0)[NT], bf16x8 (&b1)[NT],
+                                                  f32x4 (&out)[NT][3]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        const int mt = layer_mtiles(l);
+        const float* bias = G.params + b_offset(l);
+        f32x4 acc[NT][4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (m >= mt) break;
+            const uint4* a = G.fwd + (fwd_offset(l) + (m * 2) * 512) / 8 + lane;
+            const bf16x8 a0 = as_bf16x8(a[0]), a1 = as_bf16x8(a[64]);
+            f32x4 bv;
+            for (int j = 0; j < 4; ++j) bv[j] = bias[16 * m + 4 * g + j];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                acc[t][m] = mfma(a0, b0[t], bv);
+                acc[t][m] = mfma(a1, b1[t], acc[t][m]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            if (l < kLayers - 1) {
+                for (int m = 0; m < 4; ++m)
+                    for (int j = 0; j < 4; ++j) acc[t][m][j] = fmaxf(acc[t][m][j], 0.0f);
+                b0[t] = pack_b(acc[t][0], acc[t][1]);
+                b1[t] = pack_b(acc[t][2], acc[t][3]);
+            } else {
+                out[t][0] = acc[t][0];
+                out[t][1] = acc[t][1];
+                out[t][2] = acc[t][2];
+            }
+        }
+    }
+}
+
 }  // namespace guide
 }  // namespace lucid

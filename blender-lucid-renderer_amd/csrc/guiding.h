@@ -2,8 +2,8 @@
 //
 // A small MLP maps a surface point to a mixture of 8 von Mises-Fisher lobes
 // over directions:
-//   input  64 features: frequency-encoded position (scene box -> [0,1]^3),
-//          shading normal and outgoing direction (guide_feature)
+//   input  64 features: frequency-encoded position (scene box -> [0,1]^3,
+//          8 octaves), shading normal and outgoing direction
 //   hidden 3 x 64, ReLU
 //   output 40 = 8 lobes x (mean direction xyz, log concentration, weight logit)
 // Weights are BF16, arithmetic FP32 (MFMA accumulate).  The guided bounce
@@ -29,7 +29,7 @@ namespace lucid {
 namespace guide {
 
 constexpr int kIn = 64, kHid = 64, kOut = 40, kOutPad = 48, kLobes = 8, kLayers = 4;
-constexpr int kFeatPos = 36;  // frequency features of the position
+constexpr int kOctaves = 8, kFeatPos = 6 * kOctaves;  // frequency features of the position
 
 // per layer: input width, output width (padded to whole 16-column tiles)
 constexpr int layer_in(int l) { return l == 0 ? kIn : kHid; }

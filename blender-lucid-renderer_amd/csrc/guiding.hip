@@ -6,8 +6,8 @@
 //   k_fwd         MFMA forward pass, stores BF16 activations and FP32 outputs
 //   k_loss        d loss / d outputs of  L = -sum_s w_s log p(wi_s) / sum_s w_s
 //   k_bwd         MFMA backward pass through the hidden layers (BF16 deltas)
-//   k_wgrad       weight gradients dW_l = X_l^T dY_l, split-K MFMA, FP32 atomics
-//   k_bgrad       bias gradients
+//   k_wgrad       weight and bias gradients dW_l = X_l^T dY_l, db_l = 1^T dY_l,
+//                 split-K MFMA, FP32 atomics
 //   k_adam        Adam on the FP32 master parameters, then k_pack repacks
 //                 the BF16 MFMA operands
 // Samples come from the tracer (weight lum(W_bsdf * L_incident) of a guided
@@ -101,9 +101,11 @@ __device__ __forceinline__ void encode(const GuideDev& G, const Sample& s, bf16x
     float pn[3];
     for (int c = 0; c < 3; ++c) pn[c] = (s.p[c] - G.bmin[c]) * G.binv[c];
     const v3f n = mk(s.n[0], s.n[1], s.n[2]), wo = mk(s.wo[0], s.wo[1], s.wo[2]);
+    float f[kIn];
+    features(pn, n, wo, f);
     for (int i = 0; i < 8; ++i) {
-        x0[i] = (__bf16)feature(perm(0, g, i), pn, n, wo);
-        x1[i] = (__bf16)feature(perm(1, g, i), pn, n, wo);
+        x0[i] = (__bf16)f[perm(0, g, i)];
+        x1[i] = (__bf16)f[perm(1, g, i)];
     }
 }
 
@@ -223,54 +225,44 @@ __global__ __launch_bounds__(64) void k_bwd(GuideDev G, int batch, const __bf16*
 }
 
 // dW_l[in][out] = sum_b X_l[b][in] dY_l[b][out]; X_l = act[b][l].  One
-// wavefront per (layer, m, n) output tile and K range; FP32 atomics.
+// wavefront per (layer, m, n) output tile and K range; FP32 atomics.  Tile
+// row m = in / 16 (4 per layer) plus m = 4, the bias row: its A operand is a
+// constant 1 in row 0, so row 0 of the product is db_l[out] = sum_b dY_l[b][out].
 __global__ __launch_bounds__(64) void k_wgrad(int batch, int chunks_per_wave, const __bf16* act, const __bf16* dy,
                                               float* grad) {
     const int lane = threadIdx.x, g = lane >> 4, s = lane & 15;
-    int tid = blockIdx.x;
-    int l = 0, t = tid;
+    int t = blockIdx.x, l = 0;
     for (; l < kLayers; ++l) {
-        const int nt = (layer_in(l) / 16) * layer_mtiles(l);
+        const int nt = (layer_in(l) / 16 + 1) * layer_mtiles(l);
         if (t < nt) break;
         t -= nt;
     }
-    const int split = blockIdx.y;
     if (l >= kLayers) return;
     const int ntile = layer_mtiles(l);
     const int m = t / ntile, n = t % ntile;
+    const bool bias = m == layer_in(l) / 16;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const int nchunks = (batch + 31) / 32;
-    const int c0 = split * chunks_per_wave, c1 = min(nchunks, c0 + chunks_per_wave);
+    const int c0 = blockIdx.y * chunks_per_wave, c1 = min(nchunks, c0 + chunks_per_wave);
     for (int c = c0; c < c1; ++c) {
         bf16x8 a, b;
         for (int i = 0; i < 8; ++i) {
             const int bb = 32 * c + 8 * g + i;
             const bool ok = bb < batch;
-            a[i] = ok ? act[((size_t)bb * 4 + l) * 64 + 16 * m + s] : (__bf16)0.0f;
+            if (bias) a[i] = (ok && s == 0) ? (__bf16)1.0f : (__bf16)0.0f;
+            else a[i] = ok ? act[((size_t)bb * 4 + l) * 64 + 16 * m + s] : (__bf16)0.0f;
             b[i] = ok ? dy[((size_t)bb * 4 + l) * 64 + 16 * n + s] : (__bf16)0.0f;
         }
         acc = mfma(a, b, acc);
     }
-    // acc[j] = dW[in = 16m + 4g + j][out = 16n + s]
-    for (int j = 0; j < 4; ++j) {
-        const int in = 16 * m + 4 * g + j, out = 16 * n + s;
-        if (out < (l == kLayers - 1 ? kOut : layer_out(l)))
-            atomicAdd(&grad[w_offset(l) + in * layer_out(l) + out], acc[j]);
+    // acc[j] = product row 4g + j, column 16n + s
+    const int out = 16 * n + s;
+    if (out >= (l == kLayers - 1 ? kOut : layer_out(l))) return;
+    if (bias) {
+        if (g == 0) atomicAdd(&grad[b_offset(l) + out], acc[0]);
+        return;
     }
-}
-
-__global__ void k_bgrad(int batch, const __bf16* dy, float* grad) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    int l = 0, f = t;
-    for (; l < kLayers; ++l) {
-        if (f < layer_out(l)) break;
-        f -= layer_out(l);
-    }
-    if (l >= kLayers) return;
-    if (l == kLayers - 1 && f >= kOut) return;
-    float s = 0.0f;
-    for (int b = 0; b < batch; ++b) s += (float)dy[((size_t)b * 4 + l) * 64 + f];
-    grad[b_offset(l) + f] = s;
+    for (int j = 0; j < 4; ++j) atomicAdd(&grad[w_offset(l) + (16 * m + 4 * g + j) * layer_out(l) + out], acc[j]);
 }
 
 __global__ void k_adam(float* params, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
@@ -291,6 +283,33 @@ __global__ void k_zero(float* p, int n) {
 }
 __global__ void k_zero_u64(unsigned long long* p) {
     if (threadIdx.x == 0) *p = 0ull;
+}
+
+// The rendering-side forward (mlp_forward_tiles, 4 tiles per wavefront) on
+// explicit samples, for tests.
+__global__ __launch_bounds__(64) void k_infer(GuideDev G, const Sample* samples, int n, float* out) {
+    const int lane = threadIdx.x, g = lane >> 4, s = lane & 15;
+    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        bf16x8 x0[4], x1[4];
+        for (int t = 0; t < 4; ++t) {
+            const int b = base + 16 * t + s;
+            Sample smp;
+            if (b < n) smp = samples[b];
+            else __builtin_memset(&smp, 0, sizeof(smp));
+            encode(G, smp, x0[t], x1[t]);
+        }
+        f32x4 o[4][3];
+        mlp_forward_tiles<4>(G, x0, x1, o);
+        for (int t = 0; t < 4; ++t) {
+            const int b = base + 16 * t + s;
+            if (b >= n) continue;
+            for (int m = 0; m < 3; ++m)
+                for (int j = 0; j < 4; ++j) {
+                    const int f = 16 * m + 4 * g + j;
+                    if (f < kOut) out[(size_t)b * kOut + f] = o[t][m][j];
+                }
+        }
+    }
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -380,13 +399,10 @@ static void grad_step(State* s, const GuideDev& g, int batch, hipStream_t st) {
     k_loss<<<nblk(batch, 256), 256, 0, st>>>(s->ring, s->sel, batch, s->out, s->scal, s->dout, s->scal + 1);
     k_bwd<<<tiles < 4096 ? tiles : 4096, 64, 0, st>>>(g, batch, s->act, s->dout, s->dy);
     int wtiles = 0;
-    for (int l = 0; l < kLayers; ++l) wtiles += (layer_in(l) / 16) * layer_mtiles(l);
+    for (int l = 0; l < kLayers; ++l) wtiles += (layer_in(l) / 16 + 1) * layer_mtiles(l);
     const int nchunks = (batch + 31) / 32;
     const int cpw = 8, splits = (nchunks + cpw - 1) / cpw;
     k_wgrad<<<dim3(wtiles, splits), 64, 0, st>>>(batch, cpw, s->act, s->dy, s->grad);
-    int nb = 0;
-    for (int l = 0; l < kLayers; ++l) nb += layer_out(l);
-    k_bgrad<<<nblk(nb, 256), 256, 0, st>>>(batch, s->dy, s->grad);
 }
 
 void train(State* s, const GuideDev& g, int steps, int batch, float lr, hipStream_t st) {
@@ -424,6 +440,19 @@ int debug_step(State* s, const float* params, const Sample* samples, int n, cons
         for (int f = 0; f < kOut; ++f) out[(size_t)i * kOut + f] = o[(size_t)i * kOutPad + f];
     *loss = sc[1];
     // the device ring now holds the debug samples: forget them
+    k_zero_u64<<<1, 64, 0, st>>>(s->total);
+    return hipStreamSynchronize(st) == hipSuccess ? 0 : 2;
+}
+
+int debug_infer(State* s, const float* params, const Sample* samples, int n, const float bmin[3], const float bmax[3],
+                float* out, hipStream_t st) {
+    if (n <= 0 || n > s->capacity || n > s->batch_cap) return 1;
+    if (hipMemcpyAsync(s->params, params, kParams * 4, hipMemcpyHostToDevice, st) != hipSuccess) return 2;
+    k_pack<<<nblk(kPackedFwd > kPackedBwd ? kPackedFwd : kPackedBwd, 256), 256, 0, st>>>(s->params, s->fwd, s->bwd);
+    if (hipMemcpyAsync(s->ring, samples, (size_t)n * sizeof(Sample), hipMemcpyHostToDevice, st) != hipSuccess) return 2;
+    GuideDev g = dev(s, bmin, bmax, 0.5f);
+    k_infer<<<nblk(n, 64), 64, 0, st>>>(g, s->ring, n, s->out);
+    if (hipMemcpyAsync(out, s->out, (size_t)n * kOut * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
     k_zero_u64<<<1, 64, 0, st>>>(s->total);
     return hipStreamSynchronize(st) == hipSuccess ? 0 : 2;
 }

@@ -947,47 +947,62 @@ __device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float
     }
 }
 
-// Guiding network on the live queue: 64 paths per wavefront, 16 per MFMA
-// tile.  Paths without a surface hit run on zero input and are not written.
+// Guiding network on the live queue: 64 paths per wavefront (one workgroup).
+// Each lane encodes one path's 64 input features into LDS; the MFMA operands
+// are then gathered from LDS as 4 tiles of 16 that share every weight load.
+// Paths without a surface hit run on zero input and are not written.
 __global__ __launch_bounds__(64) void k_guide_infer(SceneDev S, WaveDev W, guide::GuideDev G, int qin) {
+    __shared__ __bf16 feat[64][guide::kIn + 2];
+    __shared__ int32_t path[64];
     const int n = W.cnt[qin];
     const int lane = threadIdx.x, g = lane >> 4, s = lane & 15;
     for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
-        for (int t = 0; t < 4; ++t) {
-            const int qi = base + 16 * t + s;
-            const bool valid = qi < n;
-            const int p = valid ? W.q[qin][qi] : 0;
-            bool hit = false;
+        {
+            const int qi = base + lane;
+            int p = -1;
             float pn[3] = {0.0f, 0.0f, 0.0f};
             v3f nrm = mk(0.0f, 0.0f, 1.0f), wo = mk(0.0f, 0.0f, 1.0f);
-            if (valid) {
-                const float4 hv = W.hit[p];
+            if (qi < n) {
+                const int pp = W.q[qin][qi];
+                const float4 hv = W.hit[pp];
                 if (__float_as_int(hv.y) >= 0) {
-                    const v3f o = mk(W.ray_o[p]), d = mk(W.ray_d[p]);
+                    const v3f o = mk(W.ray_o[pp]), d = mk(W.ray_d[pp]);
                     v3f point;
                     hit_frame(S, o, d, hv, point, nrm);
                     wo = -d;
                     pn[0] = (point.x - G.bmin[0]) * G.binv[0];
                     pn[1] = (point.y - G.bmin[1]) * G.binv[1];
                     pn[2] = (point.z - G.bmin[2]) * G.binv[2];
-                    hit = true;
+                    p = pp;
                 }
             }
-            guide::bf16x8 x0, x1;
+            float f[guide::kIn];
+            guide::features(pn, nrm, wo, f);
+            for (int i = 0; i < guide::kIn; ++i) feat[lane][i] = (__bf16)f[i];
+            path[lane] = p;
+        }
+        __syncthreads();
+        guide::bf16x8 x0[4], x1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
             for (int i = 0; i < 8; ++i) {
-                x0[i] = (__bf16)guide::feature(guide::perm(0, g, i), pn, nrm, wo);
-                x1[i] = (__bf16)guide::feature(guide::perm(1, g, i), pn, nrm, wo);
+                x0[t][i] = feat[16 * t + s][guide::perm(0, g, i)];
+                x1[t][i] = feat[16 * t + s][guide::perm(1, g, i)];
             }
-            guide::f32x4 o[3];
-            guide::mlp_forward(G, x0, x1, o);
-            if (hit) {
-                float* out = W.gout + (size_t)p * guide::kOut;
-                for (int m = 0; m < 3; ++m)
-                    for (int j = 0; j < 4; ++j) {
-                        const int f = 16 * m + 4 * g + j;
-                        if (f < guide::kOut) out[f] = o[m][j];
-                    }
-            }
+        int pt[4];
+        for (int t = 0; t < 4; ++t) pt[t] = path[16 * t + s];
+        __syncthreads();
+        guide::f32x4 o[4][3];
+        guide::mlp_forward_tiles<4>(G, x0, x1, o);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (pt[t] < 0) continue;
+            float* out = W.gout + (size_t)pt[t] * guide::kOut;
+            for (int m = 0; m < 3; ++m)
+                for (int j = 0; j < 4; ++j) {
+                    const int f = 16 * m + 4 * g + j;
+                    if (f < guide::kOut) out[f] = o[t][m][j];
+                }
         }
     }
 }
@@ -1462,9 +1477,9 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     // Diagnostic renders (render_tile with diagnostics enabled) also keep
     // per-path vertices and records: smaller batches, at most ~2 GB of them.
     const bool diag = mode == 0 && ctx->diag_on;
-    // Guided renders keep 40 network outputs per path: batches of <= 2^23.
+    // Guided renders keep 40 network outputs per path: batches of <= 2^25.
     const bool guided = !diag && pr->algorithm == LUCID_ALGO_GUIDED_MIS;
-    if (guided) target = std::min<int64_t>(target, (int64_t)1 << 23);
+    if (guided) target = std::min<int64_t>(target, (int64_t)1 << 25);
     const int32_t slots = std::max(1, pr->max_depth);
     if (diag) {
         const int64_t per_path = (int64_t)diag::kMaxDepth * 32 + (int64_t)slots * 32 + 4;
@@ -2186,6 +2201,22 @@ int lucid_guiding_set_params(lucid_ctx* ctx, const float* params, int32_t n) {
     if (rc) return rc;
     if (guide::set_params(ctx->gstate, params, ctx->stream)) return fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed");
     ctx->gsteps = std::max<int64_t>(ctx->gsteps, 1);  // explicit weights count as trained
+    return LUCID_OK;
+}
+
+int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
+                              const float bmin[3], const float bmax[3], float* out) {
+    if (!ctx || !params || !samples || !bmin || !bmax || !out || n < 1 || n > kGuideBatchCap) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = ensure_guide(ctx);
+    if (rc) return rc;
+    rc = guide::debug_infer(ctx->gstate, params, reinterpret_cast<const guide::Sample*>(samples), n, bmin, bmax, out,
+                            ctx->stream);
+    if (rc) return fail(ctx, rc == 1 ? LUCID_ERR_INVALID : LUCID_ERR_DEVICE, "guiding debug inference failed");
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    ctx->gsteps = 0;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
 }
 

@@ -296,6 +296,10 @@ int lucid_guiding_set_params(lucid_ctx* ctx, const float* params, int32_t n);
    the gradient check against the CPU reference. */
 int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
                              const float bmin[3], const float bmax[3], float* out, float* grad, float* loss);
+/* The rendering-side inference kernel's forward pass (4 MFMA tiles per
+   wavefront) on the same sample format: raw outputs (n x 40).  For tests. */
+int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
+                              const float bmin[3], const float bmax[3], float* out);
 
 /* Runs the device BVH traversal code on the CPU over a scene packed from
    `meshes` (no device needed), checking every node and triangle index before

@@ -8,8 +8,8 @@ learned guiding); this module restates them independently of the device
 code:
 
 * input: 64 features of (position in the scene box, shading normal,
-  outgoing direction) -- frequency encoding sin/cos(2^k pi p), k = 0..5, then
-  2p - 1, n, wo, sin(pi n), sin(pi wo), n . wo, zero padding;
+  outgoing direction) -- frequency encoding sin/cos(2^k pi p), k = 0..7, then
+  2p - 1, n, wo, sin(pi n), sin(pi wo), n . wo;
 * 3 hidden layers of 64 with ReLU, 40 outputs = 8 lobes x (mean direction,
   log concentration clamped to [-2, 7], weight logit), softmax weights;
 * density p(w) = sum_k w_k kappa_k / (2 pi (1 - e^{-2 kappa_k}))
@@ -73,17 +73,17 @@ def features(p, n, wo, bmin, bmax):
     pn = (p - np.asarray(bmin, np.float64)) / (np.asarray(bmax, np.float64) - np.asarray(bmin, np.float64))
     S = p.shape[0]
     f = np.zeros((S, 64))
-    for idx in range(36):
+    for idx in range(48):
         c, q = idx % 3, idx // 3
         k = q >> 1
         a = pn[:, c] * (np.pi * (1 << k))
         f[:, idx] = np.cos(a) if q & 1 else np.sin(a)
-    f[:, 36:39] = 2 * pn - 1
-    f[:, 39:42] = n
-    f[:, 42:45] = wo
-    f[:, 45:48] = np.sin(np.pi * n)
-    f[:, 48:51] = np.sin(np.pi * wo)
-    f[:, 51] = np.sum(n * wo, axis=1)
+    f[:, 48:51] = 2 * pn - 1
+    f[:, 51:54] = n
+    f[:, 54:57] = wo
+    f[:, 57:60] = np.sin(np.pi * n)
+    f[:, 60:63] = np.sin(np.pi * wo)
+    f[:, 63] = np.sum(n * wo, axis=1)
     return f
 
 

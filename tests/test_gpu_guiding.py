@@ -52,6 +52,24 @@ def test_training_step_matches_reference():
             assert e < 3e-2, (a, b, e)
 
 
+def test_render_inference_matches_reference():
+    rng = np.random.default_rng(21)
+    params = G.init_params(seed=22, scale_out=1.0).astype(np.float32)
+    S = 333  # not a multiple of 64: partial tiles
+    box = (np.float32([-4, 0, -6]), np.float32([4, 3, 2]))
+    dirs = rng.normal(size=(3, S, 3))
+    dirs /= np.linalg.norm(dirs, axis=2, keepdims=True)
+    smp = np.zeros((S, 16), np.float32)
+    smp[:, 0:3] = rng.uniform(box[0], box[1], (S, 3))
+    smp[:, 3:6], smp[:, 6:9], smp[:, 9:12] = dirs[0], dirs[1], dirs[2]
+    ctx = lucid_capi.Context(0)
+    out = np.zeros((S, 40), np.float32)
+    ctx.check(ctx.L.lucid_guiding_debug_infer(ctx.h, F(params), F(smp), S, F(box[0]), F(box[1]), F(out)))
+    x = G.features(smp[:, 0:3], smp[:, 3:6], smp[:, 6:9], box[0], box[1])
+    _, o_ref = G.forward(params.astype(np.float64), x, bf=True)
+    assert np.max(np.abs(out - o_ref[:, :40])) < 2e-2 * np.max(np.abs(o_ref))
+
+
 def _box_renderer(lucid, algo):
     r = make_renderer(lucid, workloads.closed_box(), workloads.CLOSED_BOX_CAMERA, algo)
     return r
