@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--record-stride", type=int, default=2)
+    ap.add_argument("--guide-depth", type=int, default=0, help="guided bounces: depth < N (0: all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -238,11 +239,21 @@ def main():
             return curve, n
 
         plain_curve, plain_spp = progressive("mis", 0)
-        gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7)
+        gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7,
+                                      args.guide_depth)
         ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
         guided_curve, guided_spp = progressive("guided", 0)
         gst = lucid_capi.GuidingStats()
         ctx.check(L.lucid_guiding_get_stats(ctx.h, C.byref(gst)))
+        # one instrumented guided pass: MFMA throughput of the guiding network
+        # (algorithmic FLOPs: 2 (64*64*3 + 64*40) per inference, one inference
+        # per closest-hit query; training: forward + backward over 3x that per sample)
+        ctx.check(L.lucid_set_profiling(ctx.h, lucid_capi.PROFILE_KERNELS))
+        gp = render("guided", spp, 3000000)
+        ctx.check(L.lucid_set_profiling(ctx.h, 0))
+        flop_inf = 2 * (64 * 64 * 3 + 64 * 40)
+        inf_tflops = gp["rays_closest"] * flop_inf / max(gp["ms_guide_infer"], 1e-9) / 1e9
+        trn_tflops = args.train_steps * args.batch * 3 * flop_inf / max(gp["ms_guide_train"], 1e-9) / 1e9
         tp, mp = time_to_target(plain_curve)
         tg, mg = time_to_target(guided_curve)
         # equal-time comparison at the end of the shorter run
@@ -256,8 +267,17 @@ def main():
             "guided_relmse_end": round(guided_curve[-1][1], 5), "guided_spp": guided_spp,
             "equal_time_relmse_ratio": round(at(plain_curve) / max(at(guided_curve), 1e-12), 3),
             "reference": {"spp": rspp, "self_relmse": round(ref_err, 5), "kind": "oracle FP64, stratified subset"},
+            "guided_pass_ms": {"render": round(gp["ms_render"], 1), "intersect": round(gp["ms_intersect"], 1),
+                               "shade": round(gp["ms_shade"], 1), "shadow": round(gp["ms_shadow"], 1),
+                               "guide_infer": round(gp["ms_guide_infer"], 1),
+                               "guide_train": round(gp["ms_guide_train"], 1)},
+            "mfma": {"bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",
+                     "infer_achieved": round(inf_tflops, 1), "infer_frac": round(inf_tflops / 2500.0, 4),
+                     "train_achieved": round(trn_tflops, 1), "train_frac": round(trn_tflops / 2500.0, 4),
+                     "flop_per_inference": flop_inf},
             "guiding": {"alpha": args.alpha, "train_steps_per_pass": args.train_steps, "batch": args.batch,
-                        "lr": args.lr, "record_stride": args.record_stride, "samples": int(gst.samples_total),
+                        "lr": args.lr, "record_stride": args.record_stride, "guided_depth": args.guide_depth,
+                        "samples": int(gst.samples_total),
                         "adam_steps": int(gst.train_steps), "last_loss": round(float(gst.last_loss), 4)},
         }
     if not args.no_cpu:

@@ -1264,7 +1264,7 @@ struct lucid_ctx {
     int32_t profiling = 0;
     // learned path guiding
     guide::State* gstate = nullptr;
-    lucid_guiding_config gcfg{0.5f, 8, 16384, 5e-3f, 8, 1u};
+    lucid_guiding_config gcfg{0.7f, 64, 32768, 3e-3f, 2, 1u, 0};
     int64_t gsteps = 0;
     float* gout = nullptr;
     int64_t gout_cap = 0;
@@ -1611,7 +1611,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
-            if (guided && G.alpha > 0.0f) {
+            const bool guide_here = guided && G.alpha > 0.0f && (ctx->gcfg.max_depth <= 0 || depth < ctx->gcfg.max_depth);
+            if (guide_here) {
                 ks = kev_begin(ctx, 3, st);
                 const unsigned gg = (unsigned)std::min<int64_t>(blocks(n, 64), (int64_t)ctx->num_cu * 32);
                 k_guide_infer<<<gg, 64, 0, st>>>(SD, W, G, q);
@@ -1631,9 +1632,12 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
                 case LUCID_ALGO_VOLUME_MIS:
                     k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
-                case LUCID_ALGO_GUIDED_MIS:
-                    k_shade<LUCID_ALGO_GUIDED_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                case LUCID_ALGO_GUIDED_MIS: {
+                    guide::GuideDev Gd = G;
+                    if (!guide_here) Gd.alpha = 0.0f;  // this bounce samples the BSDF only
+                    k_shade<LUCID_ALGO_GUIDED_MIS><<<gs, kBlock, 0, st>>>(SD, W, Gd, q, max_depth);
                     break;
+                }
                 case kAlgoDiagSimple:
                     k_shade<kAlgoDiagSimple><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;

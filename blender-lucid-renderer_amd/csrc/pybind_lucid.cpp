@@ -475,10 +475,10 @@ public:
     }
 
     // ---- learned path guiding (extension) ----
-    void set_guiding(float alpha, int train_steps, int batch, float lr, int stride, uint32_t seed) {
+    void set_guiding(float alpha, int train_steps, int batch, float lr, int stride, uint32_t seed, int max_depth) {
         std::lock_guard<std::mutex> l(render_mu_);
         ensure_ctx();
-        lucid_guiding_config c{alpha, train_steps, batch, lr, stride, seed};
+        lucid_guiding_config c{alpha, train_steps, batch, lr, stride, seed, max_depth};
         check(lucid_set_guiding(ctx_, &c));
     }
     void reset_guiding() {
@@ -838,9 +838,9 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
-        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.5f, py::arg("train_steps") = 8,
-             py::arg("batch") = 16384, py::arg("learning_rate") = 5e-3f, py::arg("record_stride") = 8,
-             py::arg("seed") = 1u,
+        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 64,
+             py::arg("batch") = 32768, py::arg("learning_rate") = 3e-3f, py::arg("record_stride") = 2,
+             py::arg("seed") = 1u, py::arg("max_depth") = 0,
              "Extension: configure learned path guiding (algorithm 'guided') and start from a fresh network")
         .def("reset_guiding", &Renderer::reset_guiding, "Extension: fresh guiding network")
         .def("guiding_stats", &Renderer::guiding_stats, "Extension: training state of the guiding network")

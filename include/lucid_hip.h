@@ -270,6 +270,7 @@ typedef struct lucid_guiding_config {
     float learning_rate;
     int32_t record_stride; /* 1 path in N records training samples, >= 1 */
     uint32_t seed;         /* network initialisation */
+    int32_t max_depth;     /* bounces guided: depth < max_depth (0: all) */
 } lucid_guiding_config;
 
 typedef struct lucid_guiding_stats {
