@@ -25,9 +25,11 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lucid_hip.h"
@@ -1101,6 +1103,22 @@ __global__ __launch_bounds__(256) void k_output(const float4* film, float* out, 
     size_t k = ((size_t)row * tw + px) * 4;
     *(float4*)(out + k) = make_float4(r, g, b, 1.0f);
 }
+// Session read-out: (sum * inv_samples) * 2^EV, clamp at 0, alpha 1, rows
+// bottom first (the async accumulator snapshot of src/gpu/path_tracer.cpp:
+// 1064-1075 followed by render_poll_async's sensitivity and clamp,
+// pybind_renderer.hpp:766-783).  inv_samples = 1 reads the plain sum.
+__global__ __launch_bounds__(256) void k_session_out(const float4* acc, float* out, int32_t w, int32_t h,
+                                                     float inv_samples, float sens) {
+    int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= w * h) return;
+    int px = pix % w, py = pix / w;
+    float4 a = acc[pix];
+    float r = smax((a.x * inv_samples) * sens, 0.0f);
+    float g = smax((a.y * inv_samples) * sens, 0.0f);
+    float b = smax((a.z * inv_samples) * sens, 0.0f);
+    size_t k = ((size_t)(h - 1 - py) * w + px) * 4;
+    *(float4*)(out + k) = make_float4(r, g, b, 1.0f);
+}
 // primary-hit export: mesh, original triangle, u, v, t
 __global__ __launch_bounds__(256) void k_hits_out(SceneDev S, WaveDev W, int32_t* mesh, int32_t* tri, float* u,
                                                   float* v, float* t, int32_t P, int32_t B, int32_t S_total,
@@ -1238,12 +1256,27 @@ struct lucid_scene {
     float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // scene box
 };
 
+// A stream with its own wavefront buffers.  Renders on a context use lane 0,
+// whose stream is the context's; a batched session pass spreads its sessions
+// over all lanes, so that the short tail of one session's launches overlaps
+// the work of another's.  Four lanes = HIP's hardware queues per process.
+constexpr int kLanes = 4;
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t wave_ev[2] = {nullptr, nullptr};  // pacing: at most two waves in flight
+    int64_t cap_paths = 0;
+    std::vector<void*> allocs;
+    void* ctr = nullptr;  // queue / ray counters, kept across reallocations
+    WaveDev W{};
+};
+
 struct lucid_ctx {
     int device = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t wave_ev[2] = {nullptr, nullptr};  // pacing: at most two waves in flight
+    Lane lanes[kLanes];
+    int32_t lanes_ready = 1;  // lanes with a stream (lane 0 always)
     std::string err;
     std::atomic<int> cancel{0};
     std::mutex mu;
@@ -1252,10 +1285,6 @@ struct lucid_ctx {
     // or 0 for one lane per queued ray.  Traversal latency is best hidden by
     // the hardware dispatcher (0); shading runs best persistent.
     int32_t traverse_groups = 0, shade_groups = 16;
-    // wave buffers
-    int64_t cap_paths = 0;
-    std::vector<void*> wave_allocs;
-    WaveDev W{};
     float4* film = nullptr;
     int64_t film_cap = 0;
     float* out_dev = nullptr;
@@ -1289,6 +1318,34 @@ struct lucid_ctx {
     size_t kev_used = 0;
 };
 
+// A progressive accumulation: one frame of one scene and camera, its
+// radiance sum in device memory, the number of samples in it, and, while an
+// async worker runs, the worker thread and the latest averaged snapshot.
+struct lucid_session {
+    lucid_ctx* ctx = nullptr;
+    lucid_scene* scene = nullptr;
+    lucid_camera cam{};
+    int32_t w = 0, h = 0, max_depth = 8, algorithm = LUCID_ALGO_NEE;
+    float4* accum = nullptr;  // radiance sum per pixel, rows top first
+    int64_t samples = 0;      // samples in `accum`
+    // async accumulation (render_start_async & co.)
+    std::thread worker;
+    std::atomic<bool> running{false};
+    std::atomic<int> stop{0};
+    std::atomic<bool> reset_pending{false};
+    std::mutex pmu;  // camera / depth of a pending reset
+    lucid_camera pend_cam{};
+    int32_t pend_depth = 8;
+    std::mutex smu;  // the snapshot
+    std::vector<float> snap;
+    int64_t snap_samples = 0;
+    std::atomic<uint32_t> revision{0};
+    float* snap_dev = nullptr;
+    float* snap_pinned = nullptr;
+    std::atomic<bool> failed{false};
+    std::string err;  // why the worker ended early: written before `failed`
+};
+
 namespace {
 
 int fail(lucid_ctx* c, int code, const std::string& msg) {
@@ -1314,34 +1371,67 @@ int upload(lucid_ctx* ctx, lucid_scene* s, const std::vector<T>& v, const T** ou
     return LUCID_OK;
 }
 
-void free_wave(lucid_ctx* ctx) {
-    for (void* p : ctx->wave_allocs) (void)hipFree(p);
-    ctx->wave_allocs.clear();
-    ctx->cap_paths = 0;
-    ctx->W = WaveDev{};
+void free_wave(Lane& L) {
+    for (void* p : L.allocs) (void)hipFree(p);
+    L.allocs.clear();
+    L.cap_paths = 0;
+    L.W = WaveDev{};
+    if (L.ctr) {
+        L.W.cnt = static_cast<int32_t*>(L.ctr);
+        L.W.rays = reinterpret_cast<unsigned long long*>(static_cast<char*>(L.ctr) + 64);
+    }
 }
 
-int ensure_wave(lucid_ctx* ctx, int64_t n) {
-    if (n <= ctx->cap_paths) return LUCID_OK;
-    free_wave(ctx);
+int ensure_counters(lucid_ctx* ctx, Lane& L) {
+    if (L.ctr) return LUCID_OK;
+    if (hipMalloc(&L.ctr, 128) != hipSuccess) {
+        L.ctr = nullptr;
+        return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (counters)");
+    }
+    HIP_OK(hipMemset(L.ctr, 0, 128));
+    L.W.cnt = static_cast<int32_t*>(L.ctr);
+    L.W.rays = reinterpret_cast<unsigned long long*>(static_cast<char*>(L.ctr) + 64);
+    return LUCID_OK;
+}
+
+int ensure_wave(lucid_ctx* ctx, Lane& L, int64_t n) {
+    int rc = ensure_counters(ctx, L);
+    if (rc) return rc;
+    if (n <= L.cap_paths) return LUCID_OK;
+    // the previous buffers may still be in use by queued work of this lane
+    HIP_OK(hipStreamSynchronize(L.stream));
+    free_wave(L);
     auto alloc = [&](size_t bytes, void** out) -> bool {
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return false;
-        ctx->wave_allocs.push_back(p);
+        L.allocs.push_back(p);
         *out = p;
         return true;
     };
-    WaveDev& W = ctx->W;
+    WaveDev& W = L.W;
     bool ok = alloc(n * 16, (void**)&W.ray_o) && alloc(n * 16, (void**)&W.ray_d) && alloc(n * 16, (void**)&W.thr) &&
               alloc(n * 16, (void**)&W.rad) && alloc(n * 16, (void**)&W.rng) && alloc(n * 16, (void**)&W.meta) &&
               alloc(n * 16, (void**)&W.hit) && alloc(n * 16, (void**)&W.sh_o) && alloc(n * 16, (void**)&W.sh_d) &&
               alloc(n * 16, (void**)&W.sh_c) && alloc(n * 4, (void**)&W.q[0]) && alloc(n * 4, (void**)&W.q[1]) &&
-              alloc(n * 4, (void**)&W.qs) && alloc(64, (void**)&W.cnt) && alloc(64, (void**)&W.rays);
+              alloc(n * 4, (void**)&W.qs);
     if (!ok) {
-        free_wave(ctx);
+        free_wave(L);
         return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (wave buffers)");
     }
-    ctx->cap_paths = n;
+    L.cap_paths = n;
+    return LUCID_OK;
+}
+
+// Creates the streams and pacing events of lanes 1..kLanes-1 (default
+// priority, non-blocking).
+int ensure_lanes(lucid_ctx* ctx) {
+    for (int i = ctx->lanes_ready; i < kLanes; ++i) {
+        Lane& L = ctx->lanes[i];
+        HIP_OK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&L.wave_ev[0], hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&L.wave_ev[1], hipEventDisableTiming));
+        ctx->lanes_ready = i + 1;
+    }
     return LUCID_OK;
 }
 
@@ -1448,17 +1538,22 @@ int validate(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const 
     return LUCID_OK;
 }
 
-// Runs the waves.  mode 0: film (render_tile), 1: per-sample radiance into
-// samples_dev, 2: primary hits only.
+// Runs the waves on lane L.  mode 0: film (render_tile), 1: per-sample
+// radiance into samples_dev, 2: primary hits only, 3: session pass (the
+// samples are added to `accum`, which is not cleared, with jitter on every
+// sample as in a render_tile call of more than one sample; no diagnostics).
 struct RunOut {
+    int32_t samples_done = 0;  // samples of the call whose waves were issued
     float* samples_dev = nullptr;
     int32_t *mesh = nullptr, *tri = nullptr;
     float *u = nullptr, *v = nullptr, *t = nullptr;
 };
 
-int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr, int mode,
-              hipStream_t st, RunOut* ro, bool* cancelled) {
+int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+              int mode, hipStream_t st, RunOut* ro, bool* cancelled, float4* accum = nullptr, int active_lanes = 1,
+              const std::atomic<int>* cancel_flag = nullptr) {
     *cancelled = false;
+    if (!cancel_flag) cancel_flag = &ctx->cancel;
     const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
     const int S = pr->samples;
     // Wavefront batch: as many paths as fit a quarter of free device memory,
@@ -1470,7 +1565,7 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         size_t free_b = 0, total_b = 0;
         target = (int64_t)1 << 26;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
-            const int64_t fit = (int64_t)(free_b / 4 / kWaveBytesPerPath) + ctx->cap_paths;
+            const int64_t fit = (int64_t)(free_b / 4 / kWaveBytesPerPath) / active_lanes + L.cap_paths;
             target = std::max<int64_t>(std::min<int64_t>(target, fit), (int64_t)1 << 16);
         }
     }
@@ -1486,12 +1581,16 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
         target = std::min<int64_t>(target, std::max<int64_t>((int64_t)1 << 16, ((int64_t)2 << 30) / per_path));
     }
     int64_t B = std::max<int64_t>(1, std::min<int64_t>(S, target / P));
-    int rc = ensure_wave(ctx, P * B);
+    int rc = ensure_wave(ctx, L, P * B);
     if (rc) return rc;
-    rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
-    if (rc) return rc;
+    float4* film = accum;
+    if (mode != 3) {
+        rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
+        if (rc) return rc;
+        film = ctx->film;
+    }
     SceneDev SD = scene->dev;
-    WaveDev W = ctx->W;
+    WaveDev W = L.W;
     const int32_t dsub = ctx->diag_sub;
     const int32_t dsw = pr->tile_w / dsub, dsh = pr->tile_h / dsub;
     if (diag) {
@@ -1562,9 +1661,11 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     const int64_t cap_trav = ctx->traverse_groups > 0 ? (int64_t)ctx->num_cu * ctx->traverse_groups : INT64_MAX;
     const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
-    ctx->kev_used = 0;
-    k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
-    k_film_clear<<<blocks(P, 256), 256, 0, st>>>(ctx->film, (int32_t)P);
+    if (mode != 3) {  // a session batch zeroes its lanes' counters once
+        ctx->kev_used = 0;
+        k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
+        k_film_clear<<<blocks(P, 256), 256, 0, st>>>(film, (int32_t)P);
+    }
     CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
     CamDev cd;
     std::memcpy(cd.pos, cb.pos, 12); std::memcpy(cd.fwd, cb.fwd, 12);
@@ -1572,7 +1673,7 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     cd.aspect = cb.aspect; cd.scale = cb.scale;
     cd.fw = pr->full_w; cd.fh = pr->full_h; cd.tx = pr->tile_x; cd.ty = pr->tile_y;
     cd.tw = pr->tile_w; cd.th = pr->tile_h;
-    cd.samples_in_call = S;
+    cd.samples_in_call = mode == 3 ? std::max(S, 2) : S;
     cd.P = (int32_t)P;
     cd.max_depth = pr->max_depth;
     uint64_t waves = 0;
@@ -1593,8 +1694,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     for (int s0 = 0; s0 < S; s0 += (int)B) {
         // Pacing: wait for the wave before the previous one, so at most two
         // waves are queued and a cancel takes effect within one wave.
-        if (waves >= 2) (void)hipEventSynchronize(ctx->wave_ev[waves & 1]);
-        if (ctx->cancel.load(std::memory_order_relaxed)) {
+        if (waves >= 2) (void)hipEventSynchronize(L.wave_ev[waves & 1]);
+        if (cancel_flag->load(std::memory_order_relaxed)) {
             *cancelled = true;
             break;
         }
@@ -1661,8 +1762,8 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             k_guide_collect<<<blocks((n + W.tstride - 1) / W.tstride, 256), 256, 0, st>>>(
                 W, (int32_t)n, guide::ring(ctx->gstate), guide::ring_total(ctx->gstate),
                 guide::ring_capacity(ctx->gstate));
-        if (mode == 0) {
-            k_film<<<blocks(P, 256), 256, 0, st>>>(W, ctx->film, (int32_t)P, b);
+        if (mode == 0 || mode == 3) {
+            k_film<<<blocks(P, 256), 256, 0, st>>>(W, film, (int32_t)P, b);
             if (diag && dsw > 0 && dsh > 0)
                 k_diag_film<<<blocks((int64_t)dsw * dsh, 64), 64, 0, st>>>(
                     W, ctx->dfilm, reinterpret_cast<diag::StrategyWelford*>(ctx->dstrat), ctx->dtotal, pr->tile_w,
@@ -1673,8 +1774,9 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
             k_hits_out<<<blocks(n, 256), 256, 0, st>>>(SD, W, ro->mesh, ro->tri, ro->u, ro->v, ro->t, (int32_t)P, b,
                                                        S, s0);
         }
-        (void)hipEventRecord(ctx->wave_ev[waves & 1], st);
+        (void)hipEventRecord(L.wave_ev[waves & 1], st);
         ++waves;
+        if (ro) ro->samples_done = s0 + b;
     }
     // online training after the render (its time is part of the render's)
     if (train && !*cancelled) {
@@ -1690,17 +1792,31 @@ int run_waves(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ctx, LUCID_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-    ctx->stats.paths = (uint64_t)P * S;
-    ctx->stats.waves = waves;
+    if (mode == 3) {
+        ctx->stats.paths += (uint64_t)P * S;
+        ctx->stats.waves += waves;
+    } else {
+        ctx->stats.paths = (uint64_t)P * S;
+        ctx->stats.waves = waves;
+    }
     return LUCID_OK;
 }
 
-int finish_stats(lucid_ctx* ctx) {
+// Reads the ray / work counters of lanes [0, nlanes) into ctx->stats.
+int finish_stats(lucid_ctx* ctx, int nlanes = 1) {
     unsigned long long rays[6] = {0, 0, 0, 0, 0, 0};
     int32_t cnt[4] = {0, 0, 0, 0};
-    HIP_OK(hipMemcpyAsync(rays, ctx->W.rays, sizeof(rays), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipMemcpyAsync(cnt, ctx->W.cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nlanes; ++i) {
+        Lane& L = ctx->lanes[i];
+        if (!L.W.rays) continue;
+        unsigned long long r[6];
+        int32_t c[4];
+        HIP_OK(hipMemcpyAsync(r, L.W.rays, sizeof(r), hipMemcpyDeviceToHost, L.stream));
+        HIP_OK(hipMemcpyAsync(c, L.W.cnt, sizeof(c), hipMemcpyDeviceToHost, L.stream));
+        HIP_OK(hipStreamSynchronize(L.stream));
+        for (int k = 0; k < 6; ++k) rays[k] += r[k];
+        cnt[3] += c[3];
+    }
     ctx->stats.rays_closest = rays[0];
     ctx->stats.rays_shadow = rays[1];
     ctx->stats.stack_overflows = (uint64_t)cnt[3];
@@ -1745,11 +1861,12 @@ int lucid_create(int32_t device, lucid_ctx** out) {
         ctx->num_cu = ncu;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->wave_ev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->wave_ev[1], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->lanes[0].wave_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->lanes[0].wave_ev[1], hipEventDisableTiming) != hipSuccess) {
         delete ctx;
         return LUCID_ERR_DEVICE;
     }
+    ctx->lanes[0].stream = ctx->stream;
     *out = ctx;
     return LUCID_OK;
 }
@@ -1757,8 +1874,11 @@ int lucid_create(int32_t device, lucid_ctx** out) {
 void lucid_destroy(lucid_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    free_wave(ctx);
+    for (int i = 0; i < ctx->lanes_ready; ++i) {
+        (void)hipStreamSynchronize(ctx->lanes[i].stream);
+        free_wave(ctx->lanes[i]);
+        if (ctx->lanes[i].ctr) (void)hipFree(ctx->lanes[i].ctr);
+    }
     guide::destroy(ctx->gstate);
     if (ctx->gout) (void)hipFree(ctx->gout);
     if (ctx->tslot) (void)hipFree(ctx->tslot);
@@ -1770,8 +1890,11 @@ void lucid_destroy(lucid_ctx* ctx) {
     if (ctx->out_dev) (void)hipFree(ctx->out_dev);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
-    (void)hipEventDestroy(ctx->wave_ev[0]);
-    (void)hipEventDestroy(ctx->wave_ev[1]);
+    for (int i = 0; i < ctx->lanes_ready; ++i) {
+        (void)hipEventDestroy(ctx->lanes[i].wave_ev[0]);
+        (void)hipEventDestroy(ctx->lanes[i].wave_ev[1]);
+        if (i > 0) (void)hipStreamDestroy(ctx->lanes[i].stream);
+    }
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1874,7 +1997,7 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     HIP_OK(hipEventRecord(ctx->ev0, st));
     bool cancelled = false;
-    rc = run_waves(ctx, scene, cam, pr, 0, st, nullptr, &cancelled);
+    rc = run_waves(ctx, ctx->lanes[0], scene, cam, pr, 0, st, nullptr, &cancelled);
     if (rc) return rc;
     CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
     int64_t P = (int64_t)pr->tile_w * pr->tile_h;
@@ -1927,7 +2050,7 @@ int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera*
     ro.samples_dev = dev;
     bool cancelled = false;
     HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
-    rc = run_waves(ctx, scene, cam, pr, 1, ctx->stream, &ro, &cancelled);
+    rc = run_waves(ctx, ctx->lanes[0], scene, cam, pr, 1, ctx->stream, &ro, &cancelled);
     if (!rc) {
         (void)hipEventRecord(ctx->ev1, ctx->stream);
         if (n) (void)hipMemcpyAsync(out_rgb, dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
@@ -1958,7 +2081,7 @@ int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
     ro.t = ro.v + n;
     bool cancelled = false;
     HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
-    rc = run_waves(ctx, scene, cam, pr, 2, ctx->stream, &ro, &cancelled);
+    rc = run_waves(ctx, ctx->lanes[0], scene, cam, pr, 2, ctx->stream, &ro, &cancelled);
     if (!rc) {
         (void)hipEventRecord(ctx->ev1, ctx->stream);
         if (n) {
@@ -2240,6 +2363,374 @@ int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* s
     ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
+}
+
+}  // extern "C"
+
+// ============================================================= sessions ==
+// Progressive accumulation (the reference's async viewport accumulator,
+// src/gpu/path_tracer.cpp:825-1296, and the sample_offset-chained
+// render_tile calls of its render sessions) and batched passes over many
+// sessions (one pass of every session, spread over the context's lanes).
+namespace {
+
+int session_clear(lucid_session* s) {
+    lucid_ctx* ctx = s->ctx;
+    const int64_t P = (int64_t)s->w * s->h;
+    k_film_clear<<<blocks(P, 256), 256, 0, ctx->stream>>>(s->accum, (int32_t)P);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    s->samples = 0;
+    return LUCID_OK;
+}
+
+// Adds up to `spp` samples to the session on lane L (caller holds ctx->mu);
+// the samples of every wave issued are counted, also when cancelled.
+int session_pass(lucid_session* s, Lane& L, int32_t spp, int active_lanes, const std::atomic<int>* cancel_flag,
+                 bool* cancelled) {
+    if (s->samples + spp > INT32_MAX) return fail(s->ctx, LUCID_ERR_INVALID, "session sample count overflows");
+    lucid_render_params pr{0, 0, s->w, s->h, s->w, s->h, spp, (int32_t)s->samples, s->max_depth, s->algorithm};
+    RunOut ro;
+    int rc = run_waves(s->ctx, L, s->scene, &s->cam, &pr, 3, L.stream, &ro, cancelled, s->accum, active_lanes,
+                       cancel_flag);
+    if (!rc) s->samples += ro.samples_done;
+    return rc;
+}
+
+// Averages the accumulator into the snapshot and bumps the revision.
+int take_snapshot(lucid_session* s) {
+    lucid_ctx* ctx = s->ctx;
+    const int64_t P = (int64_t)s->w * s->h;
+    int64_t n;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        n = s->samples;
+        if (n > 0) {
+            CameraBasis cb = camera_basis(s->cam, s->w, s->h);
+            k_session_out<<<blocks(P, 256), 256, 0, ctx->stream>>>(s->accum, s->snap_dev, s->w, s->h,
+                                                                  1.0f / (float)n, cb.sensitivity);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(s->snap_pinned, s->snap_dev, P * 4 * sizeof(float), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->smu);
+        if (n > 0) s->snap.assign(s->snap_pinned, s->snap_pinned + P * 4);
+        s->snap_samples = n;
+    }
+    s->revision.fetch_add(1, std::memory_order_release);
+    return LUCID_OK;
+}
+
+// The async worker: passes of growing sample count (1, 2, 4, ... while a
+// pass stays under ~20 ms, at most 256) with a snapshot after each, until
+// stopped.  A reset clears the accumulator at the top of the next pass.
+void session_worker(lucid_session* s) {
+    lucid_ctx* ctx = s->ctx;
+    int32_t spp = 1;
+    while (!s->stop.load(std::memory_order_acquire)) {
+        if (s->reset_pending.exchange(false, std::memory_order_acq_rel)) {
+            {
+                std::lock_guard<std::mutex> lock(s->pmu);
+                s->cam = s->pend_cam;
+                s->max_depth = s->pend_depth;
+            }
+            int rc;
+            {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                (void)hipSetDevice(ctx->device);
+                rc = session_clear(s);
+            }
+            if (rc) {
+                s->err = ctx->err;
+                break;
+            }
+            {
+                std::lock_guard<std::mutex> lock(s->smu);
+                s->snap_samples = 0;
+            }
+            spp = 1;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            (void)hipSetDevice(ctx->device);
+            bool cancelled = false;
+            rc = session_pass(s, ctx->lanes[0], spp, 1, &s->stop, &cancelled);
+            if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+                rc = fail(ctx, LUCID_ERR_DEVICE, "async pass: synchronize failed");
+        }
+        if (rc) {
+            s->err = ctx->err;
+            s->failed.store(true, std::memory_order_release);
+            break;
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms < 20.0 && spp < 256) spp *= 2;
+        else if (ms > 80.0 && spp > 1) spp /= 2;
+        // a snapshot of the old camera is not published after a reset request
+        if (!s->reset_pending.load(std::memory_order_acquire) && take_snapshot(s)) {
+            s->err = ctx->err;
+            s->failed.store(true, std::memory_order_release);
+            break;
+        }
+    }
+    if (!s->failed.load() && !s->reset_pending.load(std::memory_order_acquire)) (void)take_snapshot(s);
+}
+
+void session_stop(lucid_session* s) {
+    if (!s->worker.joinable()) {
+        s->running.store(false);
+        return;
+    }
+    s->stop.store(1, std::memory_order_release);
+    s->worker.join();
+    s->running.store(false);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lucid_session_create(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, int32_t width, int32_t height,
+                         int32_t max_depth, int32_t algorithm, lucid_session** out) {
+    if (!ctx || !out) return LUCID_ERR_INVALID;
+    *out = nullptr;
+    if (!scene || !cam) return fail(ctx, LUCID_ERR_INVALID, "null scene or camera");
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (1LL << 28))
+        return fail(ctx, LUCID_ERR_INVALID, "session size must be positive and at most 2^28 pixels");
+    if (max_depth < 0) return fail(ctx, LUCID_ERR_INVALID, "negative depth");
+    if (algorithm < 0 || algorithm > LUCID_ALGO_GUIDED_MIS) return fail(ctx, LUCID_ERR_INVALID, "unknown algorithm");
+    if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
+    lucid_session* s = new (std::nothrow) lucid_session();
+    if (!s) return fail(ctx, LUCID_ERR_NOMEM, "out of host memory");
+    s->ctx = ctx;
+    s->cam = *cam;
+    s->w = width;
+    s->h = height;
+    s->max_depth = max_depth;
+    s->algorithm = algorithm;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    int rc = LUCID_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess ||
+        hipMalloc((void**)&s->accum, (size_t)width * height * sizeof(float4)) != hipSuccess)
+        rc = fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (session accumulator)");
+    if (!rc) rc = session_clear(s);
+    if (rc) {
+        if (s->accum) (void)hipFree(s->accum);
+        delete s;
+        return rc;
+    }
+    lucid_scene_retain(scene);
+    s->scene = scene;
+    *out = s;
+    return LUCID_OK;
+}
+
+void lucid_session_destroy(lucid_session* s) {
+    if (!s) return;
+    session_stop(s);
+    lucid_ctx* ctx = s->ctx;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (s->accum) (void)hipFree(s->accum);
+        if (s->snap_dev) (void)hipFree(s->snap_dev);
+        if (s->snap_pinned) (void)hipHostFree(s->snap_pinned);
+    }
+    lucid_scene_release(s->scene);
+    delete s;
+}
+
+int lucid_session_set_camera(lucid_session* s, const lucid_camera* cam, int32_t max_depth) {
+    if (!s || !cam || max_depth < 0) return LUCID_ERR_INVALID;
+    lucid_ctx* ctx = s->ctx;
+    if (s->running.load()) return fail(ctx, LUCID_ERR_INVALID, "session is accumulating asynchronously");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    s->cam = *cam;
+    s->max_depth = max_depth;
+    return session_clear(s);
+}
+
+int lucid_session_clear(lucid_session* s) {
+    if (!s) return LUCID_ERR_INVALID;
+    lucid_ctx* ctx = s->ctx;
+    if (s->running.load()) return fail(ctx, LUCID_ERR_INVALID, "session is accumulating asynchronously");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    return session_clear(s);
+}
+
+int64_t lucid_session_samples(const lucid_session* s) { return s ? s->samples : 0; }
+
+int lucid_sessions_render(lucid_ctx* ctx, lucid_session* const* sessions, int32_t n, int32_t samples) {
+    if (!ctx) return LUCID_ERR_INVALID;
+    if (n < 0 || (n > 0 && !sessions)) return fail(ctx, LUCID_ERR_INVALID, "null session list");
+    if (samples < 1) return fail(ctx, LUCID_ERR_INVALID, "samples must be positive");
+    for (int32_t i = 0; i < n; ++i) {
+        if (!sessions[i] || sessions[i]->ctx != ctx) return fail(ctx, LUCID_ERR_INVALID, "session of another context");
+        if (sessions[i]->running.load())
+            return fail(ctx, LUCID_ERR_INVALID, "session is accumulating asynchronously");
+        for (int32_t j = 0; j < i; ++j)
+            if (sessions[j] == sessions[i]) return fail(ctx, LUCID_ERR_INVALID, "session listed twice");
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    int rc = ensure_lanes(ctx);
+    if (rc) return rc;
+    // Longest-processing-time assignment of the sessions to lanes; guided
+    // sessions share the context's network buffers and all run on lane 0.
+    std::vector<int32_t> order(n);
+    for (int32_t i = 0; i < n; ++i) order[i] = i;
+    auto work = [&](int32_t i) { return (int64_t)sessions[i]->w * sessions[i]->h; };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return work(a) > work(b); });
+    std::vector<int32_t> lane_of(n);
+    int64_t load[kLanes] = {0, 0, 0, 0};
+    for (int32_t i : order) {
+        int l = 0;
+        if (sessions[i]->algorithm != LUCID_ALGO_GUIDED_MIS)
+            for (int k = 1; k < kLanes; ++k)
+                if (load[k] < load[l]) l = k;
+        lane_of[i] = l;
+        load[l] += work(i);
+    }
+    int nl = 1;
+    for (int k = 0; k < kLanes; ++k)
+        if (load[k] > 0) nl = k + 1;
+    ctx->stats = lucid_stats{};
+    ctx->kev_used = 0;
+    HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int k = 0; k < nl; ++k) {
+        Lane& L = ctx->lanes[k];
+        rc = ensure_counters(ctx, L);
+        if (rc) return rc;
+        if (k > 0) HIP_OK(hipStreamWaitEvent(L.stream, ctx->ev0, 0));
+        k_zero_counters<<<1, 64, 0, L.stream>>>(L.W.cnt, L.W.rays);
+    }
+    // issue round-robin over the lanes so that every lane starts early
+    std::vector<std::vector<int32_t>> per_lane(kLanes);
+    for (int32_t i : order) per_lane[lane_of[i]].push_back(i);
+    bool cancelled = false;
+    for (size_t r = 0; !rc && !cancelled; ++r) {
+        bool any = false;
+        for (int k = 0; k < nl && !rc && !cancelled; ++k) {
+            if (r >= per_lane[k].size()) continue;
+            any = true;
+            rc = session_pass(sessions[per_lane[k][r]], ctx->lanes[k], samples, nl, &ctx->cancel, &cancelled);
+        }
+        if (!any) break;
+    }
+    for (int k = 1; k < nl; ++k) {
+        (void)hipEventRecord(ctx->lanes[k].wave_ev[0], ctx->lanes[k].stream);
+        (void)hipStreamWaitEvent(ctx->stream, ctx->lanes[k].wave_ev[0], 0);
+    }
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc)
+        rc = fail(ctx, LUCID_ERR_DEVICE, "session pass: synchronize failed");
+    if (rc) return rc;
+    rc = finish_stats(ctx, nl);
+    if (rc) return rc;
+    return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
+}
+
+int lucid_session_read(lucid_session* s, float* out_rgba, int32_t average, int64_t* samples) {
+    if (!s || !out_rgba) return LUCID_ERR_INVALID;
+    lucid_ctx* ctx = s->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    const int64_t P = (int64_t)s->w * s->h;
+    int rc = ensure_buf(ctx, &ctx->out_dev, &ctx->out_cap, P * 4);
+    if (rc) return rc;
+    const int64_t n = s->samples;
+    CameraBasis cb = camera_basis(s->cam, s->w, s->h);
+    const float inv = average ? (n > 0 ? 1.0f / (float)n : 0.0f) : 1.0f;
+    k_session_out<<<blocks(P, 256), 256, 0, ctx->stream>>>(s->accum, ctx->out_dev, s->w, s->h, inv, cb.sensitivity);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out_rgba, ctx->out_dev, P * 4 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (samples) *samples = n;
+    return LUCID_OK;
+}
+
+int lucid_session_start_async(lucid_session* s) {
+    if (!s) return LUCID_ERR_INVALID;
+    lucid_ctx* ctx = s->ctx;
+    session_stop(s);
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        HIP_OK(hipSetDevice(ctx->device));
+        const size_t bytes = (size_t)s->w * s->h * 4 * sizeof(float);
+        if (!s->snap_dev && hipMalloc((void**)&s->snap_dev, bytes) != hipSuccess)
+            return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (session snapshot)");
+        if (!s->snap_pinned && hipHostMalloc((void**)&s->snap_pinned, bytes, hipHostMallocDefault) != hipSuccess)
+            return fail(ctx, LUCID_ERR_NOMEM, "hipHostMalloc failed (session snapshot)");
+        int rc = session_clear(s);
+        if (rc) return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->smu);
+        s->snap_samples = 0;
+    }
+    s->err.clear();
+    s->failed.store(false);
+    s->reset_pending.store(false);
+    s->stop.store(0);
+    s->revision.store(0);
+    s->running.store(true);
+    try {
+        s->worker = std::thread(session_worker, s);
+    } catch (...) {
+        s->running.store(false);
+        return fail(ctx, LUCID_ERR_DEVICE, "cannot start the session worker thread");
+    }
+    return LUCID_OK;
+}
+
+int lucid_session_stop_async(lucid_session* s) {
+    if (!s) return LUCID_ERR_INVALID;
+    session_stop(s);
+    return LUCID_OK;
+}
+
+int lucid_session_reset_async(lucid_session* s, const lucid_camera* cam, int32_t width, int32_t height,
+                              int32_t max_depth) {
+    if (!s || !cam || max_depth < 0) return LUCID_ERR_INVALID;
+    if (!s->running.load()) return LUCID_OK;
+    if (width != s->w || height != s->h) {
+        std::lock_guard<std::mutex> lock(s->ctx->mu);  // the worker may be writing the error text
+        return fail(s->ctx, LUCID_ERR_INVALID, "reset with other dimensions: stop and start the session instead");
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->pmu);
+        s->pend_cam = *cam;
+        s->pend_depth = max_depth;
+    }
+    s->reset_pending.store(true, std::memory_order_release);
+    return LUCID_OK;
+}
+
+int lucid_session_poll_async(lucid_session* s, float* out_rgba, int64_t* samples, uint32_t* revision) {
+    if (!s) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(s->smu);
+    const int64_t n = s->running.load() ? s->snap_samples : 0;
+    if (samples) *samples = n;
+    if (revision) *revision = s->revision.load(std::memory_order_acquire);
+    if (out_rgba && n > 0) std::memcpy(out_rgba, s->snap.data(), s->snap.size() * sizeof(float));
+    return LUCID_OK;
+}
+
+int32_t lucid_session_async_running(const lucid_session* s) { return s && s->running.load() ? 1 : 0; }
+
+const char* lucid_session_error(const lucid_session* s) {
+    if (!s) return "null session";
+    return s->failed.load(std::memory_order_acquire) ? s->err.c_str() : "";
 }
 
 }  // extern "C"

@@ -296,6 +296,7 @@ class Renderer {
 public:
     Renderer() { std::cerr << "[PyRenderer] HIP wavefront path tracer (liblucid_hip)\n"; }
     ~Renderer() {
+        if (async_) lucid_session_destroy(async_);
         if (scene_) lucid_scene_release(scene_);
         if (ctx_) lucid_destroy(ctx_);
     }
@@ -368,6 +369,67 @@ public:
         }
         check(rc);
         return px;
+    }
+
+    // -- async accumulator (pybind_renderer.hpp:689-855) --------------------
+    // A session of the current scene, camera and algorithm accumulates on a
+    // worker thread; poll returns its latest averaged snapshot.
+    void render_start_async(int width, int height, int max_depth) {
+        reset_cancel();
+        std::lock_guard<std::mutex> la(async_mu_);
+        stop_async_locked();
+        if (!scene_loaded_ || !camera_set_ || width <= 0 || height <= 0) return;
+        lucid_session* s = nullptr;
+        {
+            std::lock_guard<std::mutex> l(render_mu_);
+            ensure_scene();
+            check(lucid_session_create(ctx_, scene_, &cam_, width, height, std::max(1, max_depth),
+                                       algo_code(algorithm_), &s));
+        }
+        const int rc = lucid_session_start_async(s);
+        if (rc != LUCID_OK) {
+            lucid_session_destroy(s);
+            check(rc);
+        }
+        async_ = s;
+        async_w_ = width;
+        async_h_ = height;
+    }
+    // (samples, RGBA averaged, rows bottom first); samples 0 = no snapshot
+    std::pair<int64_t, std::vector<float>> render_poll_async() {
+        std::lock_guard<std::mutex> la(async_mu_);
+        if (!async_) return {0, {}};
+        int64_t n = 0;
+        uint32_t rev = 0;
+        check(lucid_session_poll_async(async_, nullptr, &n, &rev));
+        if (n <= 0) return {0, {}};
+        std::vector<float> px((size_t)async_w_ * async_h_ * 4);
+        check(lucid_session_poll_async(async_, px.data(), &n, &rev));
+        if (n <= 0) return {0, {}};
+        return {n, std::move(px)};
+    }
+    void render_stop_async() {
+        std::lock_guard<std::mutex> la(async_mu_);
+        stop_async_locked();
+    }
+    void render_reset_async(int width, int height, int max_depth) {
+        std::lock_guard<std::mutex> la(async_mu_);
+        if (!async_ || !scene_loaded_ || !camera_set_) return;
+        const int rc = lucid_session_reset_async(async_, &cam_, width, height, std::max(1, max_depth));
+        if (rc != LUCID_OK)  // the reference logs a size mismatch and keeps running
+            std::cerr << "[PyRenderer] reset_async failed: " << lucid_last_error(ctx_) << "\n";
+    }
+    bool is_render_async_running() {
+        std::lock_guard<std::mutex> la(async_mu_);
+        return async_ != nullptr;
+    }
+    int render_snapshot_revision_async() {
+        std::lock_guard<std::mutex> la(async_mu_);
+        if (!async_) return 0;
+        int64_t n = 0;
+        uint32_t rev = 0;
+        check(lucid_session_poll_async(async_, nullptr, &n, &rev));
+        return (int)rev;
     }
 
     // Extension: per-sample radiance, shape (tile_h * tile_w, samples, 3).
@@ -680,7 +742,14 @@ public:
 
 private:
     std::atomic<bool> cancel_{false};
-    std::mutex ctx_mu_, render_mu_;
+    std::mutex ctx_mu_, render_mu_, async_mu_;  // lock order: async_mu_, render_mu_
+    lucid_session* async_ = nullptr;
+    int async_w_ = 0, async_h_ = 0;
+    void stop_async_locked() {
+        if (!async_) return;
+        lucid_session_destroy(async_);  // stops the worker after its current wave
+        async_ = nullptr;
+    }
     lucid_ctx* ctx_ = nullptr;
     lucid_scene* scene_ = nullptr;
     std::unique_ptr<HostScene> host_;
@@ -793,6 +862,34 @@ PYBIND11_MODULE(lucidrenderer, m) {
              py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"),
              py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,
              "Render a tile and return the RGBA float list (SUM over samples, bottom row first)")
+        .def("render_start_async", &Renderer::render_start_async, py::call_guard<py::gil_scoped_release>(),
+             py::arg("width"), py::arg("height"), py::arg("max_depth") = 8,
+             "Start a background accumulation of the current scene and camera into an on-GPU sum "
+             "(stops any prior session first)")
+        .def(
+            "render_poll_async",
+            [](Renderer& self) {
+                std::pair<int64_t, std::vector<float>> r;
+                {
+                    py::gil_scoped_release rel;
+                    r = self.render_poll_async();
+                }
+                if (r.second.empty()) return py::make_tuple((int)r.first, py::array_t<float>{});
+                py::array_t<float> a((py::ssize_t)r.second.size());
+                std::memcpy(a.mutable_data(), r.second.data(), r.second.size() * sizeof(float));
+                return py::make_tuple((int)r.first, std::move(a));
+            },
+            "Return (samples_completed, pixels_numpy): RGBA flat, bottom row first, alpha 1, averaged over "
+            "`samples`; samples == 0 means no snapshot yet")
+        .def("render_snapshot_revision_async", &Renderer::render_snapshot_revision_async,
+             py::call_guard<py::gil_scoped_release>(), "Monotonic snapshot counter")
+        .def("render_stop_async", &Renderer::render_stop_async, py::call_guard<py::gil_scoped_release>(),
+             "Stop the background session. No-op if not running.")
+        .def("render_reset_async", &Renderer::render_reset_async, py::call_guard<py::gil_scoped_release>(),
+             py::arg("width"), py::arg("height"), py::arg("max_depth") = 8,
+             "Non-blocking in-place reset of the running session with the current camera (same size only)")
+        .def("is_render_async_running", &Renderer::is_render_async_running,
+             py::call_guard<py::gil_scoped_release>(), "True iff a background session exists")
         .def("render_tile_gpu", &Renderer::render_tile, py::call_guard<py::gil_scoped_release>(),
              py::arg("tile_x"), py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"),
              py::arg("full_h"), py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,

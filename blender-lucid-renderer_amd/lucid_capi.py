@@ -128,6 +128,21 @@ SIGNATURES = {
                                            C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                            C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_session_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.POINTER(C.c_void_p)]),
+    "lucid_session_destroy": (None, [C.c_void_p]),
+    "lucid_session_set_camera": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32]),
+    "lucid_session_clear": (C.c_int, [C.c_void_p]),
+    "lucid_session_samples": (C.c_int64, [C.c_void_p]),
+    "lucid_sessions_render": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.c_int32]),
+    "lucid_session_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_int64)]),
+    "lucid_session_start_async": (C.c_int, [C.c_void_p]),
+    "lucid_session_stop_async": (C.c_int, [C.c_void_p]),
+    "lucid_session_reset_async": (C.c_int, [C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_int32]),
+    "lucid_session_poll_async": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_uint32)]),
+    "lucid_session_async_running": (C.c_int32, [C.c_void_p]),
+    "lucid_session_error": (C.c_char_p, [C.c_void_p]),
     "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
@@ -279,6 +294,51 @@ class Context:
         if getattr(self, "h", None):
             self.L.lucid_destroy(self.h)
             self.h = None
+
+
+class Session:
+    """A lucid_session: progressive accumulation of one frame of one scene."""
+
+    def __init__(self, ctx: Context, scene_handle, cam: Camera, width: int, height: int, max_depth: int,
+                 algorithm: str = "mis"):
+        self.ctx, self.width, self.height = ctx, int(width), int(height)
+        h = C.c_void_p()
+        ctx.check(ctx.L.lucid_session_create(ctx.h, scene_handle, C.byref(cam), self.width, self.height, int(max_depth),
+                                             ALGOS.get(algorithm, 1), C.byref(h)))
+        self.h = h
+
+    @property
+    def samples(self) -> int:
+        return int(self.ctx.L.lucid_session_samples(self.h))
+
+    def read(self, average: bool = False) -> np.ndarray:
+        out = np.zeros(self.width * self.height * 4, np.float32)
+        n = C.c_int64()
+        self.ctx.check(self.ctx.L.lucid_session_read(self.h, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                     1 if average else 0, C.byref(n)))
+        return out
+
+    def poll(self):
+        """(samples, pixels or None, revision) of the async snapshot."""
+        n, rev = C.c_int64(), C.c_uint32()
+        self.ctx.check(self.ctx.L.lucid_session_poll_async(self.h, None, C.byref(n), C.byref(rev)))
+        if n.value <= 0:
+            return 0, None, rev.value
+        out = np.zeros(self.width * self.height * 4, np.float32)
+        self.ctx.check(self.ctx.L.lucid_session_poll_async(self.h, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                                           C.byref(n), C.byref(rev)))
+        return n.value, out, rev.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.lucid_session_destroy(self.h)
+            self.h = None
+
+
+def render_sessions(ctx: Context, sessions, samples: int):
+    """One batched pass: `samples` more samples in every session."""
+    arr = (C.c_void_p * len(sessions))(*[s.h.value for s in sessions])
+    ctx.check(ctx.L.lucid_sessions_render(ctx.h, arr, len(sessions), int(samples)))
 
 
 def host_trace(scene: dict, o, d, tmin, tmax, any_hit=False):

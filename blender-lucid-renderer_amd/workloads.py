@@ -325,6 +325,79 @@ def relmse(x: np.ndarray, r: np.ndarray, eps: float = 0.01) -> float:
     return float(np.mean((x - r) ** 2 / (r * r + eps)))
 
 
+def session_scene(seed: int, triangles: int) -> dict:
+    """A scene of about `triangles` triangles for the batched-session
+    workload: a ground quad and displaced icospheres (5,120 triangles each,
+    1,280 below 10k), Principled or Emission materials (one object in seven
+    emits), and 1-8 native lights of every type."""
+    rng = np.random.default_rng(seed)
+    level = 4 if triangles >= 10000 else 3
+    if level not in _ICO_CACHE:
+        _ICO_CACHE[level] = _icosphere(level)
+    base_v, base_t = _ICO_CACHE[level]
+    count = max(1, int(round(triangles / len(base_t))))
+    extent = 3.0 + 1.5 * math.sqrt(count)
+    meshes = [_mesh(*_quad((-extent, 0, -extent), (-extent, 0, extent), (extent, 0, extent), (extent, 0, -extent)),
+                    _diffuse(rng.uniform(0.3, 0.8, 3).tolist()))]
+    for i in range(count):
+        radius = rng.uniform(0.3, 0.9)
+        center = np.array([rng.uniform(-extent, extent), rng.uniform(0.4, 3.0), rng.uniform(-extent, extent)])
+        k = rng.normal(size=(3, 3))
+        k /= np.linalg.norm(k, axis=1, keepdims=True)
+        disp = 1.0 + (rng.uniform(0.02, 0.1, 3)[None, :] * np.sin(3.0 * base_v @ k.T)).sum(1)
+        pos = base_v * (radius * disp)[:, None] + center
+        if rng.uniform() < 1.0 / 7.0:
+            mat = {"base_color": [0.0, 0.0, 0.0], "metallic": 0.0, "roughness": 1.0,
+                   "emission": (rng.uniform(0.5, 1.0, 3) * rng.uniform(2.0, 8.0)).tolist(),
+                   "transmission": 0.0, "ior": 1.5}
+        else:
+            mat = {"base_color": rng.uniform(0.15, 0.9, 3).tolist(),
+                   "metallic": 1.0 if rng.uniform() < 0.3 else 0.0,
+                   "roughness": float(rng.uniform(0.05, 1.0)), "emission": [0.0, 0.0, 0.0],
+                   "transmission": 0.0, "ior": float(rng.uniform(1.3, 1.7))}
+        meshes.append(_mesh(pos, base_t, mat))
+    lights = []
+    for j in range(int(rng.integers(1, 9))):
+        kind = ("POINT", "SUN", "SPOT", "AREA")[j % 4 if j < 4 else int(rng.integers(0, 4))]
+        pos = [float(rng.uniform(-extent, extent)), float(rng.uniform(4.0, 9.0)), float(rng.uniform(-extent, extent))]
+        col = rng.uniform(0.7, 1.0, 3).tolist()
+        if kind == "POINT":
+            lights.append({"type": "POINT", "position": pos, "color": col, "energy": float(rng.uniform(300, 1500)),
+                           "radius": float(rng.choice([0.0, 0.25]))})
+        elif kind == "SUN":
+            lights.append({"type": "SUN", "position": pos, "direction": [-0.3, -1.0, -0.2], "color": col,
+                           "energy": float(rng.uniform(1.0, 3.0))})
+        elif kind == "SPOT":
+            lights.append({"type": "SPOT", "position": pos, "direction": [0.0, -1.0, 0.1], "color": col,
+                           "energy": float(rng.uniform(500, 2000)), "spot_size": 1.0, "spot_blend": 0.2})
+        else:
+            lights.append({"type": "AREA", "position": pos, "direction": [0.0, -1.0, 0.0], "right": [1.0, 0.0, 0.0],
+                           "up": [0.0, 0.0, 1.0], "shape": "RECTANGLE", "size": 2.0, "size_y": 1.0,
+                           "color": col, "energy": float(rng.uniform(300, 900))})
+    return {"meshes": meshes, "native_lights": lights,
+            "environment": {"color": [0.05, 0.06, 0.08], "strength": 1.0}}
+
+
+def session_camera(scene: dict) -> dict:
+    """Looks at the scene from outside its ground quad, slightly above."""
+    ext = float(np.max(np.abs(np.asarray(scene["meshes"][0]["vertices"], np.float64))))
+    return {"pos": (0.0, 0.35 * ext + 2.0, 1.6 * ext), "dir": (0.0, -0.3, -1.0), "up": (0.0, 1.0, 0.0),
+            "fov": 50.0}
+
+
+def batched_sessions(seed: int = 7, count: int = 32, tri_range=(1e4, 5e5), edge_range=(128, 1024)):
+    """`count` sessions: (scene, camera, width, height) with log-uniform
+    edges and triangle counts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        w, h = (int(round(math.exp(rng.uniform(math.log(edge_range[0]), math.log(edge_range[1]))))) for _ in "wh")
+        tris = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+        sc = session_scene(int(rng.integers(1 << 30)), tris)
+        out.append((sc, session_camera(sc), w, h))
+    return out
+
+
 def to_json(scene: dict) -> str:
     return json.dumps(scene, separators=(",", ":"))
 

@@ -315,6 +315,73 @@ int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, in
 /* Instrumentation flags (LUCID_PROFILE_*), 0 = off (default). */
 int lucid_set_profiling(lucid_ctx* ctx, int32_t flags);
 
+/* ---------------------------------------------------------- sessions --
+   A session is a progressive accumulation of one full frame (width x
+   height) of one scene and camera: a device radiance sum and its sample
+   count.  Every pass adds samples [n, n + k) with the per-pixel seeds of
+   render_tile and jitter on every sample, so a session that has taken N
+   samples in any number of passes holds exactly the sum of one
+   render_tile(0, 0, w, h, w, h, N, 0, max_depth) call (N > 1).  Sessions
+   retain their scene; the library owns their device memory. */
+typedef struct lucid_session lucid_session;
+
+/* Replaces the per-session state of the reference's render sessions
+   (P:render_session.py:530-608 chaining render_tile with sample_offset =
+   running total) and of its async accumulator
+   (src/gpu/path_tracer.cpp:838-884, created by start_async :1106-1245). */
+int lucid_session_create(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, int32_t width, int32_t height,
+                         int32_t max_depth, int32_t algorithm, lucid_session** out);
+/* Stops a running worker and frees the session (stop_async + the
+   AsyncState destructor, src/gpu/path_tracer.cpp:1247-1254). */
+void lucid_session_destroy(lucid_session* s);
+/* New camera and depth; clears the accumulator.  Not while async. */
+int lucid_session_set_camera(lucid_session* s, const lucid_camera* cam, int32_t max_depth);
+int lucid_session_clear(lucid_session* s);
+int64_t lucid_session_samples(const lucid_session* s);
+
+/* One pass over many sessions: adds `samples` samples to each.  Sessions
+   are spread over the context's streams (longest first) so that their
+   launches overlap; guided sessions run one after another on the first.
+   Replaces the per-session render_tile calls of the reference's session
+   layer (P:render_session.py:103 runs one worker per session, serialised
+   by gpu_mutex_, include/pybind_renderer.hpp:1118).  Honours lucid_cancel:
+   returns LUCID_ERR_CANCELLED with every issued wave counted. */
+int lucid_sessions_render(lucid_ctx* ctx, lucid_session* const* sessions, int32_t n, int32_t samples);
+
+/* Reads the accumulator as render_tile's layout (rows bottom first, alpha
+   1): average != 0 gives (sum / samples) * 2^EV clamped at 0 (what
+   render_poll_async returns), average == 0 the sum * 2^EV clamped at 0
+   (what render_tile returns). */
+int lucid_session_read(lucid_session* s, float* out_rgba, int32_t average, int64_t* samples);
+
+/* Async accumulation on a worker thread: PyRenderer::render_start_async
+   (include/pybind_renderer.hpp:689-753; the worker of
+   src/gpu/path_tracer.cpp:887-955).  Clears the accumulator and starts
+   passes of 1, 2, 4, ... samples (grown while a pass takes < 20 ms, at most
+   256), publishing an averaged snapshot after every pass.  Cancel of the
+   context does not stop it; lucid_session_stop_async does. */
+int lucid_session_start_async(lucid_session* s);
+/* render_stop_async (pybind_renderer.hpp:787-794): stops after the current
+   wave and publishes a final snapshot. */
+int lucid_session_stop_async(lucid_session* s);
+/* render_reset_async (pybind_renderer.hpp:800-834, reset_async
+   src/gpu/path_tracer.cpp:1256-1276): new camera and depth, the
+   accumulator is cleared before the next pass and no snapshot of the old
+   camera is published after this call.  No-op when not running;
+   LUCID_ERR_INVALID when the size differs (stop and start instead). */
+int lucid_session_reset_async(lucid_session* s, const lucid_camera* cam, int32_t width, int32_t height,
+                              int32_t max_depth);
+/* render_poll_async / render_snapshot_revision_async
+   (pybind_renderer.hpp:755-785, 848-855): the latest snapshot's sample
+   count (0 = none yet, or not running), its pixels into out_rgba when it
+   is not NULL and the count is > 0 (width * height * 4 floats, rows bottom
+   first), and the monotonic snapshot revision. */
+int lucid_session_poll_async(lucid_session* s, float* out_rgba, int64_t* samples, uint32_t* revision);
+/* is_render_async_running (pybind_renderer.hpp:836-843). */
+int32_t lucid_session_async_running(const lucid_session* s);
+/* Why the async worker stopped on its own ("" if it did not). */
+const char* lucid_session_error(const lucid_session* s);
+
 #ifdef __cplusplus
 }
 #endif

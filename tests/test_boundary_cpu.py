@@ -21,7 +21,8 @@ REFERENCE_METHODS = [
     "render_tile_numpy", "is_scene_loaded", "is_camera_set", "get_algorithm", "get_mesh_count",
     "enable_diagnostics", "disable_diagnostics", "is_diagnostics_enabled", "clear_diagnostics",
     "get_diagnostic_stats", "get_diagnostic_variance_map", "export_diagnostic_json", "get_top_variance_groups",
-    "get_pixel_diagnostic",
+    "get_pixel_diagnostic", "render_start_async", "render_poll_async", "render_reset_async", "render_stop_async",
+    "render_snapshot_revision_async", "is_render_async_running",
 ]
 REFERENCE_CLASSES = [
     "Renderer", "PathRecordingConfig", "ExportedGroupInfo", "TopGroupSortBy", "PixelDiagnosticInfo",
@@ -56,6 +57,30 @@ def test_capi_rejects_bad_arguments_without_device():
     assert L.lucid_scene_num_triangles(None) == 0
     L.lucid_cancel(None)
     L.lucid_scene_release(None)
+    # sessions
+    assert L.lucid_session_create(None, None, None, 4, 4, 8, 2, None) == 1
+    assert L.lucid_sessions_render(None, None, 0, 1) == 1
+    assert L.lucid_session_poll_async(None, None, None, None) == 1
+    assert L.lucid_session_read(None, None, 1, None) == 1
+    assert L.lucid_session_start_async(None) == 1 and L.lucid_session_stop_async(None) == 1
+    assert L.lucid_session_reset_async(None, None, 4, 4, 8) == 1
+    assert L.lucid_session_samples(None) == 0 and L.lucid_session_async_running(None) == 0
+    assert L.lucid_session_error(None) == b"null session"
+    L.lucid_session_destroy(None)
+
+
+def test_async_api_without_scene_stays_idle(lucid):
+    """render_start_async without a scene or camera returns and stays idle
+    (pybind_renderer.hpp:689-715); poll gives (0, empty)."""
+    r = lucid.Renderer()
+    r.render_start_async(64, 48, 8)
+    assert r.is_render_async_running() is False
+    n, px = r.render_poll_async()
+    assert n == 0 and px.size == 0
+    assert r.render_snapshot_revision_async() == 0
+    r.render_reset_async(64, 48, 8)
+    r.render_stop_async()
+    assert r.is_render_async_running() is False
 
 
 def test_module_surface(lucid):
