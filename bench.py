@@ -29,6 +29,10 @@ counted on the device) over the HIP-event average launch duration, against
 8 TB/s -- and cpu_baseline: the oracle's -O3 -ffast-math build of the same
 algorithm on the host cores, on a bounded strip of the same frame.
 
+sessions: 32 progressive sessions of different scenes and sizes (5.7M
+pixels, 5M triangles in all), Mrays/s of batched 16-spp passes over the
+library's four stream lanes against the same passes one session at a time.
+
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--budget SECONDS]
        python bench.py --workload icospheres      (1M triangles, 1024^2, 64 spp: Mrays/s only)
 """
@@ -69,7 +73,9 @@ def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
     t0 = time.perf_counter()
     o.render_tile("mis", 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0
-    rows2 = int(max(1, min(H, rows * seconds_target / max(dt, 1e-3))))
+    total = seconds_target / max(dt / (rows * samples), 1e-9)  # row-samples that fit the target
+    rows2 = int(max(1, min(H, total / samples)))
+    samples = int(max(samples, total / rows2))  # a full frame that is quick gets more samples
     y0 = max(0, H // 2 - rows2 // 2)
     t0 = time.perf_counter()
     _, ex = o.render_tile("mis", 0, y0, W, rows2, W, H, samples=samples, max_depth=depth, threads=threads)
@@ -95,6 +101,55 @@ def time_to_target(curve):
     return c / TARGET, "extrapolated (relMSE ~ 1/t over the last half of the run)"
 
 
+def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
+    """Batched sessions: `count` sessions (edges log-uniform on [128, 1024],
+    scenes log-uniform on [10k, 500k] triangles, Principled and Emission
+    materials, 1-8 lights), one batched pass of `spp` samples per session
+    per step, against the same passes issued one session at a time."""
+    import lucid_capi
+    import workloads
+
+    t0 = time.perf_counter()
+    specs = workloads.batched_sessions(seed=seed, count=count)
+    scenes = [ctx.scene(sc) for sc, _, _, _ in specs]
+    sessions = [lucid_capi.Session(ctx, h, lucid_capi.camera(cam), w, hh, 8, "mis")
+                for h, (_, cam, w, hh) in zip(scenes, specs)]
+    tris = sum(int(ctx.L.lucid_scene_num_triangles(h)) for h in scenes)
+    pixels = sum(w * hh for _, _, w, hh in specs)
+    log(f"sessions: {count} scenes, {tris} triangles, {pixels} pixels, set up in {time.perf_counter() - t0:.1f} s")
+    lucid_capi.render_sessions(ctx, sessions, spp)  # warm-up: allocates every lane's buffers
+    for s in sessions:  # and the serial path's
+        lucid_capi.render_sessions(ctx, [s], 1)
+
+    def timed(batched):
+        rays, t = 0, 0.0
+        for _ in range(passes):
+            ts = time.perf_counter()
+            if batched:
+                lucid_capi.render_sessions(ctx, sessions, spp)
+                st = ctx.stats()
+                rays += st["rays_closest"] + st["rays_shadow"]
+            else:
+                for s in sessions:
+                    lucid_capi.render_sessions(ctx, [s], spp)
+                    st = ctx.stats()
+                    rays += st["rays_closest"] + st["rays_shadow"]
+            t += time.perf_counter() - ts
+        return rays / t / 1e6, t / passes * 1e3
+
+    b_mrays, b_ms = timed(True)
+    s_mrays, s_ms = timed(False)
+    log(f"sessions: batched {b_mrays:.0f} Mrays/s ({b_ms:.0f} ms/pass), one at a time {s_mrays:.0f} Mrays/s")
+    for s in sessions:
+        s.close()
+    for h in scenes:
+        ctx.L.lucid_scene_release(h)
+    return {"count": count, "seed": seed, "triangles": tris, "pixels": pixels, "pass_spp": spp, "max_depth": 8,
+            "algorithm": "mis", "passes": passes, "batched_mrays_s": round(b_mrays, 1),
+            "batched_ms_per_pass": round(b_ms, 1), "serial_mrays_s": round(s_mrays, 1),
+            "serial_ms_per_pass": round(s_ms, 1), "speedup": round(b_mrays / max(s_mrays, 1e-9), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +160,8 @@ def main():
     ap.add_argument("--pass-spp", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE leg")
+    ap.add_argument("--no-sessions", action="store_true", help="skip the 32-session batched leg")
+    ap.add_argument("--session-passes", type=int, default=2)
     ap.add_argument("--alpha", type=float, default=0.7)
     ap.add_argument("--train-steps", type=int, default=64)
     ap.add_argument("--batch", type=int, default=32768)
@@ -280,9 +337,14 @@ def main():
                         "samples": int(gst.samples_total),
                         "adam_steps": int(gst.train_steps), "last_loss": round(float(gst.last_loss), 4)},
         }
+    if not args.no_sessions:
+        result["sessions"] = sessions_leg(ctx, args.session_passes)
     if not args.no_cpu:
         log("cpu baseline (oracle, fast build) ...")
         result["cpu_baseline"] = cpu_baseline(scene, cam, W, H, depth)
+        if "time_to_relmse" in result:  # same algorithm and spp: scaled by the throughput ratio
+            tr = result["time_to_relmse"]
+            tr["cpu_plain_s_est"] = round(tr["plain_s"] * mrays / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result), flush=True)
     L.lucid_scene_release(sc)
     ctx.close()

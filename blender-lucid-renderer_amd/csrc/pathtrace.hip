@@ -59,7 +59,8 @@ struct SceneDev {
     const float* cdf;
     const int32_t* native_ids;
     const GVolume* vols;
-    const float* grid;
+    const float* grid;      // FP32 density grids, or null when
+    const __half* grid16;   // the scene stores them as FP16
     int32_t n_lights, n_native, n_vols, root;
     float total_area;
     float env[3];
@@ -331,6 +332,10 @@ __device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float 
 
 
 // ------------------------------------------------------------------ media --
+__device__ __forceinline__ float grid_value(const SceneDev& S, const GVolume& v, float x, float y, float z) {
+    return S.grid16 ? grid_trilinear(v, S.grid16, x, y, z) : grid_trilinear(v, S.grid, x, y, z);
+}
+
 // transmittance_along_segment (absorption, transmittance.hpp:77-128) and
 // extinction_transmittance_along_segment (:145-209): midpoint ray march at
 // half a voxel over every gridded medium, T = exp(-tau) per channel.
@@ -352,7 +357,7 @@ __device__ __noinline__ v3f vol_transmittance(const SceneDev& S, v3f o, v3f d, f
         for (int i = 0; i < n; ++i) {
             const float t = tin + ((float)i + 0.5f) * step;
             if (t > tout) break;
-            const float dd = grid_trilinear(v, S.grid, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+            const float dd = grid_value(S, v, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
             if (dd <= 0.0f) continue;
             const float base = v.density * dd * step;
             if (extinction) {
@@ -390,7 +395,7 @@ __device__ __noinline__ int vol_sample_distance(const SceneDev& S, v3f o, v3f d,
             const float us = smax(1e-12f, 1.0f - randf(rng));
             t += -logf(us) / sig;
             if (t >= seg_end) break;
-            const float dd = grid_trilinear(v, S.grid, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+            const float dd = grid_value(S, v, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
             const float st = v.density * dd * emax;
             if (randf(rng) * sig < st) {
                 sc = true;
@@ -1253,6 +1258,7 @@ struct lucid_scene {
     PackedScene host;
     std::vector<void*> allocs;
     SceneDev dev;
+    size_t grid_bytes = 0;  // device bytes of the density grids
     float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // scene box
 };
 
@@ -1285,6 +1291,7 @@ struct lucid_ctx {
     // or 0 for one lane per queued ray.  Traversal latency is best hidden by
     // the hardware dispatcher (0); shading runs best persistent.
     int32_t traverse_groups = 0, shade_groups = 16;
+    int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
     int64_t film_cap = 0;
     float* out_dev = nullptr;
@@ -1940,8 +1947,38 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     if (!rc) rc = upload(ctx, s, ps.lights, &D.lights);
     if (!rc) rc = upload(ctx, s, ps.cdf, &D.cdf);
     if (!rc) rc = upload(ctx, s, ps.native_light_ids, &D.native_ids);
-    if (!rc) rc = upload(ctx, s, ps.volumes, &D.vols);
-    if (!rc) rc = upload(ctx, s, ps.grid, &D.grid);
+    if (!rc) rc = upload(ctx, s, ps.volumes, &D.vols);  // replaced below for FP16 grids
+    if (!rc && ctx->grid_bits == 16 && !ps.grid.empty()) {
+        // FP16 storage: round to nearest even, and take each medium's
+        // majorant from the rounded values it will be tracked against
+        std::vector<__half> g16(ps.grid.size());
+        for (size_t i = 0; i < g16.size(); ++i) {
+            if (!(std::fabs(ps.grid[i]) <= 65504.0f)) {
+                lucid_scene_release(s);
+                return fail(ctx, LUCID_ERR_INVALID, "density grid value outside the FP16 range");
+            }
+            g16[i] = __float2half_rn(ps.grid[i]);
+        }
+        std::vector<GVolume> vols = ps.volumes;
+        for (GVolume& v : vols) {
+            if (!v.has_grid) continue;
+            const size_t n = (size_t)v.dims[0] * v.dims[1] * v.dims[2];
+            float mx = 0.0f;
+            for (size_t i = 0; i < n; ++i) mx = std::max(mx, __half2float(g16[(size_t)v.grid_offset + i]));
+            v.grid_max = mx;
+        }
+        const GVolume* dv = nullptr;
+        rc = upload(ctx, s, vols, &dv);
+        if (!rc) {
+            D.vols = dv;
+            s->host.volumes = vols;
+            rc = upload(ctx, s, g16, &D.grid16);
+        }
+        s->grid_bytes = g16.size() * sizeof(__half);
+    } else if (!rc) {
+        rc = upload(ctx, s, ps.grid, &D.grid);
+        s->grid_bytes = ps.grid.size() * sizeof(float);
+    }
     if (rc) {
         lucid_scene_release(s);
         return rc;
@@ -2156,6 +2193,15 @@ int lucid_set_profiling(lucid_ctx* ctx, int32_t flags) {
     ctx->profiling = flags;
     return LUCID_OK;
 }
+
+int lucid_set_grid_format(lucid_ctx* ctx, int32_t bits) {
+    if (!ctx || (bits != 16 && bits != 32)) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->grid_bits = bits;
+    return LUCID_OK;
+}
+
+int64_t lucid_scene_grid_bytes(const lucid_scene* s) { return s ? (int64_t)s->grid_bytes : 0; }
 
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;

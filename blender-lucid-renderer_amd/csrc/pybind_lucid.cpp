@@ -343,6 +343,19 @@ public:
         camera_set_ = true;
     }
     void set_algorithm(const std::string& a) { algorithm_ = a; }
+    // Extension: density-grid storage (32 = FP32, the reference's; 16 =
+    // FP16); the scene is re-uploaded at the next render.
+    void set_grid_precision(int bits) {
+        if (bits != 16 && bits != 32) throw std::invalid_argument("grid precision must be 16 or 32 bits");
+        std::lock_guard<std::mutex> l(render_mu_);
+        if (bits == grid_bits_) return;
+        grid_bits_ = bits;
+        if (scene_) {
+            lucid_scene_release(scene_);
+            scene_ = nullptr;
+        }
+    }
+    int get_grid_precision() const { return grid_bits_; }
     std::string get_algorithm() const { return algorithm_; }
     bool is_scene_loaded() const { return scene_loaded_; }
     bool is_camera_set() const { return camera_set_; }
@@ -755,6 +768,7 @@ private:
     std::unique_ptr<HostScene> host_;
     bool scene_loaded_ = false, camera_set_ = false;
     lucid_camera cam_{};
+    int grid_bits_ = 32;
     std::string algorithm_ = "nee";
     std::vector<std::string> names_;
     bool diag_on_ = false;
@@ -816,6 +830,7 @@ private:
     void ensure_scene() {
         ensure_ctx();
         if (scene_) return;
+        check(lucid_set_grid_format(ctx_, grid_bits_));
         std::vector<lucid_mesh_desc> md;
         for (auto& m : host_->meshes) md.push_back(m->desc);
         lucid_scene* s = nullptr;
@@ -855,6 +870,9 @@ PYBIND11_MODULE(lucidrenderer, m) {
              py::arg("up_z"), py::arg("fov_deg"))
         .def("set_algorithm", &Renderer::set_algorithm, py::arg("algorithm"))
         .def("get_algorithm", &Renderer::get_algorithm)
+        .def("set_grid_precision", &Renderer::set_grid_precision, py::arg("bits"),
+             "Extension: store density grids as FP32 (32, default) or FP16 (16)")
+        .def("get_grid_precision", &Renderer::get_grid_precision)
         .def("set_object_names", &Renderer::set_object_names, py::arg("names"))
         .def("get_object_name", &Renderer::get_object_name, py::arg("object_id"))
         .def("object_path_string", &Renderer::object_path_string, py::arg("object_ids"))

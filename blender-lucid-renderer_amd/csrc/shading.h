@@ -5,6 +5,8 @@
 // follows the reference so that IEEE-exact steps give identical bits.
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "device_math.h"
 #include "scene_pack.h"
 
@@ -398,7 +400,13 @@ __device__ __forceinline__ bool ray_aabb_segment(v3f o, v3f d, float tmax, const
     tout = t1;
     return tout > tin;
 }
-__device__ __forceinline__ float grid_trilinear(const GVolume& v, const float* grid, float wx, float wy, float wz) {
+__device__ __forceinline__ float grid_f32(float x) { return x; }
+__device__ __forceinline__ float grid_f32(__half x) { return __half2float(x); }
+
+// Density grids are stored as FP32 (the reference's values) or FP16 (half
+// the bytes; computed in FP32 after widening, which is exact).
+template <class G>
+__device__ __forceinline__ float grid_trilinear(const GVolume& v, const G* grid, float wx, float wy, float wz) {
     float u = (wx - v.wmin[0]) / smax(1e-12f, v.wmax[0] - v.wmin[0]);
     float vv = (wy - v.wmin[1]) / smax(1e-12f, v.wmax[1] - v.wmin[1]);
     float w = (wz - v.wmin[2]) / smax(1e-12f, v.wmax[2] - v.wmin[2]);
@@ -411,11 +419,13 @@ __device__ __forceinline__ float grid_trilinear(const GVolume& v, const float* g
     int iy = min(max((int)fy, 0), ny - 2);
     int iz = min(max((int)fz, 0), nz - 2);
     float tx = fx - (float)ix, ty = fy - (float)iy, tz = fz - (float)iz;
-    const float* g = grid + v.grid_offset;
+    const G* g = grid + v.grid_offset;
     size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
     size_t b0 = (size_t)ix + sy * iy + sz * iz;
-    float c000 = g[b0], c100 = g[b0 + sx], c010 = g[b0 + sy], c110 = g[b0 + sx + sy];
-    float c001 = g[b0 + sz], c101 = g[b0 + sx + sz], c011 = g[b0 + sy + sz], c111 = g[b0 + sx + sy + sz];
+    float c000 = grid_f32(g[b0]), c100 = grid_f32(g[b0 + sx]);
+    float c010 = grid_f32(g[b0 + sy]), c110 = grid_f32(g[b0 + sx + sy]);
+    float c001 = grid_f32(g[b0 + sz]), c101 = grid_f32(g[b0 + sx + sz]);
+    float c011 = grid_f32(g[b0 + sy + sz]), c111 = grid_f32(g[b0 + sx + sy + sz]);
     float c00 = c000 * (1.0f - tx) + c100 * tx;
     float c10 = c010 * (1.0f - tx) + c110 * tx;
     float c01 = c001 * (1.0f - tx) + c101 * tx;

@@ -329,7 +329,8 @@ def session_scene(seed: int, triangles: int) -> dict:
     """A scene of about `triangles` triangles for the batched-session
     workload: a ground quad and displaced icospheres (5,120 triangles each,
     1,280 below 10k), Principled or Emission materials (one object in seven
-    emits), and 1-8 native lights of every type."""
+    emits), and 1-8 native lights of every type.  Vertex and triangle
+    arrays stay numpy arrays (to_json converts them)."""
     rng = np.random.default_rng(seed)
     level = 4 if triangles >= 10000 else 3
     if level not in _ICO_CACHE:
@@ -355,7 +356,7 @@ def session_scene(seed: int, triangles: int) -> dict:
                    "metallic": 1.0 if rng.uniform() < 0.3 else 0.0,
                    "roughness": float(rng.uniform(0.05, 1.0)), "emission": [0.0, 0.0, 0.0],
                    "transmission": 0.0, "ior": float(rng.uniform(1.3, 1.7))}
-        meshes.append(_mesh(pos, base_t, mat))
+        meshes.append({"vertices": pos.astype(np.float32), "triangles": base_t.astype(np.int32), "material": mat})
     lights = []
     for j in range(int(rng.integers(1, 9))):
         kind = ("POINT", "SUN", "SPOT", "AREA")[j % 4 if j < 4 else int(rng.integers(0, 4))]
@@ -399,7 +400,7 @@ def batched_sessions(seed: int = 7, count: int = 32, tri_range=(1e4, 5e5), edge_
 
 
 def to_json(scene: dict) -> str:
-    return json.dumps(scene, separators=(",", ":"))
+    return json.dumps(scene, separators=(",", ":"), default=lambda a: np.asarray(a).tolist())
 
 
 WORKLOADS = {

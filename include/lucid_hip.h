@@ -312,6 +312,17 @@ int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, in
                            const float* d, const float* tmin, const float* tmax, int32_t any_hit, int32_t* mesh,
                            int32_t* tri, float* u, float* v, float* t);
 
+/* Storage of the density grids of scenes created after this call: 32 (FP32,
+   the reference's values, default) or 16 (FP16, round to nearest even; half
+   the device bytes, arithmetic still FP32; each medium's delta-tracking
+   majorant is taken from the rounded values).  FP16 refuses grids with
+   values beyond +-65504.  Replaces the FP32 grid upload of the reference's
+   GPU packer (src/gpu/path_tracer.cpp:162-469, volume_grid_buf) and the
+   float grid of sample_grid_trilinear (include/volume/transmittance.hpp:298-352). */
+int lucid_set_grid_format(lucid_ctx* ctx, int32_t bits);
+/* Device bytes of a scene's density grids. */
+int64_t lucid_scene_grid_bytes(const lucid_scene* scene);
+
 /* Instrumentation flags (LUCID_PROFILE_*), 0 = off (default). */
 int lucid_set_profiling(lucid_ctx* ctx, int32_t flags);
 
