@@ -55,6 +55,9 @@ for _p in (PKG, ORACLE):
 METRIC = "Mrays/s + time-to-relMSE 0.01 (guided vs plain), 1080p, 1×MI355X"
 HBM_PEAK_GBS = 8000.0
 REFERENCE = os.path.join(ROOT, "tests", "golden", "occluded_room_reference.npz")
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of k_intersect on the default workload,
+# scaled by a 1 GiB device copy (profiles/ holds the commands and the table)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_intersect_occluded_room_1080p_16spp.md")
 TARGET = 0.01
 
 
@@ -162,6 +165,8 @@ def main():
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE leg")
     ap.add_argument("--no-sessions", action="store_true", help="skip the 32-session batched leg")
     ap.add_argument("--session-passes", type=int, default=2)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (calibrates profiler byte counters)")
     ap.add_argument("--alpha", type=float, default=0.7)
     ap.add_argument("--train-steps", type=int, default=64)
     ap.add_argument("--batch", type=int, default=32768)
@@ -208,6 +213,15 @@ def main():
         ctx.check(L.lucid_synchronize(ctx.h))
         return ctx.stats()
 
+    if args.copy_probe_mib > 0:
+        a = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log(f"copy probe: {args.copy_probe_mib} MiB read + written")
+
     # ---- timed plain passes: Mrays/s ----
     for i in range(args.warmup):
         s = render(algo, spp, 1000000 + i * spp)
@@ -238,6 +252,11 @@ def main():
     log(f"intersect: {n_launch} launches, avg {avg_ms:.3f} ms, nodes/query {p['nodes_closest'] / max(q, 1):.1f}, "
         f"tris/query {p['tris_closest'] / max(q, 1):.1f}; shade {p['ms_shade']:.1f} ms, shadow {p['ms_shadow']:.1f} ms")
 
+    traffic = None
+    if args.workload == "occluded_room" and spp == 16 and os.path.exists(TRAFFIC_PROFILE):
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
     lights = {"occluded_room": "1 area light behind a 5 cm slit",
               "icospheres": "1 point + 1 sun + 4 area lights"}.get(args.workload, "")
     result = {
@@ -259,7 +278,9 @@ def main():
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/k_intersect_occluded_room_1080p_16spp.md (rocprofv3 PMC)"
+                     if traffic else None,
                      "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
                      "launches": n_launch,
                      "per_query": {"nodes": round(p["nodes_closest"] / max(q, 1), 2),
