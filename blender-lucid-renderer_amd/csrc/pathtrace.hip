@@ -102,6 +102,7 @@ struct WaveDev {
     int32_t tstride;
 };
 constexpr int kTrainSlots = 6;
+constexpr int32_t kRecValid = 1;  // bit 0 of a diagnostic record's packed meta
 
 struct CamDev {
     float pos[3], fwd[3], right[3], up[3];
@@ -222,71 +223,82 @@ LUCID_HD void note_overflow(int32_t* ovf) {
 #endif
 }
 
+// One step of the traversal: visits `node` (tests both child boxes, the
+// triangles of leaf children, then descends, pushes or pops).  Returns false
+// when the traversal has ended (stack empty, or an any-hit found).  The
+// state (node, sp, h) lives with the caller, so a persistent kernel can
+// interleave steps of different rays on one lane.
 template <bool ANY>
-LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk, int sd, int32_t* ovf,
-                       uint32_t& nn, uint32_t& nt, TraceHit& h, HostBounds* hb = nullptr) {
+LUCID_HD bool trav_step(const SceneDev& S, v3f o, v3f d, v3f id, float tmin, int32_t* stk, int sd, int32_t* ovf,
+                        uint32_t& nn, uint32_t& nt, TraceHit& h, int32_t& node, int& sp, HostBounds* hb) {
     (void)hb;
-    h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-    const v3f id = inv_dir(d);
-    int sp = 0;
-    int32_t node = S.root;
-    for (;;) {
-        LUCID_HOST_CHECK(node >= 0 && node < hb->n_nodes);
-        const float4* n4 = reinterpret_cast<const float4*>(S.nodes + node);
-        const float4 b0 = n4[0], b1 = n4[1], b2 = n4[2];
-        const int2 cc = *reinterpret_cast<const int2*>(n4 + 3);
-        ++nn;
-        const float lim = prune_bound(h.t);
-        float te0 = 0.0f, te1 = 0.0f;
-        bool h0 = cc.x != kEmptyChild && box_hit(o, d, id, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, te0) && te0 <= lim;
-        bool h1 = cc.y != kEmptyChild && box_hit(o, d, id, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w, te1) && te1 <= lim;
-        const bool sw = h1 && (!h0 || te1 < te0);
-        int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
-        bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
-        const float tf = sw ? te0 : te1;
-        // Leaf children are consumed here, whether tested or pruned, so only
-        // internal nodes (index >= 0) are ever descended into or pushed.
+    LUCID_HOST_CHECK(node >= 0 && node < hb->n_nodes);
+    const float4* n4 = reinterpret_cast<const float4*>(S.nodes + node);
+    const float4 b0 = n4[0], b1 = n4[1], b2 = n4[2];
+    const int2 cc = *reinterpret_cast<const int2*>(n4 + 3);
+    ++nn;
+    const float lim = prune_bound(h.t);
+    float te0 = 0.0f, te1 = 0.0f;
+    bool h0 = cc.x != kEmptyChild && box_hit(o, d, id, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, te0) && te0 <= lim;
+    bool h1 = cc.y != kEmptyChild && box_hit(o, d, id, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w, te1) && te1 <= lim;
+    const bool sw = h1 && (!h0 || te1 < te0);
+    int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
+    bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
+    const float tf = sw ? te0 : te1;
+    // Leaf children are consumed here, whether tested or pruned, so only
+    // internal nodes (index >= 0) are ever descended into or pushed.
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int32_t c = k == 0 ? cn : cf;
-            const bool take = k == 0 ? hn : hf;
-            if (take && c < 0) {
-                if (k == 0) hn = false; else hf = false;
-                if (k == 1 && !(tf <= prune_bound(h.t))) continue;  // pruned by the near leaf's hit
-                const int32_t enc = ~c;
-                const int32_t start = enc >> 3, count = enc & 7;
-                nt += count;
-                for (int i = 0; i < count; ++i) {
-                    const int32_t prim = start + i;
-                    LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
-                    float t, u, v;
-                    if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
-                        if (ANY) {
-                            h.t = t; h.u = u; h.v = v; h.prim = prim;
-                            return true;
-                        }
-                        if (t < h.t || prim > h.prim) {
-                            h.t = t; h.u = u; h.v = v; h.prim = prim;
-                        }
+    for (int k = 0; k < 2; ++k) {
+        const int32_t c = k == 0 ? cn : cf;
+        const bool take = k == 0 ? hn : hf;
+        if (take && c < 0) {
+            if (k == 0) hn = false; else hf = false;
+            if (k == 1 && !(tf <= prune_bound(h.t))) continue;  // pruned by the near leaf's hit
+            const int32_t enc = ~c;
+            const int32_t start = enc >> 3, count = enc & 7;
+            nt += count;
+            for (int i = 0; i < count; ++i) {
+                const int32_t prim = start + i;
+                LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
+                float t, u, v;
+                if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
+                    if (ANY) {
+                        h.t = t; h.u = u; h.v = v; h.prim = prim;
+                        return false;
+                    }
+                    if (t < h.t || prim > h.prim) {
+                        h.t = t; h.u = u; h.v = v; h.prim = prim;
                     }
                 }
             }
         }
-        if (hn && hf) {
-            if (sp < sd) {
-                stk[sp * kBlock] = cf;
-                ++sp;
-            } else {
-                note_overflow(ovf);
-            }
-            node = cn;
-            continue;
+    }
+    if (hn && hf) {
+        if (sp < sd) {
+            stk[sp * kBlock] = cf;
+            ++sp;
+        } else {
+            note_overflow(ovf);
         }
-        if (hn) { node = cn; continue; }
-        if (hf) { node = cf; continue; }
-        if (sp == 0) break;
-        --sp;
-        node = stk[sp * kBlock];
+        node = cn;
+        return true;
+    }
+    if (hn) { node = cn; return true; }
+    if (hf) { node = cf; return true; }
+    if (sp == 0) return false;
+    --sp;
+    node = stk[sp * kBlock];
+    return true;
+}
+
+template <bool ANY>
+LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, int32_t* stk, int sd, int32_t* ovf,
+                       uint32_t& nn, uint32_t& nt, TraceHit& h, HostBounds* hb = nullptr) {
+    h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+    const v3f id = inv_dir(d);
+    int sp = 0;
+    int32_t node = S.root;
+    while (trav_step<ANY>(S, o, d, id, tmin, stk, sd, ovf, nn, nt, h, node, sp, hb)) {
     }
     return h.prim >= 0;
 }
@@ -422,7 +434,7 @@ __device__ __forceinline__ v3f cam_dir(const CamDev& c, float nx, float ny) {
 
 __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
     int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; }
+    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; }
     if (p >= n_paths) return;
     int b = p / c.P, pix = p - b * c.P;
     int px = pix % c.tw, py = pix / c.tw;
@@ -475,6 +487,113 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int
     if (W.count_work && (nn | nt)) {
         atomicAdd(&W.rays[2], (unsigned long long)nn);
         atomicAdd(&W.rays[3], (unsigned long long)nt);
+    }
+}
+
+// Persistent traversal with ray refill.  A grid of a few workgroups per CU
+// walks the queue; every lane traces one ray at a time, one node per loop
+// iteration, and a wavefront fetches new rays for its idle lanes (one atomic
+// per fetch) once `refill` or more of them are idle.  Lanes of a wavefront
+// thus stay busy while their rays take different numbers of steps, instead
+// of the wavefront waiting for its longest ray.  Fetch counters: cnt[4]
+// (closest hit) and cnt[5] (shadow), zeroed by k_raygen and k_shade.
+// Results per ray are those of closest_real / shadowed (same steps, same
+// order within a ray).
+template <bool SHADOW>
+__global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill) {
+    extern __shared__ int32_t lds_stack[];
+    int32_t* stk = lds_stack + threadIdx.x;
+    const int n = SHADOW ? W.cnt[2] : W.cnt[qin];
+    int32_t* fetch = &W.cnt[SHADOW ? 5 : 4];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (SHADOW) {
+            atomicAdd(&W.rays[1], (unsigned long long)n);
+        } else {
+            W.cnt[1 - qin] = 0;
+            W.cnt[2] = 0;
+            atomicAdd(&W.rays[0], (unsigned long long)n);
+        }
+    }
+    uint32_t nn = 0, nt = 0;
+    int p = -1;  // path of this lane's ray, -1 idle
+    bool exhausted = false;  // wavefront-uniform: the queue has been handed out
+    v3f o = mk(0.0f, 0.0f, 0.0f), d = o, id = o;
+    float tmin = 0.0f;
+    TraceHit h;
+    h.t = 0.0f; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+    int32_t node = 0;
+    int sp = 0, requery = 0;
+    for (;;) {
+        const unsigned long long im = __ballot(p < 0);
+        const int nidle = __popcll(im);
+        if (!exhausted && (nidle >= refill || nidle == 64)) {
+            const int leader = __ffsll((long long)im) - 1;
+            int base = 0;
+            if (lane_id() == leader) base = atomicAdd(fetch, nidle);
+            base = __shfl(base, leader);
+            exhausted = base + nidle >= n;
+            if (p < 0) {
+                const int i = base + __popcll(im & ((1ull << lane_id()) - 1ull));
+                if (i < n) {
+                    if (SHADOW) {
+                        p = W.qs[i];
+                        const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
+                        o = mk(o4); d = mk(d4);
+                        tmin = RAY_T_MIN;
+                        h.t = o4.w;
+                    } else {
+                        p = W.q[qin][i];
+                        const float4 o4 = W.ray_o[p], d4 = W.ray_d[p];
+                        o = mk(o4); d = mk(d4);
+                        tmin = o4.w;
+                        h.t = RAY_T_MAX;
+                    }
+                    id = inv_dir(d);
+                    h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+                    node = S.root;
+                    sp = 0;
+                    requery = 0;
+                }
+            }
+        }
+        if (__ballot(p >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (p < 0) continue;
+        if (trav_step<SHADOW>(S, o, d, id, tmin, stk, sd, &W.cnt[3], nn, nt, h, node, sp, nullptr)) continue;
+        if (SHADOW) {
+            if (h.prim < 0) {  // unoccluded: the pending NEE contribution counts
+                const float4 L = W.rad[p], c = W.sh_c[p];
+                W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+                const int32_t tag = __float_as_int(W.sh_d[p].w);
+                if (tag > 0) {
+                    float4* r = W.dr0 + (size_t)(tag - 1) * (size_t)W.dstride + (size_t)p;
+                    r->w = __int_as_float(__float_as_int(r->w) | kRecValid);
+                }
+            }
+        } else {
+            // intersect_skipping_volume_boundaries: re-query past a
+            // boundary-only mesh; the result of an 8th re-query is discarded
+            // by the reference, so a 7th boundary hit ends with no hit
+            if (S.any_vbo && h.prim >= 0 && (S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) {
+                if (requery < 7) {
+                    ++requery;
+                    tmin = h.t + 1e-3f;
+                    h.t = RAY_T_MAX; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+                    node = S.root;
+                    sp = 0;
+                    continue;
+                }
+                h.prim = -1;
+            }
+            W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+        }
+        p = -1;
+    }
+    if (W.count_work && (nn | nt)) {
+        atomicAdd(&W.rays[SHADOW ? 4 : 2], (unsigned long long)nn);
+        atomicAdd(&W.rays[SHADOW ? 5 : 3], (unsigned long long)nt);
     }
 }
 
@@ -561,7 +680,6 @@ constexpr int kAlgoDiagBase = 16;
 constexpr int kAlgoDiagSimple = kAlgoDiagBase + LUCID_ALGO_SIMPLE;
 constexpr int kAlgoDiagNee = kAlgoDiagBase + LUCID_ALGO_NEE;
 constexpr int kAlgoDiagMis = kAlgoDiagBase + LUCID_ALGO_MIS;
-constexpr int32_t kRecValid = 1;  // bit 0 of a record's packed meta
 
 // classify_bsdf / is_delta_bsdf (diagnostic_integrator.hpp:24-60) on the
 // material parameters after getMaterialParams.
@@ -936,6 +1054,8 @@ template <int ALGO>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::GuideDev G, int qin,
                                                   int32_t max_depth) {
     const int n = W.cnt[qin];
+    // ray fetch counters of the next persistent traversal launches
+    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; }
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
         shade_one<ALGO>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
 }
@@ -1287,10 +1407,14 @@ struct lucid_ctx {
     std::atomic<int> cancel{0};
     std::mutex mu;
     int64_t wave_paths = 0;
-    // Launch shape: workgroups per CU of the persistent grid-stride kernels,
-    // or 0 for one lane per queued ray.  Traversal latency is best hidden by
-    // the hardware dispatcher (0); shading runs best persistent.
-    int32_t traverse_groups = 0, shade_groups = 16;
+    // Launch shape: workgroups per CU of the persistent kernels, or 0 for
+    // one lane per queued ray.  Persistent traversal with ray refill beats
+    // one lane per ray 1.9-2.0x (1080p room, 1M-triangle spheres): a
+    // wavefront no longer waits for its longest ray.  16 groups per CU
+    // saturate the LDS stacks; a fetch once 48 of 64 lanes are idle
+    // amortises the atomic.
+    int32_t traverse_groups = 16, shade_groups = 16;
+    int32_t refill = 48;  // idle lanes that make a wavefront fetch new rays
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
     int64_t film_cap = 0;
@@ -1665,7 +1789,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // workgroups per CU walk the live queues (grid-stride).
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
     const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
-    const int64_t cap_trav = ctx->traverse_groups > 0 ? (int64_t)ctx->num_cu * ctx->traverse_groups : INT64_MAX;
+    // traverse_groups > 0: persistent traversal with ray refill on that many
+    // workgroups per CU; 0: one lane per queued ray
+    const bool refill = ctx->traverse_groups > 0;
+    const int64_t cap_trav = refill ? (int64_t)ctx->num_cu * ctx->traverse_groups : INT64_MAX;
     const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
@@ -1716,7 +1843,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             int ks = kev_begin(ctx, 0, st);
             const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_trav);
             const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
-            k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
+            if (refill)
+                k_trace_refill<false><<<g, kBlock, lds, st>>>(SD, W, q, sd, ctx->refill);
+            else
+                k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
             const bool guide_here = guided && G.alpha > 0.0f && (ctx->gcfg.max_depth <= 0 || depth < ctx->gcfg.max_depth);
@@ -1761,7 +1891,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
-            k_shadow<<<g, kBlock, lds, st>>>(SD, W, sd);
+            if (refill && !SD.any_vbo)  // the refill kernel runs the any-hit query
+                k_trace_refill<true><<<g, kBlock, lds, st>>>(SD, W, 0, sd, ctx->refill);
+            else
+                k_shadow<<<g, kBlock, lds, st>>>(SD, W, sd);
             kev_end(ctx, ks, st);
             q = 1 - q;
         }
@@ -2206,6 +2339,12 @@ int64_t lucid_scene_grid_bytes(const lucid_scene* s) { return s ? (int64_t)s->gr
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;
     ctx->wave_paths = paths;
+    return LUCID_OK;
+}
+
+int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
+    if (!ctx || idle_lanes < 1 || idle_lanes > 64) return LUCID_ERR_INVALID;
+    ctx->refill = idle_lanes;
     return LUCID_OK;
 }
 

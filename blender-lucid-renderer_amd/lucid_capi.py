@@ -111,6 +111,7 @@ SIGNATURES = {
     "lucid_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
     "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "lucid_set_refill_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(DiagConfig)]),
     "lucid_diag_get_info": (C.c_int, [C.c_void_p, C.POINTER(DiagInfo)]),
     "lucid_diag_read_pixels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),

@@ -197,12 +197,19 @@ int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out);
    quarter of free device memory at 172 B per path). */
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
 
-/* Launch shape of the traversal (closest and shadow) and shading kernels:
-   persistent grids of N workgroups per compute unit walk the ray queues
-   (1..64), or 0 for one lane per queued ray.  Defaults: traversal 0,
-   shading 16.  A tuning knob with no counterpart in the reference (its Dawn
-   backend fixes workgroup counts per dispatch). */
+/* Launch shape of the traversal (closest and shadow) and shading kernels.
+   Traversal: N > 0 workgroups per compute unit run persistently, each lane
+   tracing one ray at a time and wavefronts fetching new rays for idle lanes
+   (see lucid_set_refill_threshold); 0 launches one lane per queued ray.
+   Shading: persistent grid-stride over the queue with N workgroups per CU,
+   or 0 for one lane per path.  Defaults: 16 and 16.  A tuning knob with no
+   counterpart in the reference (its Dawn backend fixes workgroup counts per
+   dispatch). */
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
+/* Persistent traversal: a wavefront fetches new rays once this many of its
+   64 lanes are idle (1..64, default 48).  A tuning knob, no counterpart in
+   the reference. */
+int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes);
 
 /* ------------------------------------------------------------------------
    DiagnosticFilm (diagnostics/diagnostic_film.hpp, pixel_path_data.hpp).
