@@ -171,7 +171,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=64)
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--record-stride", type=int, default=2)
+    ap.add_argument("--record-stride", type=int, default=32)
     ap.add_argument("--guide-depth", type=int, default=0, help="guided bounces: depth < N (0: all)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -4,6 +4,8 @@
 // shading bounce.
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "device_math.h"
 #include "guiding.h"
 
@@ -54,6 +56,30 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
     f[63] = dot(n, wo);
 }
 
+// features() written straight into a row of BF16 (no 64-float array in
+// registers): the same values.
+__device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, __bf16* row) {
+    for (int c = 0; c < 3; ++c) {
+        float sn, cs;
+        sincosf(PI_F * pn[c], &sn, &cs);
+#pragma unroll
+        for (int k = 0; k < kOctaves; ++k) {
+            row[3 * (2 * k) + c] = (__bf16)sn;
+            row[3 * (2 * k + 1) + c] = (__bf16)cs;
+            const float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
+            sn = s2;
+            cs = c2;
+        }
+    }
+    row[48] = (__bf16)(2.0f * pn[0] - 1.0f); row[49] = (__bf16)(2.0f * pn[1] - 1.0f);
+    row[50] = (__bf16)(2.0f * pn[2] - 1.0f);
+    row[51] = (__bf16)n.x; row[52] = (__bf16)n.y; row[53] = (__bf16)n.z;
+    row[54] = (__bf16)wo.x; row[55] = (__bf16)wo.y; row[56] = (__bf16)wo.z;
+    row[57] = (__bf16)sinf(PI_F * n.x); row[58] = (__bf16)sinf(PI_F * n.y); row[59] = (__bf16)sinf(PI_F * n.z);
+    row[60] = (__bf16)sinf(PI_F * wo.x); row[61] = (__bf16)sinf(PI_F * wo.y); row[62] = (__bf16)sinf(PI_F * wo.z);
+    row[63] = (__bf16)dot(n, wo);
+}
+
 // The mixture decoded from the 40 raw outputs: lobe k has mean direction
 // normalize(o[5k..5k+2]), concentration exp(clamp(o[5k+3], -2, 7)) and
 // weight softmax(o[5k+4]).
@@ -68,18 +94,24 @@ __device__ __forceinline__ float vmf_logc(float kappa) {
     return logf(kappa) - 1.8378770664093453f - logf(-expm1f(-2.0f * kappa));
 }
 
-__device__ __forceinline__ void decode(const float* o, Mixture& m) {
+__device__ __forceinline__ float out_f32(float x) { return x; }
+__device__ __forceinline__ float out_f32(__half x) { return __half2float(x); }
+
+// Raw outputs as FP32 (training) or FP16 (what the render-side inference
+// stores per path: half the bytes; sampling and pdf read the same values).
+template <class T>
+__device__ __forceinline__ void decode(const T* o, Mixture& m) {
     float mx = -1e30f;
-    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, o[5 * k + 4]);
+    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, out_f32(o[5 * k + 4]));
     float sum = 0.0f;
     for (int k = 0; k < kLobes; ++k) {
-        const v3f r = mk(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+        const v3f r = mk(out_f32(o[5 * k]), out_f32(o[5 * k + 1]), out_f32(o[5 * k + 2]));
         const float len = sqrtf(dot(r, r)) + 1e-6f;
         m.mu[k] = r * (1.0f / len);
-        const float lk = fminf(fmaxf(o[5 * k + 3], kLogKappaMin), kLogKappaMax);
+        const float lk = fminf(fmaxf(out_f32(o[5 * k + 3]), kLogKappaMin), kLogKappaMax);
         m.kappa[k] = expf(lk);
         m.logc[k] = vmf_logc(m.kappa[k]);
-        m.w[k] = expf(o[5 * k + 4] - mx);
+        m.w[k] = expf(out_f32(o[5 * k + 4]) - mx);
         sum += m.w[k];
     }
     const float inv = 1.0f / sum;
@@ -95,14 +127,22 @@ __device__ __forceinline__ float mixture_pdf(const Mixture& m, v3f w) {
 // Samples a lobe with u0, then the lobe with (u1, u2) (the inversion
 // w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa of the vMF cosine).
 __device__ __forceinline__ v3f mixture_sample(const Mixture& m, float u0, float u1, float u2) {
-    int k = 0;
-    float c = m.w[0];
-    while (k < kLobes - 1 && u0 >= c) c += m.w[++k];
-    const float ka = m.kappa[k];
+    // the first lobe whose running weight sum exceeds u0 (the last one if
+    // none does), selected without a dynamic register index
+    float c = 0.0f, ka = m.kappa[kLobes - 1];
+    v3f mu = m.mu[kLobes - 1];
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < kLobes - 1; ++i) {
+        c += m.w[i];
+        const bool take = !found && u0 < c;
+        ka = take ? m.kappa[i] : ka;
+        mu = take ? m.mu[i] : mu;
+        found = found || take;
+    }
     const float wz = fminf(fmaxf(1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * ka)) / ka, -1.0f), 1.0f);
     const float r = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
     const float phi = 2.0f * PI_F * u2;
-    const v3f mu = m.mu[k];
     // orthonormal basis around mu (Duff et al. 2017, "Building an orthonormal basis, revisited")
     const float s = copysignf(1.0f, mu.z);
     const float a = -1.0f / (s + mu.z), b = mu.x * mu.y * a;

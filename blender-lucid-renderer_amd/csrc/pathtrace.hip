@@ -97,11 +97,29 @@ struct WaveDev {
     // guided runs: network outputs (40 per path) and, for 1 path in
     // `tstride`, training slots of kTrainSlots bounces x 5 float4:
     // (p, T_in.x) (n, T_in.y) (wo, T_in.z) (wi, valid) (L_after, valid)
-    float* gout;
+    __half* gout;
     float4* tslot;
     int32_t tstride;
+    int64_t tnrec;  // records (paths with slots); slot arrays are [k * 5 + field][record]
 };
 constexpr int kTrainSlots = 6;
+// Paths that keep training slots: one per block of `tstride` consecutive
+// paths, at a hashed offset so that every pixel column contributes.
+__device__ __forceinline__ int train_record_path(const WaveDev& W, int r) {
+    uint32_t h = (uint32_t)r * 0x9e3779b1u;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    return r * W.tstride + (int)(h % (uint32_t)W.tstride);
+}
+__device__ __forceinline__ bool keeps_record(const WaveDev& W, int p) {
+    return W.tslot && train_record_path(W, p / W.tstride) == p;
+}
+// Field f (0..4) of training slot k of record r: structure of arrays, so
+// consecutive records are consecutive float4s.
+__device__ __forceinline__ float4* tslot_at(const WaveDev& W, int r, int k, int f) {
+    return W.tslot + (size_t)(k * 5 + f) * (size_t)W.tnrec + (size_t)r;
+}
 constexpr int32_t kRecValid = 1;  // bit 0 of a diagnostic record's packed meta
 
 struct CamDev {
@@ -127,6 +145,19 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
     base = __shfl(base, leader);
     int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
     return base + rank;
+}
+
+// Same for a 64-bit running total whose positions wrap around a ring of
+// `capacity` entries: returns this lane's ring index.
+__device__ __forceinline__ int wave_append_u64(unsigned long long* total, bool pred, int capacity) {
+    unsigned long long mask = __ballot(pred);
+    int n = __popcll(mask);
+    unsigned long long base = 0;
+    int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+    if (n > 0 && lane_id() == leader) base = atomicAdd(total, (unsigned long long)n);
+    base = __shfl(base, leader);
+    int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
+    return (int)((base + (unsigned long long)rank) % (unsigned long long)capacity);
 }
 
 LUCID_HD v3f inv_dir(v3f d) {  // intersection.hpp:412-419
@@ -459,11 +490,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
     W.meta[p] = make_int4(pix, s, 0, 0);
     W.q[0][p] = p;
     if (W.dcnt) W.dcnt[p] = 0;
-    if (W.tslot && p % W.tstride == 0) {
-        float4* ts = W.tslot + (size_t)(p / W.tstride) * (kTrainSlots * 5);
+    if (keeps_record(W, p)) {
+        const int r = p / W.tstride;
         for (int k = 0; k < kTrainSlots; ++k) {
-            ts[k * 5 + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ts[k * 5 + 4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            *tslot_at(W, r, k, 3) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            *tslot_at(W, r, k, 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
 }
@@ -606,7 +637,7 @@ struct NativeHit {
 };
 // intersectNativeLights (scene_lights.hpp:176-246); only area and sphere
 // lights are visible, and only for depth > 0.
-__device__ __noinline__ NativeHit native_lights(const SceneDev& S, v3f o, v3f d) {
+__device__ __forceinline__ NativeHit native_lights(const SceneDev& S, v3f o, v3f d) {
     NativeHit r;
     r.hit = false; r.t = 1e30f; r.index = -1;
     for (int i = 0; i < S.n_native; ++i) {
@@ -743,10 +774,11 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
         v3f L = mk(Lv);
         // guided runs: the radiance gathered up to here closes the previous
         // bounce's training slot (everything after it came through wi)
-        float4* ts = nullptr;
-        if (GUIDED && W.tslot && p % W.tstride == 0) {
-            ts = W.tslot + (size_t)(p / W.tstride) * (kTrainSlots * 5);
-            if (depth >= 1 && depth - 1 < kTrainSlots) ts[(depth - 1) * 5 + 4] = make_float4(L.x, L.y, L.z, 1.0f);
+        int trec = -1;  // training record of this path, if it keeps one
+        if (GUIDED && keeps_record(W, p)) {
+            trec = p / W.tstride;
+            if (depth >= 1 && depth - 1 < kTrainSlots)
+                *tslot_at(W, trec, depth - 1, 4) = make_float4(L.x, L.y, L.z, 1.0f);
         }
         // diagnostic path: vertex count so far; this bounce's record slot
         // starts empty (it may hold a record of an earlier batch)
@@ -1008,16 +1040,15 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 else w = bsdf_weight(bs.fw, cs, bs.pdf);
             }
             // training slot of this bounce: where, how much arrived, which way it left
-            if (GUIDED && ts && depth < kTrainSlots) {
+            if (GUIDED && trec >= 0 && depth < kTrainSlots) {
                 const bool rec = ok && guidable && (w.x > 0.0f || w.y > 0.0f || w.z > 0.0f);
-                float4* t = ts + depth * 5;
                 if (rec) {
-                    t[0] = make_float4(point.x, point.y, point.z, thr_in.x);
-                    t[1] = make_float4(nrm.x, nrm.y, nrm.z, thr_in.y);
-                    t[2] = make_float4(wo.x, wo.y, wo.z, thr_in.z);
+                    *tslot_at(W, trec, depth, 0) = make_float4(point.x, point.y, point.z, thr_in.x);
+                    *tslot_at(W, trec, depth, 1) = make_float4(nrm.x, nrm.y, nrm.z, thr_in.y);
+                    *tslot_at(W, trec, depth, 2) = make_float4(wo.x, wo.y, wo.z, thr_in.z);
                 }
-                t[3] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, rec ? 1.0f : 0.0f);
-                t[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                *tslot_at(W, trec, depth, 3) = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, rec ? 1.0f : 0.0f);
+                *tslot_at(W, trec, depth, 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
             if (ok) {
                 thr = cmul(thr, w);
@@ -1078,6 +1109,7 @@ __device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float
 // Each lane encodes one path's 64 input features into LDS; the MFMA operands
 // are then gathered from LDS as 4 tiles of 16 that share every weight load.
 // Paths without a surface hit run on zero input and are not written.
+constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 __global__ __launch_bounds__(64) void k_guide_infer(SceneDev S, WaveDev W, guide::GuideDev G, int qin) {
     __shared__ __bf16 feat[64][guide::kIn + 2];
     __shared__ int32_t path[64];
@@ -1103,34 +1135,43 @@ __global__ __launch_bounds__(64) void k_guide_infer(SceneDev S, WaveDev W, guide
                     p = pp;
                 }
             }
-            float f[guide::kIn];
-            guide::features(pn, nrm, wo, f);
-            for (int i = 0; i < guide::kIn; ++i) feat[lane][i] = (__bf16)f[i];
+            guide::features_bf16(pn, nrm, wo, &feat[lane][0]);
             path[lane] = p;
         }
         __syncthreads();
-        guide::bf16x8 x0[4], x1[4];
+        // the 4 tiles of 16 samples in passes of kInferTiles tiles (fewer
+        // live accumulators per pass, more wavefronts per SIMD)
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-            for (int i = 0; i < 8; ++i) {
-                x0[t][i] = feat[16 * t + s][guide::perm(0, g, i)];
-                x1[t][i] = feat[16 * t + s][guide::perm(1, g, i)];
-            }
-        int pt[4];
-        for (int t = 0; t < 4; ++t) pt[t] = path[16 * t + s];
-        __syncthreads();
-        guide::f32x4 o[4][3];
-        guide::mlp_forward_tiles<4>(G, x0, x1, o);
+        for (int t0 = 0; t0 < 4; t0 += kInferTiles) {
+            guide::bf16x8 x0[kInferTiles], x1[kInferTiles];
+            int pt[kInferTiles];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            if (pt[t] < 0) continue;
-            float* out = W.gout + (size_t)pt[t] * guide::kOut;
-            for (int m = 0; m < 3; ++m)
-                for (int j = 0; j < 4; ++j) {
-                    const int f = 16 * m + 4 * g + j;
-                    if (f < guide::kOut) out[f] = o[t][m][j];
+            for (int t = 0; t < kInferTiles; ++t) {
+                for (int i = 0; i < 8; ++i) {
+                    x0[t][i] = feat[16 * (t0 + t) + s][guide::perm(0, g, i)];
+                    x1[t][i] = feat[16 * (t0 + t) + s][guide::perm(1, g, i)];
                 }
+                pt[t] = path[16 * (t0 + t) + s];
+            }
+            guide::f32x4 o[kInferTiles][3];
+            guide::mlp_forward_tiles<kInferTiles>(G, x0, x1, o);
+#pragma unroll
+            for (int t = 0; t < kInferTiles; ++t) {
+                if (pt[t] < 0) continue;
+                __half* out = W.gout + (size_t)pt[t] * guide::kOut;
+                for (int m = 0; m < 3; ++m) {
+                    const int f = 16 * m + 4 * g;  // 4 outputs per lane and tile, 8-byte aligned
+                    if (f >= guide::kOut) continue;
+                    __half2 lo = __floats2half2_rn(o[t][m][0], o[t][m][1]);
+                    __half2 hi = __floats2half2_rn(o[t][m][2], o[t][m][3]);
+                    uint2 v;
+                    __builtin_memcpy(&v.x, &lo, 4);
+                    __builtin_memcpy(&v.y, &hi, 4);
+                    *reinterpret_cast<uint2*>(out + f) = v;
+                }
+            }
         }
+        __syncthreads();  // the next iteration overwrites feat / path
     }
 }
 
@@ -1140,20 +1181,27 @@ __global__ __launch_bounds__(64) void k_guide_infer(SceneDev S, WaveDev W, guide
 __global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_paths, guide::Sample* ring,
                                                        unsigned long long* total, int capacity) {
     const int r = blockIdx.x * 256 + threadIdx.x;
-    const int p = r * W.tstride;
-    if (p >= n_paths) return;
-    const float4* ts = W.tslot + (size_t)r * (kTrainSlots * 5);
-    const float4 le = W.rad[p];
+    const int p = train_record_path(W, r);
+    const bool live = p < n_paths;
+    const float4 le = live ? W.rad[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int k = 0; k < kTrainSlots; ++k) {
-        const float4 a = ts[k * 5], b = ts[k * 5 + 1], c = ts[k * 5 + 2], d = ts[k * 5 + 3], e = ts[k * 5 + 4];
-        if (d.w != 1.0f || e.w != 1.0f) continue;
-        const float tr = a.w, tg = b.w, tb = c.w;
-        const float cr = tr > 0.0f ? (le.x - e.x) / tr : 0.0f;
-        const float cg = tg > 0.0f ? (le.y - e.y) / tg : 0.0f;
-        const float cb = tb > 0.0f ? (le.z - e.z) / tb : 0.0f;
-        const float wgt = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
-        if (!(wgt > 0.0f) || !isfinite(wgt)) continue;
-        const unsigned long long slot = atomicAdd(total, 1ull);
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c = a, d = a, e = a;
+        if (live) {
+            a = *tslot_at(W, r, k, 0); b = *tslot_at(W, r, k, 1); c = *tslot_at(W, r, k, 2);
+            d = *tslot_at(W, r, k, 3); e = *tslot_at(W, r, k, 4);
+        }
+        float wgt = 0.0f;
+        if (d.w == 1.0f && e.w == 1.0f) {
+            const float tr = a.w, tg = b.w, tb = c.w;
+            const float cr = tr > 0.0f ? (le.x - e.x) / tr : 0.0f;
+            const float cg = tg > 0.0f ? (le.y - e.y) / tg : 0.0f;
+            const float cb = tb > 0.0f ? (le.z - e.z) / tb : 0.0f;
+            wgt = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
+        }
+        const bool keep = wgt > 0.0f && isfinite(wgt);
+        // ring slots: one atomic per wavefront and bounce slot
+        const int slot = wave_append_u64(total, keep, capacity);
+        if (!keep) continue;
         guide::Sample sm;
         sm.p[0] = a.x; sm.p[1] = a.y; sm.p[2] = a.z;
         sm.n[0] = b.x; sm.n[1] = b.y; sm.n[2] = b.z;
@@ -1161,7 +1209,7 @@ __global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_path
         sm.wi[0] = d.x; sm.wi[1] = d.y; sm.wi[2] = d.z;
         sm.weight = wgt;
         sm.pad[0] = sm.pad[1] = sm.pad[2] = 0.0f;
-        ring[slot % (unsigned long long)capacity] = sm;
+        ring[slot] = sm;
     }
 }
 
@@ -1412,8 +1460,9 @@ struct lucid_ctx {
     // one lane per ray 1.9-2.0x (1080p room, 1M-triangle spheres): a
     // wavefront no longer waits for its longest ray.  16 groups per CU
     // saturate the LDS stacks; a fetch once 48 of 64 lanes are idle
-    // amortises the atomic.
-    int32_t traverse_groups = 16, shade_groups = 16;
+    // amortises the atomic.  Shading: 8 groups per CU = what its register
+    // use lets run at once.
+    int32_t traverse_groups = 16, shade_groups = 8;
     int32_t refill = 48;  // idle lanes that make a wavefront fetch new rays
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
@@ -1424,9 +1473,9 @@ struct lucid_ctx {
     int32_t profiling = 0;
     // learned path guiding
     guide::State* gstate = nullptr;
-    lucid_guiding_config gcfg{0.7f, 64, 32768, 3e-3f, 2, 1u, 0};
+    lucid_guiding_config gcfg{0.7f, 64, 32768, 3e-3f, 32, 1u, 0};
     int64_t gsteps = 0;
-    float* gout = nullptr;
+    __half* gout = nullptr;  // FP16 network outputs, 40 per path
     int64_t gout_cap = 0;
     float4* tslot = nullptr;
     int64_t tslot_cap = 0;
@@ -1768,6 +1817,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     W.gout = nullptr;
     W.tslot = nullptr;
     W.tstride = 1;
+    W.tnrec = 0;
     const bool train = guided && ctx->gcfg.train_steps > 0;
     if (guided) {
         rc = ensure_guide(ctx);
@@ -1780,6 +1830,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             const int64_t nrec = (P * B + W.tstride - 1) / W.tstride;
             rc = ensure_buf(ctx, &ctx->tslot, &ctx->tslot_cap, nrec * kTrainSlots * 5);
             if (rc) return rc;
+            W.tnrec = nrec;
             W.tslot = ctx->tslot;
         }
         // the guide samples only once it has been trained
@@ -1898,10 +1949,13 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             kev_end(ctx, ks, st);
             q = 1 - q;
         }
-        if (train)
+        if (train) {  // timed with the training
+            const int ks = kev_begin(ctx, 4, st);
             k_guide_collect<<<blocks((n + W.tstride - 1) / W.tstride, 256), 256, 0, st>>>(
                 W, (int32_t)n, guide::ring(ctx->gstate), guide::ring_total(ctx->gstate),
                 guide::ring_capacity(ctx->gstate));
+            kev_end(ctx, ks, st);
+        }
         if (mode == 0 || mode == 3) {
             k_film<<<blocks(P, 256), 256, 0, st>>>(W, film, (int32_t)P, b);
             if (diag && dsw > 0 && dsh > 0)

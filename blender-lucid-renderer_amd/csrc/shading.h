@@ -172,7 +172,7 @@ struct BSample {
 };
 // sampleBSDF, bsdf/bsdf.hpp:198-345 (u1..u3 drawn left to right by the caller;
 // the glass branch draws one more number, bsdf.hpp:221)
-__device__ __noinline__ BSample sample_bsdf(const Mat& m, v3f wo, v3f n, float u1, float u2, float u3, Pcg& rng) {
+__device__ __forceinline__ BSample sample_bsdf(const Mat& m, v3f wo, v3f n, float u1, float u2, float u3, Pcg& rng) {
     BSample s;
     s.wi = mk(0.0f, 0.0f, 1.0f);
     s.fw = mk(0.0f, 0.0f, 0.0f);
@@ -250,7 +250,7 @@ struct LSample {
     v3f pos, dir, emission;
     float pdf, dist;
 };
-__device__ __noinline__ LSample sample_light(const GLight& l, v3f p, float u1, float u2) {
+__device__ __forceinline__ LSample sample_light(const GLight& l, v3f p, float u1, float u2) {
     LSample s;
     s.emission = mk(l.emission[0], l.emission[1], l.emission[2]);
     s.pdf = 0.0f;

@@ -202,7 +202,7 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
    tracing one ray at a time and wavefronts fetching new rays for idle lanes
    (see lucid_set_refill_threshold); 0 launches one lane per queued ray.
    Shading: persistent grid-stride over the queue with N workgroups per CU,
-   or 0 for one lane per path.  Defaults: 16 and 16.  A tuning knob with no
+   or 0 for one lane per path.  Defaults: 16 and 8.  A tuning knob with no
    counterpart in the reference (its Dawn backend fixes workgroup counts per
    dispatch). */
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
@@ -275,7 +275,8 @@ typedef struct lucid_guiding_config {
     int32_t train_steps;   /* Adam steps after each guided render (0: frozen) */
     int32_t batch;         /* training batch, <= 65536 */
     float learning_rate;
-    int32_t record_stride; /* 1 path in N records training samples, >= 1 */
+    int32_t record_stride; /* 1 path in N (hashed within each block of N)
+                              records training samples, >= 1; default 32 */
     uint32_t seed;         /* network initialisation */
     int32_t max_depth;     /* bounces guided: depth < max_depth (0: all) */
 } lucid_guiding_config;
