@@ -276,7 +276,7 @@ def main():
                                f"{algo}, {lights}",
                    "rays_per_step": rays // max(args.steps, 1), "paths_per_step": W * H * spp,
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
-        "roofline": {"bound": "hbm", "kernel": "k_intersect (closest-hit BVH traversal)",
+        "roofline": {"bound": "hbm", "kernel": "k_trace_refill<false> (closest-hit BVH traversal, persistent with ray refill)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/k_intersect_occluded_room_1080p_16spp.md (rocprofv3 PMC)"

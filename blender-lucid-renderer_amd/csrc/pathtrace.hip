@@ -44,6 +44,7 @@ namespace lucid {
 
 constexpr int kBlock = 128;      // 2 wavefronts per workgroup
 constexpr int kStackMax = 64;    // LDS traversal stack entries per lane, upper bound
+constexpr int kMinRaysPerLane = 8;  // persistent traversal: grid shrinks for small waves
 constexpr int64_t kWaveBytesPerPath = 10 * 16 + 3 * 4;  // WaveDev SoA bytes per path slot
 
 // ------------------------------------------------------------- device scene --
@@ -1843,7 +1844,13 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // traverse_groups > 0: persistent traversal with ray refill on that many
     // workgroups per CU; 0: one lane per queued ray
     const bool refill = ctx->traverse_groups > 0;
-    const int64_t cap_trav = refill ? (int64_t)ctx->num_cu * ctx->traverse_groups : INT64_MAX;
+    // Batched session passes run up to four lanes at once: each gets half
+    // the groups (measured best on 32 sessions).  Small waves keep at least
+    // kMinRaysPerLane rays per lane, so no wavefront is left near-idle.
+    const int tgroups = active_lanes > 1 ? std::max(4, ctx->traverse_groups / 2) : ctx->traverse_groups;
+    const int64_t cap_trav = refill ? std::min<int64_t>((int64_t)ctx->num_cu * tgroups,
+                                                        blocks(P * B, kBlock * kMinRaysPerLane))
+                                    : INT64_MAX;
     const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
