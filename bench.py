@@ -55,9 +55,9 @@ for _p in (PKG, ORACLE):
 METRIC = "Mrays/s + time-to-relMSE 0.01 (guided vs plain), 1080p, 1×MI355X"
 HBM_PEAK_GBS = 8000.0
 REFERENCE = os.path.join(ROOT, "tests", "golden", "occluded_room_reference.npz")
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of k_intersect on the default workload,
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of the closest-hit traversal on the default workload,
 # scaled by a 1 GiB device copy (profiles/ holds the commands and the table)
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_intersect_occluded_room_1080p_16spp.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_trace_refill_occluded_room_1080p_16spp.md")
 TARGET = 0.01
 
 
@@ -279,7 +279,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_trace_refill<false> (closest-hit BVH traversal, persistent with ray refill)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": "profiles/k_intersect_occluded_room_1080p_16spp.md (rocprofv3 PMC)"
+                     "traffic_source": "profiles/k_trace_refill_occluded_room_1080p_16spp.md (rocprofv3 PMC)"
                      if traffic else None,
                      "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 4),
                      "launches": n_launch,
