@@ -50,9 +50,7 @@ constexpr int64_t kWaveBytesPerPath = 10 * 16 + 3 * 4;  // WaveDev SoA bytes per
 // ------------------------------------------------------------- device scene --
 struct SceneDev {
     const GNode* nodes;
-    const float4* v0;
-    const float4* e1;
-    const float4* e2;
+    const float4* tri;      // per triangle: v0, e1, e2 (the Moller-Trumbore operands), 48 B
     const float4* fn;
     const float4* vn;       // 3 per triangle, or null
     const GMat* mats;
@@ -293,7 +291,8 @@ LUCID_HD bool trav_step(const SceneDev& S, v3f o, v3f d, v3f id, float tmin, int
                 const int32_t prim = start + i;
                 LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
                 float t, u, v;
-                if (tri_test(o, d, S.v0[prim], S.e1[prim], S.e2[prim], tmin, h.t, t, u, v)) {
+                const float4* tp = S.tri + 3 * (size_t)prim;
+                if (tri_test(o, d, tp[0], tp[1], tp[2], tmin, h.t, t, u, v)) {
                     if (ANY) {
                         h.t = t; h.u = u; h.v = v; h.prim = prim;
                         return false;
@@ -335,7 +334,9 @@ LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, 
     return h.prim >= 0;
 }
 
-__device__ __forceinline__ int32_t prim_mesh(const SceneDev& S, int32_t prim) { return __float_as_int(S.v0[prim].w); }
+__device__ __forceinline__ int32_t prim_mesh(const SceneDev& S, int32_t prim) {
+    return __float_as_int(S.tri[3 * (size_t)prim].w);
+}
 
 // intersect_skipping_volume_boundaries (path_tracer.hpp:82-95): the closest
 // real surface; boundary-only meshes are skipped by re-querying past them.
@@ -891,7 +892,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
         }
         if (!done) {
             float t = hv.x, u = hv.z, v = hv.w;
-            float4 v0 = S.v0[prim];
+            float4 v0 = S.tri[3 * (size_t)prim];
             int32_t mesh = __float_as_int(v0.w);
             GMat gm = S.mats[mesh];
             Mat mat;
@@ -940,7 +941,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             } else if (has_em) {  // traceMIS / traceVolumeMIS
                 float mw = 1.0f;
                 if (S.n_lights > 0 && last_pdf > MIN_PDF) {
-                    int32_t li = __float_as_int(S.e2[prim].w);
+                    int32_t li = __float_as_int(S.tri[3 * (size_t)prim + 2].w);
                     if (li >= 0) {
                         float sp = area_ratio(S.lights[li].area, S.total_area);
                         if (sp > 0.0f) {
@@ -1304,8 +1305,8 @@ __global__ __launch_bounds__(256) void k_hits_out(SceneDev S, WaveDev W, int32_t
     float4 h = W.hit[p];
     int32_t prim = __float_as_int(h.y);
     if (prim >= 0) {
-        mesh[k] = __float_as_int(S.v0[prim].w);
-        tri[k] = __float_as_int(S.e1[prim].w);
+        mesh[k] = __float_as_int(S.tri[3 * (size_t)prim].w);
+        tri[k] = __float_as_int(S.tri[3 * (size_t)prim + 1].w);
         u[k] = h.z; v[k] = h.w; t[k] = h.x;
     } else {
         mesh[k] = -1; tri[k] = -1; u[k] = 0.0f; v[k] = 0.0f; t[k] = 0.0f;
@@ -1550,6 +1551,17 @@ int upload(lucid_ctx* ctx, lucid_scene* s, const std::vector<T>& v, const T** ou
     HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     *out = static_cast<const T*>(p);
     return LUCID_OK;
+}
+
+// v0, e1, e2 of every triangle as 12 consecutive floats.
+std::vector<float> interleave_triangles(const PackedScene& ps) {
+    std::vector<float> t((size_t)ps.num_triangles * 12);
+    for (size_t i = 0; i < (size_t)ps.num_triangles; ++i) {
+        std::memcpy(&t[12 * i], &ps.tri_v0[4 * i], 16);
+        std::memcpy(&t[12 * i + 4], &ps.tri_e1[4 * i], 16);
+        std::memcpy(&t[12 * i + 8], &ps.tri_e2[4 * i], 16);
+    }
+    return t;
 }
 
 void free_wave(Lane& L) {
@@ -2132,9 +2144,9 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
         return rc;
     };
     int rc = upload(ctx, s, ps.nodes, &D.nodes);
-    if (!rc) rc = up4(ps.tri_v0, &D.v0);
-    if (!rc) rc = up4(ps.tri_e1, &D.e1);
-    if (!rc) rc = up4(ps.tri_e2, &D.e2);
+    // triangles interleaved: a leaf's triangles are one contiguous block
+    std::vector<float> tri3 = interleave_triangles(ps);
+    if (!rc) rc = up4(tri3, &D.tri);
     if (!rc) rc = up4(ps.tri_fn, &D.fn);
     if (!rc) rc = up4(ps.tri_vn, &D.vn);
     if (!rc) rc = upload(ctx, s, ps.mats, &D.mats);
@@ -2355,9 +2367,8 @@ int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, in
     SceneDev S;
     std::memset(&S, 0, sizeof(S));
     S.nodes = ps.nodes.data();
-    S.v0 = reinterpret_cast<const float4*>(ps.tri_v0.data());
-    S.e1 = reinterpret_cast<const float4*>(ps.tri_e1.data());
-    S.e2 = reinterpret_cast<const float4*>(ps.tri_e2.data());
+    const std::vector<float> tri3 = interleave_triangles(ps);
+    S.tri = reinterpret_cast<const float4*>(tri3.data());
     S.root = ps.root;
     const int sd = std::min(kStackMax, ps.max_depth + 2);
     std::vector<int32_t> stack((size_t)sd * kBlock);
