@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
     ap.add_argument("--alpha", type=float, default=0.7)
-    ap.add_argument("--train-steps", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=32)
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--record-stride", type=int, default=32)

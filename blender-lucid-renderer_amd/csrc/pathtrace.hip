@@ -1475,7 +1475,7 @@ struct lucid_ctx {
     int32_t profiling = 0;
     // learned path guiding
     guide::State* gstate = nullptr;
-    lucid_guiding_config gcfg{0.7f, 64, 32768, 3e-3f, 32, 1u, 0};
+    lucid_guiding_config gcfg{0.7f, 32, 32768, 3e-3f, 32, 1u, 0};
     int64_t gsteps = 0;
     __half* gout = nullptr;  // FP16 network outputs, 40 per path
     int64_t gout_cap = 0;

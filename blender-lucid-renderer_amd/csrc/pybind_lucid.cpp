@@ -953,7 +953,7 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
-        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 64,
+        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 32,
              py::arg("batch") = 32768, py::arg("learning_rate") = 3e-3f, py::arg("record_stride") = 32,
              py::arg("seed") = 1u, py::arg("max_depth") = 0,
              "Extension: configure learned path guiding (algorithm 'guided') and start from a fresh network")
