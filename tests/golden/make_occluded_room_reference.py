@@ -56,8 +56,10 @@ def main():
         r = sm / spp
         var = np.maximum(sq / spp - r * r, 0.0) * spp / max(spp - 1, 1)
         self_relmse = float(np.mean(var / spp / (r * r + 0.01)))
-        np.savez(OUT, xs=xs, ys=ys, sum=sm, sumsq=sq, spp=spp, offset0=OFFSET0, algorithm="mis", max_depth=depth,
+        tmp = OUT[:-4] + ".partial.npz"  # written whole, then renamed over the fixture
+        np.savez(tmp, xs=xs, ys=ys, sum=sm, sumsq=sq, spp=spp, offset0=OFFSET0, algorithm="mis", max_depth=depth,
                  width=W, height=H, self_relmse=self_relmse)
+        os.replace(tmp, OUT)
         print(f"spp {spp}: round {time.time() - t0:.1f} s, reference relMSE {self_relmse:.5f}", flush=True)
 
 
