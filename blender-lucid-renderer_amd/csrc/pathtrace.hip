@@ -530,10 +530,24 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int
 // thus stay busy while their rays take different numbers of steps, instead
 // of the wavefront waiting for its longest ray.  Fetch counters: cnt[4]
 // (closest hit) and cnt[5] (shadow), zeroed by k_raygen and k_shade.
-// Results per ray are those of closest_real / shadowed (same steps, same
-// order within a ray).
-template <bool SHADOW>
+// MODE 0: closest hit with volume-boundary re-queries (closest_real);
+// MODE 1: shadow any-hit; MODE 2: shadow past boundary-only meshes
+// (shadowed with re-queries from a moving origin).  Per ray the steps and
+// their order are those of closest_real / shadowed.
+__device__ __forceinline__ void shadow_unoccluded(const WaveDev& W, int p) {
+    const float4 L = W.rad[p], c = W.sh_c[p];
+    W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+    // diagnostic runs: the pending NEE record of this bounce is kept
+    const int32_t tag = __float_as_int(W.sh_d[p].w);
+    if (tag > 0) {
+        float4* r = W.dr0 + (size_t)(tag - 1) * (size_t)W.dstride + (size_t)p;
+        r->w = __int_as_float(__float_as_int(r->w) | kRecValid);
+    }
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill) {
+    constexpr bool SHADOW = MODE != 0, ANY = MODE == 1;
     extern __shared__ int32_t lds_stack[];
     int32_t* stk = lds_stack + threadIdx.x;
     const int n = SHADOW ? W.cnt[2] : W.cnt[qin];
@@ -551,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
     int p = -1;  // path of this lane's ray, -1 idle
     bool exhausted = false;  // wavefront-uniform: the queue has been handed out
     v3f o = mk(0.0f, 0.0f, 0.0f), d = o, id = o;
-    float tmin = 0.0f;
+    float tmin = 0.0f, tmax = 0.0f, toff = 0.0f;  // MODE 2: shadow distance, offset of the moving origin
     TraceHit h;
     h.t = 0.0f; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
     int32_t node = 0;
@@ -573,6 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                         const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
                         o = mk(o4); d = mk(d4);
                         tmin = RAY_T_MIN;
+                        tmax = o4.w;
+                        toff = 0.0f;
                         h.t = o4.w;
                     } else {
                         p = W.q[qin][i];
@@ -586,6 +602,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                     node = S.root;
                     sp = 0;
                     requery = 0;
+                    if (MODE == 2 && !(tmax > 0.0f)) {  // nothing left to test: unoccluded
+                        shadow_unoccluded(W, p);
+                        p = -1;
+                    }
                 }
             }
         }
@@ -594,18 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             continue;
         }
         if (p < 0) continue;
-        if (trav_step<SHADOW>(S, o, d, id, tmin, stk, sd, &W.cnt[3], nn, nt, h, node, sp, nullptr)) continue;
-        if (SHADOW) {
-            if (h.prim < 0) {  // unoccluded: the pending NEE contribution counts
-                const float4 L = W.rad[p], c = W.sh_c[p];
-                W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
-                const int32_t tag = __float_as_int(W.sh_d[p].w);
-                if (tag > 0) {
-                    float4* r = W.dr0 + (size_t)(tag - 1) * (size_t)W.dstride + (size_t)p;
-                    r->w = __int_as_float(__float_as_int(r->w) | kRecValid);
-                }
-            }
-        } else {
+        if (trav_step<ANY>(S, o, d, id, tmin, stk, sd, &W.cnt[3], nn, nt, h, node, sp, nullptr)) continue;
+        if (MODE == 0) {
             // intersect_skipping_volume_boundaries: re-query past a
             // boundary-only mesh; the result of an 8th re-query is discarded
             // by the reference, so a 7th boundary hit ends with no hit
@@ -621,6 +631,28 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                 h.prim = -1;
             }
             W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
+        } else if (MODE == 1) {
+            if (h.prim < 0) shadow_unoccluded(W, p);
+        } else {
+            // is_shadowed_skipping_volume_boundaries: a real surface within
+            // the remaining distance occludes; a boundary-only one moves the
+            // origin 1e-3 past its hit, at most 8 queries in all
+            bool unoccluded = h.prim < 0;
+            if (!unoccluded && (S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) {
+                const float skip = h.t + 1e-3f;
+                toff += skip;
+                o = o + d * skip;
+                ++requery;
+                const float remaining = tmax - toff;
+                if (requery < 8 && remaining > 0.0f) {
+                    h.t = remaining; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
+                    node = S.root;
+                    sp = 0;
+                    continue;
+                }
+                unoccluded = true;
+            }
+            if (unoccluded) shadow_unoccluded(W, p);
         }
         p = -1;
     }
@@ -1914,7 +1946,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_trav);
             const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
             if (refill)
-                k_trace_refill<false><<<g, kBlock, lds, st>>>(SD, W, q, sd, ctx->refill);
+                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, q, sd, ctx->refill);
             else
                 k_intersect<<<g, kBlock, lds, st>>>(SD, W, q, sd);
             kev_end(ctx, ks, st);
@@ -1961,8 +1993,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
-            if (refill && !SD.any_vbo)  // the refill kernel runs the any-hit query
-                k_trace_refill<true><<<g, kBlock, lds, st>>>(SD, W, 0, sd, ctx->refill);
+            if (refill && SD.any_vbo)
+                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, W, 0, sd, ctx->refill);
+            else if (refill)
+                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, W, 0, sd, ctx->refill);
             else
                 k_shadow<<<g, kBlock, lds, st>>>(SD, W, sd);
             kev_end(ctx, ks, st);

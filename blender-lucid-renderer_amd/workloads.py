@@ -399,6 +399,63 @@ def batched_sessions(seed: int = 7, count: int = 32, tri_range=(1e4, 5e5), edge_
     return out
 
 
+def volume_scene(sidecar_path: str, seed: int = 11, spheres: int = 780, grid_n: int = 256) -> dict:
+    """The 4M-triangle participating-media workload: `spheres` displaced
+    icospheres (5,120 triangles each) on a ground quad around a 4 m cube of
+    fBm smoke (albedo 0.9, an grid_n^3 density grid written to
+    `sidecar_path`, boundary-only cube), an emissive quad and a spot light.
+    Vertex and triangle arrays stay numpy arrays."""
+    rng = np.random.default_rng(seed)
+    if 4 not in _ICO_CACHE:
+        _ICO_CACHE[4] = _icosphere(4)
+    base_v, base_t = _ICO_CACHE[4]
+    ext = 22.0
+    meshes = [_mesh(*_quad((-ext, 0, -ext), (-ext, 0, ext), (ext, 0, ext), (ext, 0, -ext)), _diffuse([0.6, 0.6, 0.6]))]
+    for i in range(spheres):
+        radius = rng.uniform(0.3, 0.8)
+        while True:
+            c = np.array([rng.uniform(-ext, ext), rng.uniform(0.4, 4.0), rng.uniform(-ext, ext)])
+            if max(abs(c[0]), abs(c[2])) > 3.0:  # keep the smoke cube free
+                break
+        k = rng.normal(size=(3, 3))
+        k /= np.linalg.norm(k, axis=1, keepdims=True)
+        disp = 1.0 + (rng.uniform(0.02, 0.1, 3)[None, :] * np.sin(3.0 * base_v @ k.T)).sum(1)
+        mat = {"base_color": rng.uniform(0.15, 0.9, 3).tolist(), "metallic": 1.0 if rng.uniform() < 0.3 else 0.0,
+               "roughness": float(rng.uniform(0.05, 1.0)), "emission": [0.0, 0.0, 0.0], "transmission": 0.0,
+               "ior": float(rng.uniform(1.3, 1.7))}
+        meshes.append({"vertices": (base_v * (radius * disp)[:, None] + c).astype(np.float32),
+                       "triangles": base_t.astype(np.int32), "material": mat})
+    fbm_grid(grid_n, seed).tofile(sidecar_path)
+    lo, hi = [-2.0, 0.05, -2.0], [2.0, 4.05, 2.0]
+    v, t = _box([(lo[i] + hi[i]) / 2 for i in range(3)], [(hi[i] - lo[i]) / 2 for i in range(3)], 0.0)
+    vol = {"color": [0.9, 0.9, 0.9], "absorption_color": [0.0, 0.0, 0.0], "density": 4.0, "anisotropy": 0.3,
+           "smoke_grid": {"dims": [grid_n] * 3, "world_min": lo, "world_max": hi, "sidecar_path": sidecar_path}}
+    meshes.append(_mesh(v, t, {"base_color": [0.8, 0.8, 0.8], "metallic": 0.0, "roughness": 0.5,
+                               "emission": [0.0, 0.0, 0.0], "node_tree": boundary_node_tree(), "volume": vol}))
+    ev, et = _quad([-1.0, 7.0, -1.0], [1.0, 7.0, -1.0], [1.0, 7.0, 1.0], [-1.0, 7.0, 1.0])
+    meshes.append(_mesh(ev, et, {"base_color": [0.0, 0.0, 0.0], "metallic": 0.0, "roughness": 1.0,
+                                 "emission": [12.0, 11.0, 9.0], "transmission": 0.0, "ior": 1.45}))
+    lights = [{"type": "SPOT", "position": [6.0, 8.0, 6.0], "direction": [-0.55, -0.6, -0.55], "spot_size": 0.8,
+               "spot_blend": 0.2, "color": [1.0, 0.95, 0.9], "energy": 6000.0}]
+    return {"meshes": meshes, "native_lights": lights, "environment": {"color": [0.05, 0.06, 0.08], "strength": 1.0}}
+
+
+def volume_sessions(seed: int = 12, count: int = 16):
+    """`count` (camera, width, height) views of volume_scene: edges
+    log-uniform between 256 x 256 and 1920 x 1080, cameras on a ring around
+    the smoke cube."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        w = int(round(math.exp(rng.uniform(math.log(256), math.log(1920)))))
+        h = int(round(math.exp(rng.uniform(math.log(256), math.log(1080)))))
+        a = 2.0 * math.pi * i / count
+        pos = (9.0 * math.cos(a), 3.0 + rng.uniform(0.0, 2.0), 9.0 * math.sin(a))
+        d = (-pos[0], 2.0 - pos[1], -pos[2])
+        out.append(({"pos": pos, "dir": d, "up": (0.0, 1.0, 0.0), "fov": 50.0}, w, h))
+    return out
+
+
 def to_json(scene: dict) -> str:
     return json.dumps(scene, separators=(",", ":"), default=lambda a: np.asarray(a).tolist())
 
