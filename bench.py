@@ -153,6 +153,50 @@ def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
             "serial_ms_per_pass": round(s_ms, 1), "speedup": round(b_mrays / max(s_mrays, 1e-9), 3)}
 
 
+def volumes_leg(ctx, passes, spp=16):
+    """Participating media at scale: 16 sessions (256^2 .. 1080p) of one
+    3,993,616-triangle scene with a 256^3 fBm smoke grid, traceVolumeMIS at
+    depth 8, one batched pass of `spp` samples per step; FP16 grid storage
+    against FP32."""
+    import tempfile
+
+    import lucid_capi
+    import workloads
+
+    views = workloads.volume_sessions()
+    res = {"sessions": len(views), "pixels": sum(w * h for _, w, h in views), "pass_spp": spp,
+           "max_depth": 8, "algorithm": "volume_mis"}
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        scene = workloads.volume_scene(os.path.join(td, "smoke256.bin"))
+        log(f"volumes: scene generated in {time.perf_counter() - t0:.1f} s")
+        for bits in (16, 32):
+            ctx.check(ctx.L.lucid_set_grid_format(ctx.h, bits))
+            t0 = time.perf_counter()
+            h = ctx.scene(scene)
+            pack_s = time.perf_counter() - t0
+            ss = [lucid_capi.Session(ctx, h, lucid_capi.camera(cam), w, hh, 8, "volume_mis") for cam, w, hh in views]
+            lucid_capi.render_sessions(ctx, ss, spp)  # warm-up
+            rays, t = 0, 0.0
+            for _ in range(passes):
+                ts = time.perf_counter()
+                lucid_capi.render_sessions(ctx, ss, spp)
+                t += time.perf_counter() - ts
+                st = ctx.stats()
+                rays += st["rays_closest"] + st["rays_shadow"]
+            mrays = rays / t / 1e6
+            log(f"volumes: FP{bits} grid ({ctx.L.lucid_scene_grid_bytes(h) >> 20} MiB): {mrays:.0f} Mrays/s, "
+                f"{t / passes * 1e3:.0f} ms per pass (packed in {pack_s:.1f} s)")
+            res[f"fp{bits}"] = {"mrays_s": round(mrays, 1), "ms_per_pass": round(t / passes * 1e3, 1),
+                                "grid_bytes": int(ctx.L.lucid_scene_grid_bytes(h))}
+            res["triangles"] = int(ctx.L.lucid_scene_num_triangles(h))
+            for x in ss:
+                x.close()
+            ctx.L.lucid_scene_release(h)
+        ctx.check(ctx.L.lucid_set_grid_format(ctx.h, 32))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE leg")
     ap.add_argument("--no-sessions", action="store_true", help="skip the 32-session batched leg")
+    ap.add_argument("--volumes", action="store_true",
+                    help="also measure 16 volume sessions of the 4M-triangle smoke scene (FP16 vs FP32 grid)")
     ap.add_argument("--session-passes", type=int, default=2)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
@@ -360,6 +406,8 @@ def main():
         }
     if not args.no_sessions:
         result["sessions"] = sessions_leg(ctx, args.session_passes)
+    if args.volumes:
+        result["volumes"] = volumes_leg(ctx, args.session_passes)
     if not args.no_cpu:
         log("cpu baseline (oracle, fast build) ...")
         result["cpu_baseline"] = cpu_baseline(scene, cam, W, H, depth)

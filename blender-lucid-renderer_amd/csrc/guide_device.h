@@ -21,15 +21,15 @@ namespace guide {
 __device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f wo) {
     if (idx < kFeatPos) {
         const int c = idx % 3, q = idx / 3, k = q >> 1;
-        const float a = pn[c] * (PI_F * (float)(1 << k));
-        return (q & 1) ? cosf(a) : sinf(a);
+        const float a = pn[c] * (float)(1 << k);  // in half turns
+        return (q & 1) ? cospif(a) : sinpif(a);
     }
     const float nn[3] = {n.x, n.y, n.z}, ww[3] = {wo.x, wo.y, wo.z};
     if (idx < 51) return 2.0f * pn[idx - 48] - 1.0f;
     if (idx < 54) return nn[idx - 51];
     if (idx < 57) return ww[idx - 54];
-    if (idx < 60) return sinf(PI_F * nn[idx - 57]);
-    if (idx < 63) return sinf(PI_F * ww[idx - 60]);
+    if (idx < 60) return sinpif(nn[idx - 57]);
+    if (idx < 63) return sinpif(ww[idx - 60]);
     return dot(n, wo);
 }
 
@@ -39,7 +39,7 @@ __device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f 
 __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float f[kIn]) {
     for (int c = 0; c < 3; ++c) {
         float sn, cs;
-        sincosf(PI_F * pn[c], &sn, &cs);
+        sincospif(pn[c], &sn, &cs);
         for (int k = 0; k < kOctaves; ++k) {
             f[3 * (2 * k) + c] = sn;
             f[3 * (2 * k + 1) + c] = cs;
@@ -51,8 +51,8 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
     f[48] = 2.0f * pn[0] - 1.0f; f[49] = 2.0f * pn[1] - 1.0f; f[50] = 2.0f * pn[2] - 1.0f;
     f[51] = n.x; f[52] = n.y; f[53] = n.z;
     f[54] = wo.x; f[55] = wo.y; f[56] = wo.z;
-    f[57] = sinf(PI_F * n.x); f[58] = sinf(PI_F * n.y); f[59] = sinf(PI_F * n.z);
-    f[60] = sinf(PI_F * wo.x); f[61] = sinf(PI_F * wo.y); f[62] = sinf(PI_F * wo.z);
+    f[57] = sinpif(n.x); f[58] = sinpif(n.y); f[59] = sinpif(n.z);
+    f[60] = sinpif(wo.x); f[61] = sinpif(wo.y); f[62] = sinpif(wo.z);
     f[63] = dot(n, wo);
 }
 
@@ -61,7 +61,7 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
 __device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, __bf16* row) {
     for (int c = 0; c < 3; ++c) {
         float sn, cs;
-        sincosf(PI_F * pn[c], &sn, &cs);
+        sincospif(pn[c], &sn, &cs);
 #pragma unroll
         for (int k = 0; k < kOctaves; ++k) {
             row[3 * (2 * k) + c] = (__bf16)sn;
@@ -75,8 +75,8 @@ __device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, 
     row[50] = (__bf16)(2.0f * pn[2] - 1.0f);
     row[51] = (__bf16)n.x; row[52] = (__bf16)n.y; row[53] = (__bf16)n.z;
     row[54] = (__bf16)wo.x; row[55] = (__bf16)wo.y; row[56] = (__bf16)wo.z;
-    row[57] = (__bf16)sinf(PI_F * n.x); row[58] = (__bf16)sinf(PI_F * n.y); row[59] = (__bf16)sinf(PI_F * n.z);
-    row[60] = (__bf16)sinf(PI_F * wo.x); row[61] = (__bf16)sinf(PI_F * wo.y); row[62] = (__bf16)sinf(PI_F * wo.z);
+    row[57] = (__bf16)sinpif(n.x); row[58] = (__bf16)sinpif(n.y); row[59] = (__bf16)sinpif(n.z);
+    row[60] = (__bf16)sinpif(wo.x); row[61] = (__bf16)sinpif(wo.y); row[62] = (__bf16)sinpif(wo.z);
     row[63] = (__bf16)dot(n, wo);
 }
 
