@@ -85,6 +85,10 @@ struct WaveDev {
     float4* sh_o;    // shadow origin, distance
     float4* sh_d;    // shadow direction
     float4* sh_c;    // pending NEE contribution
+    // traceVolumeMIS with gridded media: the NEE contribution's factors
+    // until k_vol_march has the shadow ray's transmittance T:
+    // sh_c = (sh_a * (sh_e * T)) * sh_t
+    float4 *sh_a, *sh_e, *sh_t;
     int32_t* q[2];   // live-path queues
     int32_t* qs;     // shadow queue
     int32_t* cnt;    // [0],[1] queue counts, [2] shadow count, [3] stack overflows
@@ -415,6 +419,14 @@ __device__ __noinline__ v3f vol_transmittance(const SceneDev& S, v3f o, v3f d, f
     return mk(expf(-tr), expf(-tg), expf(-tb));
 }
 
+// The factors of a traceVolumeMIS NEE contribution whose transmittance is
+// marched later by k_vol_march.
+__device__ __forceinline__ void defer_transmittance(const WaveDev& W, int p, v3f a, v3f e, v3f thr) {
+    W.sh_a[p] = make_float4(a.x, a.y, a.z, 0.0f);
+    W.sh_e[p] = make_float4(e.x, e.y, e.z, 0.0f);
+    W.sh_t[p] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+}
+
 // sample_volume_distance (transmittance.hpp:220-289): Woodcock / delta
 // tracking against the max-density majorant of each gridded medium, in mesh
 // order, at most 4096 steps; draws from the path's PCG stream in the
@@ -467,7 +479,7 @@ __device__ __forceinline__ v3f cam_dir(const CamDev& c, float nx, float ny) {
 
 __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
     int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; }
+    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
     if (p >= n_paths) return;
     int b = p / c.P, pix = p - b * c.P;
     int px = pix % c.tw, py = pix / c.tw;
@@ -662,6 +674,92 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
     }
 }
 
+// Extinction transmittance of the queued traceVolumeMIS shadow rays
+// (vol_transmittance with extinction, the same march in the same order),
+// persistent with refill like the traversal: a lane marches up to kMarchChunk
+// steps per loop iteration, so lanes whose rays cross long stretches of
+// medium do not hold back the rest of their wavefront.  Then
+// sh_c = (sh_a * (sh_e * T)) * sh_t.  Fetch counter cnt[6].
+constexpr int kMarchChunk = 8;
+__global__ __launch_bounds__(kBlock) void k_vol_march(SceneDev S, WaveDev W, int refill) {
+    const int n = W.cnt[2];
+    int32_t* fetch = &W.cnt[6];
+    int p = -1;
+    bool exhausted = false;
+    v3f o = mk(0.0f, 0.0f, 0.0f), d = o;
+    float tmax = 0.0f, tin = 0.0f, tout = 0.0f, step = 0.0f, er = 0.0f, eg = 0.0f, eb = 0.0f;
+    float tr = 0.0f, tg = 0.0f, tb = 0.0f;
+    int vi = 0, i = 0, ns = 0;
+    for (;;) {
+        const unsigned long long im = __ballot(p < 0);
+        const int nidle = __popcll(im);
+        if (!exhausted && (nidle >= refill || nidle == 64)) {
+            const int leader = __ffsll((long long)im) - 1;
+            int base = 0;
+            if (lane_id() == leader) base = atomicAdd(fetch, nidle);
+            base = __shfl(base, leader);
+            exhausted = base + nidle >= n;
+            if (p < 0) {
+                const int q = base + __popcll(im & ((1ull << lane_id()) - 1ull));
+                if (q < n) {
+                    p = W.qs[q];
+                    const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
+                    o = mk(o4); d = mk(d4);
+                    tmax = o4.w;
+                    tr = tg = tb = 0.0f;
+                    vi = -1;
+                    i = ns = 0;  // no medium entered yet
+                }
+            }
+        }
+        if (__ballot(p >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (p < 0) continue;
+        if (i >= ns) {
+            // next gridded medium the shadow segment crosses, or done
+            ++vi;
+            for (; vi < S.n_vols; ++vi) {
+                const GVolume& v = S.vols[vi];
+                if (!v.has_grid) continue;
+                if (!ray_aabb_segment(o, d, tmax, v.wmin, v.wmax, tin, tout)) continue;
+                const float seg = tout - tin;
+                if (seg <= 0.0f) continue;
+                const float ext = max3(v.wmax[0] - v.wmin[0], v.wmax[1] - v.wmin[1], v.wmax[2] - v.wmin[2]);
+                const int md = max(max(v.dims[0], v.dims[1]), v.dims[2]);
+                const float vox = ext / (float)max(1, md);
+                step = smax(1e-4f, vox * 0.5f);
+                ns = min(2048, (int)(seg / step) + 1);
+                er = 1.0f + v.absorption[0]; eg = 1.0f + v.absorption[1]; eb = 1.0f + v.absorption[2];
+                i = 0;
+                break;
+            }
+            if (vi >= S.n_vols) {
+                const v3f T = mk(expf(-tr), expf(-tg), expf(-tb));
+                const float4 a = W.sh_a[p], e = W.sh_e[p], t = W.sh_t[p];
+                const v3f cc = cmul(cmul(mk(a), cmul(mk(e), T)), mk(t));
+                W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                p = -1;
+                continue;
+            }
+        }
+        const GVolume& v = S.vols[vi];
+        for (int k = 0; k < kMarchChunk && i < ns; ++k) {
+            const float t = tin + ((float)i + 0.5f) * step;
+            if (t > tout) {
+                i = ns;
+                break;
+            }
+            ++i;
+            const float dd = grid_value(S, v, o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+            if (dd <= 0.0f) continue;
+            const float base = v.density * dd * step;
+            tr += base * er; tg += base * eg; tb += base * eb;
+        }
+    }
+}
+
 // ----------------------------------------------------------------- shade --
 struct NativeHit {
     bool hit;
@@ -853,17 +951,16 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                             const float a1 = randf(rng), a2 = randf(rng);
                             const LSample ls = sample_light(S.lights[li], x, a1, a2);
                             if (ls.pdf > MIN_PDF) {
-                                const v3f T = vol_transmittance(S, x, ls.dir, ls.dist, true);
                                 const float ct = dot(wo, ls.dir);
                                 const float pe = hg_eval(g, ct);
                                 const float pp = hg_eval(g, ct);
                                 const float pl = ls.pdf * sp;
                                 const float mw = power_heuristic(pl, pp);
                                 const v3f pw = bsdf_weight(mk(pe, pe, pe), 1.0f, pl);
-                                const v3f cc = cmul(cmul(pw * mw, cmul(ls.emission, T)), thr);
+                                // the transmittance march runs in k_vol_march
+                                defer_transmittance(W, p, pw * mw, ls.emission, thr);
                                 W.sh_o[p] = make_float4(x.x, x.y, x.z, ls.dist);
                                 W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
-                                W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
                                 nee = true;
                             }
                         }
@@ -1016,12 +1113,16 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
-                                // surface NEE through media (path_tracer.hpp:783-797)
-                                const v3f T = S.n_vols > 0 ? vol_transmittance(S, point, ls.dir, ls.dist, true)
-                                                           : mk(1.0f, 1.0f, 1.0f);
+                                // surface NEE through media (path_tracer.hpp:783-797); with
+                                // gridded media the transmittance march runs in k_vol_march
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
-                                cc = cmul(bsdf_weight(f, nl, pl) * mw, cmul(ls.emission, T));
+                                if (S.n_vols > 0) {
+                                    defer_transmittance(W, p, bsdf_weight(f, nl, pl) * mw, ls.emission, thr);
+                                    cc = mk(0.0f, 0.0f, 0.0f);
+                                } else {
+                                    cc = cmul(bsdf_weight(f, nl, pl) * mw, cmul(ls.emission, mk(1.0f, 1.0f, 1.0f)));
+                                }
                             } else {
                                 cc = cmul(bsdf_weight(f, nl, pl), ls.emission);
                             }
@@ -1120,7 +1221,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
                                                   int32_t max_depth) {
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
-    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
         shade_one<ALGO>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
 }
@@ -1476,6 +1577,8 @@ struct Lane {
     std::vector<void*> allocs;
     void* ctr = nullptr;  // queue / ray counters, kept across reallocations
     WaveDev W{};
+    std::vector<void*> vallocs;  // deferred-transmittance factors (traceVolumeMIS)
+    int64_t vcap = 0;
 };
 
 struct lucid_ctx {
@@ -1644,6 +1747,26 @@ int ensure_wave(lucid_ctx* ctx, Lane& L, int64_t n) {
         return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (wave buffers)");
     }
     L.cap_paths = n;
+    return LUCID_OK;
+}
+
+// sh_a / sh_e / sh_t of lane L for n paths (traceVolumeMIS with media).
+int ensure_vol_wave(lucid_ctx* ctx, Lane& L, int64_t n) {
+    if (n <= L.vcap) return LUCID_OK;
+    HIP_OK(hipStreamSynchronize(L.stream));
+    for (void* q : L.vallocs) (void)hipFree(q);
+    L.vallocs.clear();
+    L.vcap = 0;
+    for (int k = 0; k < 3; ++k) {
+        void* q = nullptr;
+        if (hipMalloc(&q, (size_t)n * 16) != hipSuccess) {
+            for (void* r : L.vallocs) (void)hipFree(r);
+            L.vallocs.clear();
+            return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (volume NEE factors)");
+        }
+        L.vallocs.push_back(q);
+    }
+    L.vcap = n;
     return LUCID_OK;
 }
 
@@ -1816,6 +1939,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     }
     SceneDev SD = scene->dev;
     WaveDev W = L.W;
+    W.sh_a = W.sh_e = W.sh_t = nullptr;
     const int32_t dsub = ctx->diag_sub;
     const int32_t dsw = pr->tile_w / dsub, dsh = pr->tile_h / dsub;
     if (diag) {
@@ -1927,6 +2051,16 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         algo = pr->algorithm == LUCID_ALGO_SIMPLE ? kAlgoDiagSimple
                : (pr->algorithm == LUCID_ALGO_MIS || pr->algorithm == LUCID_ALGO_GUIDED_MIS) ? kAlgoDiagMis
                                                                                                : kAlgoDiagNee;
+    // traceVolumeMIS through gridded media marches its NEE transmittance in
+    // k_vol_march, between shading and the shadow rays
+    const bool march = algo == LUCID_ALGO_VOLUME_MIS && SD.n_vols > 0;
+    if (march) {
+        rc = ensure_vol_wave(ctx, L, P * B);
+        if (rc) return rc;
+        W.sh_a = static_cast<float4*>(L.vallocs[0]);
+        W.sh_e = static_cast<float4*>(L.vallocs[1]);
+        W.sh_t = static_cast<float4*>(L.vallocs[2]);
+    }
     for (int s0 = 0; s0 < S; s0 += (int)B) {
         // Pacing: wait for the wave before the previous one, so at most two
         // waves are queued and a cancel takes effect within one wave.
@@ -1990,6 +2124,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                 default:
                     k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
                     break;
+            }
+            if (march) {
+                const unsigned gm = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
+                k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, ctx->refill);
             }
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
@@ -2125,6 +2263,7 @@ void lucid_destroy(lucid_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->lanes[i].stream);
         free_wave(ctx->lanes[i]);
         if (ctx->lanes[i].ctr) (void)hipFree(ctx->lanes[i].ctr);
+        for (void* q : ctx->lanes[i].vallocs) (void)hipFree(q);
     }
     guide::destroy(ctx->gstate);
     if (ctx->gout) (void)hipFree(ctx->gout);
