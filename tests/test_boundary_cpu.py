@@ -67,6 +67,9 @@ def test_capi_rejects_bad_arguments_without_device():
     assert L.lucid_session_samples(None) == 0 and L.lucid_session_async_running(None) == 0
     assert L.lucid_session_error(None) == b"null session"
     L.lucid_session_destroy(None)
+    assert L.lucid_set_grid_format(None, 16) == 1 and L.lucid_scene_grid_bytes(None) == 0
+    assert L.lucid_set_refill_threshold(None, 48) == 1
+    assert L.lucid_set_launch_shape(None, 16, 8) == 1
 
 
 def test_async_api_without_scene_stays_idle(lucid):

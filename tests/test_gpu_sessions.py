@@ -159,3 +159,50 @@ def test_renderer_async_api(lucid):
     # a render_tile after the async session is unaffected
     px2 = r.render_tile(0, 0, 16, 16, 16, 16, 4, 0, 8)
     assert len(px2) == 16 * 16 * 4
+
+
+def test_batch_with_guided_session_and_empty_scene(ctx):
+    """Guided sessions share the context's network and run on the first
+    lane; a scene without geometry renders the environment; the plain
+    sessions of the same batch stay bit-exact."""
+    box = workloads.closed_box()
+    sh_box = ctx.scene(box)
+    env_only = {"meshes": [], "native_lights": [], "environment": {"color": [0.2, 0.3, 0.4], "strength": 2.0}}
+    sh_env = ctx.scene(env_only)
+    cam = lucid_capi.camera(workloads.CLOSED_BOX_CAMERA)
+    gc = lucid_capi.GuidingConfig(0.5, 8, 4096, 5e-3, 1, 3, 0)
+    ctx.check(ctx.L.lucid_set_guiding(ctx.h, lucid_capi.C.byref(gc)))
+    guided = lucid_capi.Session(ctx, sh_box, cam, 40, 30, 8, "guided")
+    plain = lucid_capi.Session(ctx, sh_box, cam, 40, 30, 8, "mis")
+    empty = lucid_capi.Session(ctx, sh_env, cam, 16, 8, 8, "mis")
+    for _ in range(3):
+        lucid_capi.render_sessions(ctx, [guided, plain, empty], 4)
+    assert guided.samples == plain.samples == empty.samples == 12
+    g = guided.read(average=True)
+    assert np.all(np.isfinite(g)) and g[0::4].mean() > 0
+    assert np.array_equal(plain.read(), _render_tile(ctx, sh_box, cam, 40, 30, 12, 8, "mis"))
+    e = empty.read(average=True).reshape(-1, 4)
+    np.testing.assert_allclose(e[:, :3], np.tile(np.float32([0.4, 0.6, 0.8]), (16 * 8, 1)), rtol=1e-6)
+    gs = lucid_capi.GuidingStats()
+    ctx.check(ctx.L.lucid_guiding_get_stats(ctx.h, lucid_capi.C.byref(gs)))
+    assert gs.train_steps == 3 * 8 and gs.samples_total > 0
+    for s in (guided, plain, empty):
+        s.close()
+    ctx.L.lucid_scene_release(sh_box)
+    ctx.L.lucid_scene_release(sh_env)
+
+
+def test_async_guided_session(ctx):
+    scene = workloads.closed_box()
+    sh = ctx.scene(scene)
+    gc = lucid_capi.GuidingConfig(0.5, 4, 4096, 5e-3, 1, 5, 0)
+    ctx.check(ctx.L.lucid_set_guiding(ctx.h, lucid_capi.C.byref(gc)))
+    s = lucid_capi.Session(ctx, sh, lucid_capi.camera(workloads.CLOSED_BOX_CAMERA), 48, 32, 6, "guided")
+    ctx.check(ctx.L.lucid_session_start_async(s.h))
+    _wait(lambda: s.poll()[0] >= 32)
+    ctx.check(ctx.L.lucid_session_stop_async(s.h))
+    assert ctx.L.lucid_session_error(s.h) == b""
+    avg = s.read(average=True)
+    assert np.all(np.isfinite(avg)) and avg[0::4].mean() > 0
+    s.close()
+    ctx.L.lucid_scene_release(sh)
