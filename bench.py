@@ -211,6 +211,7 @@ def main():
     ap.add_argument("--volumes", action="store_true",
                     help="also measure 16 volume sessions of the 4M-triangle smoke scene (FP16 vs FP32 grid)")
     ap.add_argument("--session-passes", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
     ap.add_argument("--alpha", type=float, default=0.7)
@@ -268,6 +269,16 @@ def main():
         del a, b
         log(f"copy probe: {args.copy_probe_mib} MiB read + written")
 
+    shape = None
+    if not args.no_autotune:  # launch shapes do not change results, only speed
+        ls = lucid_capi.LaunchShape()
+        t0 = time.perf_counter()
+        tune_pr = lucid_capi.params(0, 0, W, H, W, H, spp, 0, depth, algo)  # one pass of the measured size
+        ctx.check(L.lucid_autotune(ctx.h, sc, C.byref(camd), C.byref(tune_pr), C.byref(ls)))
+        shape = {"traverse_groups_per_cu": ls.traverse_groups, "refill_idle_lanes": ls.refill,
+                 "shade_groups_per_cu": ls.shade_groups}
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {shape}, {ls.ms:.1f} ms per pass")
+
     # ---- timed plain passes: Mrays/s ----
     for i in range(args.warmup):
         s = render(algo, spp, 1000000 + i * spp)
@@ -321,6 +332,7 @@ def main():
         "config": {"workload": f"{args.workload}: {ntri} triangles, {W}x{H}, {spp} spp per step, depth {depth}, "
                                f"{algo}, {lights}",
                    "rays_per_step": rays // max(args.steps, 1), "paths_per_step": W * H * spp,
+                   "launch_shape": shape,
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_trace_refill<false> (closest-hit BVH traversal, persistent with ray refill)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

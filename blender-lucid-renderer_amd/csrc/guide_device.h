@@ -99,8 +99,10 @@ __device__ __forceinline__ float out_f32(__half x) { return __half2float(x); }
 
 // Raw outputs as FP32 (training) or FP16 (what the render-side inference
 // stores per path: half the bytes; sampling and pdf read the same values).
+// Returns false for a degenerate mixture (a non-finite weight, mean,
+// concentration or normalisation), which the caller must not sample.
 template <class T>
-__device__ __forceinline__ void decode(const T* o, Mixture& m) {
+__device__ __forceinline__ bool decode(const T* o, Mixture& m) {
     float mx = -1e30f;
     for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, out_f32(o[5 * k + 4]));
     float sum = 0.0f;
@@ -115,7 +117,13 @@ __device__ __forceinline__ void decode(const T* o, Mixture& m) {
         sum += m.w[k];
     }
     const float inv = 1.0f / sum;
-    for (int k = 0; k < kLobes; ++k) m.w[k] *= inv;
+    bool ok = isfinite(inv) && sum > 0.0f;
+    for (int k = 0; k < kLobes; ++k) {
+        m.w[k] *= inv;
+        ok = ok && isfinite(m.w[k]) && isfinite(m.logc[k]) && isfinite(m.mu[k].x) && isfinite(m.mu[k].y) &&
+             isfinite(m.mu[k].z);
+    }
+    return ok;
 }
 
 __device__ __forceinline__ float mixture_pdf(const Mixture& m, v3f w) {

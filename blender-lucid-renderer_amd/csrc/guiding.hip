@@ -152,7 +152,7 @@ __global__ void k_loss(const Sample* samples, const int32_t* sel, int batch, con
     float* d = dout + (size_t)b * kOutPad;
     const float c = *wsum > 0.0f ? s.weight / *wsum : 0.0f;
     Mixture mx;
-    decode(o, mx);
+    (void)decode(o, mx);  // training takes the raw outputs as they are
     const v3f wi = mk(s.wi[0], s.wi[1], s.wi[2]);
     float lf[kLobes], p = 0.0f;
     for (int k = 0; k < kLobes; ++k) {

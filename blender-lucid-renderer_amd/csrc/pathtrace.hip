@@ -1046,9 +1046,11 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             // guided bounce: opaque, non-delta surfaces (the BSDF sample is an
             // evaluated f / pdf pair there)
             const bool guidable = GUIDED && mat.transmission == 0.0f && !(mat.roughness < 0.05f && mat.metallic > 0.5f);
-            const bool guided = guidable && G.alpha > 0.0f;
+            bool guided = guidable && G.alpha > 0.0f;
             guide::Mixture gmx;
-            if (GUIDED && guided) guide::decode(W.gout + (size_t)p * guide::kOut, gmx);
+            // a degenerate learned density (non-finite outputs) leaves this
+            // vertex to plain BSDF sampling and NEE
+            if (GUIDED && guided) guided = guide::decode(W.gout + (size_t)p * guide::kOut, gmx);
             if (SIMPLE) {
                 if (DIAG) {
                     if (has_em) {
@@ -2585,6 +2587,75 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;
     ctx->wave_paths = paths;
     return LUCID_OK;
+}
+
+// Device time of one render of `pr` on lane 0 (output discarded).
+static int timed_render(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                        double* ms) {
+    bool cancelled = false;
+    HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
+    int rc = run_waves(ctx, ctx->lanes[0], scene, cam, pr, 0, ctx->stream, nullptr, &cancelled);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    float e = 0.0f;
+    HIP_OK(hipEventElapsedTime(&e, ctx->ev0, ctx->ev1));
+    *ms = e;
+    return cancelled ? fail(ctx, LUCID_ERR_CANCELLED, "autotune cancelled") : LUCID_OK;
+}
+
+int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                   lucid_launch_shape* best) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!best) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    if (ctx->diag_on) return fail(ctx, LUCID_ERR_INVALID, "autotune with diagnostics enabled");
+    lucid_render_params tp = *pr;
+    // guided renders would train the network: tune on the plain tracer
+    if (tp.algorithm == LUCID_ALGO_GUIDED_MIS) tp.algorithm = LUCID_ALGO_MIS;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    const int32_t prof = ctx->profiling;
+    ctx->profiling = 0;
+    auto measure = [&](int32_t tg, int32_t rf, int32_t sg, double* ms) {
+        ctx->traverse_groups = tg;
+        ctx->refill = rf;
+        ctx->shade_groups = sg;
+        double a = 0.0, b = 0.0;
+        int r = timed_render(ctx, scene, cam, &tp, &a);  // first run: allocation, caches
+        if (!r) r = timed_render(ctx, scene, cam, &tp, &b);
+        *ms = std::min(a, b);
+        return r;
+    };
+    // coordinate descent from the current shape over measured-good ranges
+    int32_t tg = ctx->traverse_groups, rf = ctx->refill, sg = ctx->shade_groups;
+    double t_best = 0.0;
+    rc = measure(tg, rf, sg, &t_best);
+    const int32_t tgs[] = {8, 12, 16, 24}, rfs[] = {32, 48, 60}, sgs[] = {4, 8, 16};
+    for (int32_t v : tgs) {
+        double t = 0.0;
+        if (!rc && v != tg && !(rc = measure(v, rf, sg, &t)) && t < t_best) { t_best = t; tg = v; }
+    }
+    for (int32_t v : rfs) {
+        double t = 0.0;
+        if (!rc && v != rf && !(rc = measure(tg, v, sg, &t)) && t < t_best) { t_best = t; rf = v; }
+    }
+    for (int32_t v : sgs) {
+        double t = 0.0;
+        if (!rc && v != sg && !(rc = measure(tg, rf, v, &t)) && t < t_best) { t_best = t; sg = v; }
+    }
+    ctx->traverse_groups = tg;
+    ctx->refill = rf;
+    ctx->shade_groups = sg;
+    ctx->profiling = prof;
+    if (rc) return rc;
+    rc = finish_stats(ctx);
+    best->traverse_groups = tg;
+    best->refill = rf;
+    best->shade_groups = sg;
+    best->ms = t_best;
+    return rc;
 }
 
 int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {

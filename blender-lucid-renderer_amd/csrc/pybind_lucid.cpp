@@ -445,6 +445,27 @@ public:
         return (int)rev;
     }
 
+    // Extension: launch-shape autotuner on a full-frame render of the
+    // current scene, camera and algorithm (the output is discarded).
+    py::dict autotune(int width, int height, int samples, int max_depth) {
+        require_ready();
+        lucid_launch_shape b{};
+        {
+            py::gil_scoped_release rel;
+            std::lock_guard<std::mutex> l(render_mu_);
+            ensure_scene();
+            apply_diagnostics();
+            lucid_render_params pr = params(0, 0, width, height, width, height, samples, 0, max_depth);
+            check(lucid_autotune(ctx_, scene_, &cam_, &pr, &b));
+        }
+        py::dict d;
+        d["traverse_groups"] = b.traverse_groups;
+        d["refill"] = b.refill;
+        d["shade_groups"] = b.shade_groups;
+        d["ms"] = b.ms;
+        return d;
+    }
+
     // Extension: per-sample radiance, shape (tile_h * tile_w, samples, 3).
     py::array_t<float> debug_render_samples(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset,
                                             int max_depth) {
@@ -870,6 +891,9 @@ PYBIND11_MODULE(lucidrenderer, m) {
              py::arg("up_z"), py::arg("fov_deg"))
         .def("set_algorithm", &Renderer::set_algorithm, py::arg("algorithm"))
         .def("get_algorithm", &Renderer::get_algorithm)
+        .def("autotune", &Renderer::autotune, py::arg("width"), py::arg("height"), py::arg("samples") = 4,
+             py::arg("max_depth") = 8,
+             "Extension: pick the fastest launch shape for renders like this one (results do not change)")
         .def("set_grid_precision", &Renderer::set_grid_precision, py::arg("bits"),
              "Extension: store density grids as FP32 (32, default) or FP16 (16)")
         .def("get_grid_precision", &Renderer::get_grid_precision)

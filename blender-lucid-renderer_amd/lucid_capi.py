@@ -86,6 +86,11 @@ class GuidingStats(C.Structure):
                 ("active", C.c_int32), ("num_params", C.c_int32)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("traverse_groups", C.c_int32), ("refill", C.c_int32), ("shade_groups", C.c_int32),
+                ("ms", C.c_double)]
+
+
 SIGNATURES = {
     "lucid_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
     "lucid_destroy": (None, [C.c_void_p]),
@@ -112,6 +117,8 @@ SIGNATURES = {
     "lucid_set_wave_paths": (C.c_int, [C.c_void_p, C.c_int64]),
     "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lucid_set_refill_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
+                                 C.POINTER(LaunchShape)]),
     "lucid_set_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(DiagConfig)]),
     "lucid_diag_get_info": (C.c_int, [C.c_void_p, C.POINTER(DiagInfo)]),
     "lucid_diag_read_pixels": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),

@@ -206,6 +206,19 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
    counterpart in the reference (its Dawn backend fixes workgroup counts per
    dispatch). */
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
+/* Autotuner (no counterpart in the reference): times launch shapes on
+   renders of `params` (discarded; guided requests are tuned on plain MIS)
+   by coordinate descent -- traversal groups per CU in {8, 12, 16, 24}, then
+   the refill threshold in {32, 48, 60}, then shading groups per CU in
+   {4, 8, 16}, two renders each -- and keeps the fastest on the context.
+   Launch shapes do not change results.  Not with diagnostics enabled. */
+typedef struct lucid_launch_shape {
+    int32_t traverse_groups, refill, shade_groups;
+    double ms;  /* device time of the chosen shape's render */
+} lucid_launch_shape;
+int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
+                   lucid_launch_shape* best);
+
 /* Persistent traversal: a wavefront fetches new rays once this many of its
    64 lanes are idle (1..64, default 48).  A tuning knob, no counterpart in
    the reference. */

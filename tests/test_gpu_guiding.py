@@ -106,19 +106,56 @@ def test_guided_is_unbiased(lucid):
 
 
 def test_guiding_gain_occluded_room(lucid):
+    """What the trained network gains: on the occluded room (light behind a
+    5 cm slit), guided MIS reaches a lower relMSE than plain MIS at equal
+    spp.  relMSE is measured against a 32,768-spp plain MIS image of a
+    96 x 54 frame; the guide trains on 24 passes of a 480 x 270 frame of the
+    same view (its input is the scene position, so a finer frame only gives
+    it more samples).  Two independent seeds, summed errors: measured gain
+    about 2x; the bar is 1.3x."""
     W, H = 96, 54
     wl = workloads.WORKLOADS["occluded_room"]
     scene, cam = wl["make"](), wl["camera"]
     ref = make_renderer(lucid, scene, cam, "mis")
-    R = np.asarray(ref.render_tile(0, 0, W, H, W, H, 8192, 1 << 20, 12), np.float64).reshape(H, W, 4)[..., :3] / 8192
-    spp = 64
-    plain = make_renderer(lucid, scene, cam, "mis")
-    X = np.asarray(plain.render_tile(0, 0, W, H, W, H, spp, 0, 12), np.float64).reshape(H, W, 4)[..., :3] / spp
-    gr = make_renderer(lucid, scene, cam, "guided")
-    gr.set_guiding(alpha=0.5, train_steps=32, batch=16384, learning_rate=5e-3, record_stride=1, seed=9)
-    for i in range(8):
-        gr.render_tile(0, 0, W, H, W, H, 16, 500000 + 16 * i, 12)
-    Y = np.asarray(gr.render_tile(0, 0, W, H, W, H, spp, 0, 12), np.float64).reshape(H, W, 4)[..., :3] / spp
-    e_plain, e_guided = workloads.relmse(X, R), workloads.relmse(Y, R)
-    print(f"relMSE at {spp} spp: plain {e_plain:.4f}, guided {e_guided:.4f}")
-    assert e_guided < e_plain
+    R = np.zeros((H, W, 3))
+    for k in range(8):
+        R += np.asarray(ref.render_tile(0, 0, W, H, W, H, 4096, (1 << 22) + 4096 * k, 12), np.float64).reshape(H, W, 4)[..., :3]
+    R /= 8 * 4096
+
+    def relmse(X):
+        return float(np.mean((X - R) ** 2 / (R * R + 0.01)))
+
+    spp = 512
+    e_plain = e_guided = 0.0
+    for seed in range(2):
+        p = make_renderer(lucid, scene, cam, "mis")
+        X = np.asarray(p.render_tile(0, 0, W, H, W, H, spp, seed * 100000, 12), np.float64).reshape(H, W, 4)[..., :3]
+        g = make_renderer(lucid, scene, cam, "guided")
+        g.set_guiding(alpha=0.7, train_steps=32, batch=16384, learning_rate=5e-3, record_stride=1, seed=9 + seed)
+        for i in range(24):
+            g.render_tile(0, 0, 480, 270, 480, 270, 16, 500000 + 16 * i, 12)
+        Y = np.asarray(g.render_tile(0, 0, W, H, W, H, spp, seed * 100000, 12), np.float64).reshape(H, W, 4)[..., :3]
+        e_plain += relmse(X / spp)
+        e_guided += relmse(Y / spp)
+    print(f"relMSE at {spp} spp (2 seeds): plain {e_plain / 2:.4f}, guided {e_guided / 2:.4f}")
+    assert e_plain > 1.3 * e_guided
+
+
+def test_degenerate_guide_falls_back_to_plain(lucid):
+    """A learned density that is not finite (here: weight logits beyond the
+    FP16 range of the stored outputs) is not sampled: every guided vertex
+    falls back to BSDF sampling and plain NEE weights, so the frozen guided
+    tracer renders exactly what plain MIS renders on the same seeds."""
+    box = workloads.closed_box()
+    cam = workloads.CLOSED_BOX_CAMERA
+    g = make_renderer(lucid, box, cam, "guided")
+    g.set_guiding(alpha=0.7, train_steps=0, batch=4096, learning_rate=1e-3, record_stride=1, seed=4)
+    params = np.asarray(g.get_guiding_params(), np.float32).copy()
+    _, b3 = G.OFFS[3]
+    for k in range(G.N_LOBES):
+        params[b3 + 5 * k + 4] = 1e6  # weight logits: +inf once stored as FP16
+    g.set_guiding_params(params)
+    a = g.debug_render_samples(0, 0, 32, 24, 32, 24, 4, 11, 8)
+    p = make_renderer(lucid, box, cam, "mis").debug_render_samples(0, 0, 32, 24, 32, 24, 4, 11, 8)
+    assert np.array_equal(a, p)
+    assert g.guiding_stats()["active"]

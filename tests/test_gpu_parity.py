@@ -180,3 +180,13 @@ def test_traversal_launch_shapes_agree(case, tmp_path):
     assert outs[0][0].sum() > 0
     ctx.L.lucid_scene_release(sh)
     ctx.close()
+
+
+def test_autotune_picks_a_shape_and_keeps_results(lucid):
+    r = make_renderer(lucid, BOX, BOX_CAM, "mis")
+    before = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
+    shape = r.autotune(64, 48, 4, 8)
+    assert shape["traverse_groups"] in (8, 12, 16, 24) and shape["refill"] in (32, 48, 60)
+    assert shape["shade_groups"] in (4, 8, 16) and shape["ms"] > 0
+    after = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
+    assert np.array_equal(before, after)
