@@ -54,6 +54,8 @@ struct SceneDev {
     const float4* fn;
     const float4* vn;       // 3 per triangle, or null
     const GMat* mats;
+    const GNodeOp* prog;    // node-tree programs (MF_NODES materials), or null
+    const float2* tuv;      // 3 UVs per triangle, BVH order, when a material has MF_NODES
     const GLight* lights;
     const float* cdf;
     const int32_t* native_ids;
@@ -880,7 +882,9 @@ __device__ __forceinline__ void put_record(const WaveDev& W, int p, int slot, v3
 
 // One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570),
 // their volume variants (:209-283, 586-835), or the diagnostic tracers.
-template <int ALGO>
+// NODES: the scene has node-tree programs (separate instantiations keep
+// their registers out of the kernels of scenes without them).
+template <int ALGO, bool NODES>
 __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, const guide::GuideDev& G, int qin,
                                           int32_t max_depth, int i, int n) {
     constexpr bool DIAG = ALGO >= kAlgoDiagBase;
@@ -1029,6 +1033,21 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             mat.metallic = gm.metallic; mat.roughness = gm.roughness;
             mat.transmission = gm.transmission; mat.ior = gm.ior;
             v3f em = mk(gm.emission[0], gm.emission[1], gm.emission[2]);
+            bool has_em = (gm.flags & MF_EMISSIVE) != 0;
+            if (NODES && (gm.flags & MF_NODES)) {
+                // hit.uv = uv0 * w + uv1 * u + uv2 * v (geometry/bvh.hpp:67-69),
+                // then the material's programs at that uv
+                const float2* tu = S.tuv + 3 * (size_t)prim;
+                const float2 t0 = tu[0], t1 = tu[1], t2 = tu[2];
+                const float w = 1.0f - u - v;
+                const float hu = t0.x * w + t1.x * u + t2.x * v;
+                const float hv2 = t0.y * w + t1.y * u + t2.y * v;
+                if (gm.prog_albedo >= 0) mat.albedo = run_node_program(S.prog + gm.prog_albedo, hu, hv2);
+                if (gm.flags & MF_EM_UV) {
+                    em = run_node_program(S.prog + gm.prog_emission, hu, hv2);
+                    has_em = luminance(em) > 1e-6f;  // is_emissive, render_units.hpp:1200-1202
+                }
+            }
             v3f point = o + d * t;
             float4 fnv = S.fn[prim];
             v3f nrm = mk(fnv);
@@ -1041,7 +1060,6 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             v3f wo = -d;
             bool front = dot(wo, nrm) > 0.0f;
             v3f sn = (mat.transmission < 0.5f && !front) ? -nrm : nrm;
-            bool has_em = (gm.flags & MF_EMISSIVE) != 0;
             bool stop = false;  // traceSimpleWithDiagnostics ends at an emitter
             // guided bounce: opaque, non-delta surfaces (the BSDF sample is an
             // evaluated f / pdf pair there)
@@ -1218,14 +1236,23 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
 
 // Grid-stride over the live queue; the trip count is uniform per workgroup so
 // every lane reaches the wavefront ballots of shade_one.
-template <int ALGO>
+template <int ALGO, bool NODES>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::GuideDev G, int qin,
                                                   int32_t max_depth) {
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
     if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
-        shade_one<ALGO>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
+        shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
+}
+
+template <int ALGO>
+void launch_shade(unsigned gs, hipStream_t st, const SceneDev& S, const WaveDev& W, const guide::GuideDev& G,
+                  int qin, int32_t max_depth) {
+    if (S.prog)
+        k_shade<ALGO, true><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+    else
+        k_shade<ALGO, false><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
 }
 
 // Surface point and shading normal of a closest hit (as the shade kernel
@@ -2097,34 +2124,34 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             ks = kev_begin(ctx, 1, st);
             switch (algo) {
                 case LUCID_ALGO_SIMPLE:
-                    k_shade<LUCID_ALGO_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_SIMPLE>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_MIS:
-                    k_shade<LUCID_ALGO_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_MIS>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_SIMPLE:
-                    k_shade<LUCID_ALGO_VOLUME_SIMPLE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_VOLUME_SIMPLE>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_MIS:
-                    k_shade<LUCID_ALGO_VOLUME_MIS><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_VOLUME_MIS>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case LUCID_ALGO_GUIDED_MIS: {
                     guide::GuideDev Gd = G;
                     if (!guide_here) Gd.alpha = 0.0f;  // this bounce samples the BSDF only
-                    k_shade<LUCID_ALGO_GUIDED_MIS><<<gs, kBlock, 0, st>>>(SD, W, Gd, q, max_depth);
+                    launch_shade<LUCID_ALGO_GUIDED_MIS>(gs, st, SD, W, Gd, q, max_depth);
                     break;
                 }
                 case kAlgoDiagSimple:
-                    k_shade<kAlgoDiagSimple><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagSimple>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case kAlgoDiagNee:
-                    k_shade<kAlgoDiagNee><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagNee>(gs, st, SD, W, G, q, max_depth);
                     break;
                 case kAlgoDiagMis:
-                    k_shade<kAlgoDiagMis><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagMis>(gs, st, SD, W, G, q, max_depth);
                     break;
                 default:
-                    k_shade<LUCID_ALGO_NEE><<<gs, kBlock, 0, st>>>(SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_NEE>(gs, st, SD, W, G, q, max_depth);
                     break;
             }
             if (march) {
@@ -2325,6 +2352,12 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     if (!rc) rc = up4(ps.tri_fn, &D.fn);
     if (!rc) rc = up4(ps.tri_vn, &D.vn);
     if (!rc) rc = upload(ctx, s, ps.mats, &D.mats);
+    if (!rc) rc = upload(ctx, s, ps.node_ops, &D.prog);
+    if (!rc) {
+        const float* uv = nullptr;
+        rc = upload(ctx, s, ps.tri_uv, &uv);
+        D.tuv = reinterpret_cast<const float2*>(uv);
+    }
     if (!rc) rc = upload(ctx, s, ps.lights, &D.lights);
     if (!rc) rc = upload(ctx, s, ps.cdf, &D.cdf);
     if (!rc) rc = upload(ctx, s, ps.native_light_ids, &D.native_ids);

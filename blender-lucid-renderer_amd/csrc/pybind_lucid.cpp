@@ -15,7 +15,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <fstream>
+#include <map>
 #include <iostream>
 #include <memory>
 #include <mutex>
@@ -37,6 +39,11 @@ struct HostMesh {
     std::vector<int32_t> indices;
     std::vector<float> tri_normals;
     std::vector<float> grid;
+    std::vector<float> tri_uvs;
+    // node tree: names live in `strings` (stable addresses), sockets per node
+    std::deque<std::string> strings;
+    std::vector<std::vector<lucid_node_socket>> node_inputs, node_outputs;
+    std::vector<lucid_node> nodes;
     lucid_mesh_desc desc{};
 };
 
@@ -87,6 +94,92 @@ bool surface_linked(const Value& mat) {
     return true;
 }
 
+const char* keep(HostMesh& hm, const std::string& text) {
+    hm.strings.push_back(text);
+    return hm.strings.back().c_str();
+}
+
+std::string str_or_empty(const Value* v) { return (v && v->kind == Value::Str) ? v->s : std::string(); }
+
+// parseSocketValue (node_evaluator.cpp:47-69): a number is FLOAT, a numeric
+// 3-array VEC3, a 4-array VEC4; anything else carries no value the
+// evaluator reads.
+void socket_value(const Value* v, lucid_node_socket& s) {
+    s.kind = LUCID_SOCK_NONE;
+    if (!v) return;
+    if (v->kind == Value::Num) {
+        s.kind = LUCID_SOCK_FLOAT;
+        s.value[0] = (float)v->n;
+    } else if (v->kind == Value::NumArr && v->shape.size() == 1 && (v->flat.size() == 3 || v->flat.size() == 4)) {
+        s.kind = v->flat.size() == 3 ? LUCID_SOCK_VEC3 : LUCID_SOCK_VEC4;
+        for (size_t k = 0; k < v->flat.size(); ++k) s.value[k] = (float)v->flat[k];
+    }
+}
+
+// parseNodeTree (node_evaluator.cpp:82-149): nodes in JSON order, each input
+// socket linked from the (node, socket) of the last "links" entry that
+// targets it; useNodes = "use_nodes" && a tree with nodes
+// (pybind_renderer.hpp:1381-1384).
+void parse_node_tree(const Value& mat, HostMesh& hm, lucid_mesh_desc& d) {
+    const Value* nt = mat.get("node_tree");
+    if (!nt || !nt->is_object()) return;
+    const Value* nodes = nt->get("nodes");
+    if (!nodes || nodes->kind != Value::Arr || nodes->arr.empty()) return;
+    std::map<std::pair<std::string, std::string>, std::pair<std::string, std::string>> conn;
+    if (const Value* links = nt->get("links"); links && links->kind == Value::Arr) {
+        for (const Value& l : links->arr) {
+            auto field = [&](const char* key) {
+                const Value* v = l.get(key);
+                if (!v || v->kind != Value::Str) throw std::runtime_error(std::string("node_tree link without ") + key);
+                return v->s;
+            };
+            conn[{field("to_node"), field("to_socket")}] = {field("from_node"), field("from_socket")};
+        }
+    }
+    for (const Value& nj : nodes->arr) {
+        const std::string name = str_or_empty(nj.get("name"));
+        lucid_node n{};
+        n.name = keep(hm, name);
+        n.type = keep(hm, str_or_empty(nj.get("type")));
+        std::vector<lucid_node_socket> ins, outs;
+        if (const Value* a = nj.get("inputs"); a && a->kind == Value::Arr) {
+            for (const Value& sj : a->arr) {
+                lucid_node_socket s{};
+                const std::string sn = str_or_empty(sj.get("name"));
+                s.name = keep(hm, sn);
+                socket_value(sj.get("default_value"), s);
+                auto it = conn.find({name, sn});
+                if (it != conn.end()) {
+                    s.linked_node = keep(hm, it->second.first);
+                    s.linked_socket = keep(hm, it->second.second);
+                }
+                ins.push_back(s);
+            }
+        }
+        if (const Value* a = nj.get("outputs"); a && a->kind == Value::Arr) {
+            for (const Value& sj : a->arr) {
+                lucid_node_socket s{};
+                s.name = keep(hm, str_or_empty(sj.get("name")));
+                socket_value(sj.get("default_value"), s);
+                outs.push_back(s);
+            }
+        }
+        hm.node_inputs.push_back(std::move(ins));
+        hm.node_outputs.push_back(std::move(outs));
+        hm.nodes.push_back(n);
+    }
+    for (size_t i = 0; i < hm.nodes.size(); ++i) {
+        hm.nodes[i].inputs = hm.node_inputs[i].data();
+        hm.nodes[i].num_inputs = (int32_t)hm.node_inputs[i].size();
+        hm.nodes[i].outputs = hm.node_outputs[i].data();
+        hm.nodes[i].num_outputs = (int32_t)hm.node_outputs[i].size();
+    }
+    d.nodes = hm.nodes.data();
+    d.num_nodes = (int32_t)hm.nodes.size();
+    const Value* un = mat.get("use_nodes");
+    d.use_nodes = (un && lucid::json::boolean(*un)) ? 1 : 0;
+}
+
 // loadSceneFromJsonString (pybind_renderer.hpp:1258-1542) and
 // parseNativeLights field reading (:1544-1746); unit conversions happen in
 // the library (lucid_scene_create).
@@ -132,6 +225,24 @@ void parse_scene(const std::string& text, HostScene& sc) {
                     }
                 }
             }
+            // "triangle_uvs": [[u, v] x 3] per triangle (pybind_renderer.hpp:1295-1307)
+            if (const Value* tu = mj.get("triangle_uvs")) {
+                const int64_t n = tu->size();
+                hm->tri_uvs.resize(6 * n);
+                if (tu->kind == Value::NumArr) {
+                    if (n > 0 && (tu->flat.size() != (size_t)(6 * n) || !tu->regular))
+                        throw std::runtime_error("triangle_uvs must be 3 x [u, v] per triangle");
+                    for (int64_t i = 0; i < 6 * n; ++i) hm->tri_uvs[i] = (float)tu->flat[i];
+                } else {
+                    for (int64_t i = 0; i < n; ++i) {
+                        const Value& t3 = tu->arr[i];
+                        if (t3.kind != Value::NumArr || t3.flat.size() < 6)
+                            throw std::runtime_error("triangle_uvs entry malformed");
+                        for (int k = 0; k < 6; ++k) hm->tri_uvs[6 * i + k] = (float)t3.flat[k];
+                    }
+                }
+                d.num_tri_uvs = (int32_t)n;
+            }
             bool smooth = false;
             if (const Value* sm = mj.get("smooth")) smooth = lucid::json::boolean(*sm);
             if (const Value* ts = mj.get("triangles")) {
@@ -166,6 +277,7 @@ void parse_scene(const std::string& text, HostScene& sc) {
                 if (const Value* v = mat->get("transmission")) d.transmission = f(*v);
                 if (const Value* v = mat->get("ior")) d.ior = f(*v);
                 d.surface_linked = surface_linked(*mat) ? 1 : 0;
+                parse_node_tree(*mat, *hm, d);
                 const Value* vol = mat->get("volume");
                 if (vol && vol->is_object()) {
                     d.has_volume = 1;
@@ -212,6 +324,7 @@ void parse_scene(const std::string& text, HostScene& sc) {
             d.tri_normals = hm->tri_normals.empty() ? nullptr : hm->tri_normals.data();
             d.num_tri_normals = (int32_t)ntn;
             d.grid = hm->grid.empty() ? nullptr : hm->grid.data();
+            d.tri_uvs = hm->tri_uvs.empty() ? nullptr : hm->tri_uvs.data();
             for (int64_t i = 0; i < 3LL * d.num_triangles; ++i)
                 if (hm->indices[i] < 0 || hm->indices[i] >= d.num_vertices)
                     throw std::runtime_error("triangle index out of range");
@@ -360,6 +473,23 @@ public:
     bool is_scene_loaded() const { return scene_loaded_; }
     bool is_camera_set() const { return camera_set_; }
     int get_mesh_count() const { return scene_loaded_ && host_ ? (int)host_->meshes.size() : 0; }
+
+    // The loaded scene's material of mesh `mesh` at texture coordinates uv
+    // ((n, 2)): getMaterialParams + getEmission (path_tracer.hpp:45-72)
+    // through lucid_material_eval, host only.  (n, 10): albedo RGB,
+    // emission RGB, metallic, roughness, transmission, ior.
+    py::array_t<float> debug_material_eval(int mesh, py::array_t<float, py::array::c_style | py::array::forcecast> uv) {
+        if (!scene_loaded_ || !host_ || mesh < 0 || mesh >= (int)host_->meshes.size())
+            throw std::out_of_range("no such mesh");
+        if (uv.ndim() != 2 || uv.shape(1) != 2) throw std::invalid_argument("uv must have shape (n, 2)");
+        const int32_t n = (int32_t)uv.shape(0);
+        py::array_t<float> out({(py::ssize_t)n, (py::ssize_t)10});
+        char err[256];
+        if (lucid_material_eval(&host_->meshes[mesh]->desc, n, uv.data(), out.mutable_data(), err, sizeof(err)) !=
+            LUCID_OK)
+            throw std::runtime_error(err);
+        return out;
+    }
 
     std::vector<float> render_tile(int tx, int ty, int tw, int th, int fw, int fh, int samples, int offset,
                                    int max_depth) {
@@ -957,6 +1087,7 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("debug_primary_hits", &Renderer::debug_primary_hits, py::arg("tile_x"), py::arg("tile_y"),
              py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"), py::arg("samples") = 1,
              py::arg("sample_offset") = 0)
+        .def("debug_material_eval", &Renderer::debug_material_eval, py::arg("mesh"), py::arg("uv"))
         .def("last_render_stats", &Renderer::last_render_stats)
         .def("set_wave_paths", &Renderer::set_wave_paths, py::arg("paths"))
         .def("is_scene_loaded", &Renderer::is_scene_loaded)

@@ -341,14 +341,36 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
         const lucid_mesh_desc& d = meshes[m];
         GMat& g = ps.mats[m];
         std::memset(&g, 0, sizeof(g));
-        put3(g.albedo, get3(d.base_color));
-        g.metallic = d.metallic;
-        g.roughness = hi(d.roughness, 0.01f);
-        g.transmission = d.transmission;
-        g.ior = d.ior;
-        put3(g.emission, get3(d.emission));
+        // getMaterialParams / getEmission (path_tracer.hpp:45-72): the node
+        // tree, when used, replaces every constant field
+        CompiledMaterial cm;
+        if (!compile_material(d, cm, err)) {
+            err = "mesh " + std::to_string(m) + ": " + err;
+            return false;
+        }
+        put3(g.albedo, get3(cm.albedo));
+        g.metallic = cm.metallic;
+        g.roughness = hi(cm.roughness, 0.01f);
+        g.transmission = cm.transmission;
+        g.ior = cm.ior;
         g.volume = -1;
-        if (luminance(d.emission) > 1e-6f) g.flags |= MF_EMISSIVE;
+        g.prog_albedo = g.prog_emission = -1;
+        if (!cm.albedo_prog.empty()) {
+            g.prog_albedo = (int32_t)ps.node_ops.size();
+            ps.node_ops.insert(ps.node_ops.end(), cm.albedo_prog.begin(), cm.albedo_prog.end());
+            g.flags |= MF_NODES;
+        }
+        // the light list takes the emission at uv (0, 0) (scene_lights.hpp:64-67)
+        float em00[3] = {cm.emission[0], cm.emission[1], cm.emission[2]};
+        if (!cm.emission_prog.empty()) {
+            g.prog_emission = (int32_t)ps.node_ops.size();
+            ps.node_ops.insert(ps.node_ops.end(), cm.emission_prog.begin(), cm.emission_prog.end());
+            run_node_program(&ps.node_ops[g.prog_emission], 0.0f, 0.0f, em00);
+            g.flags |= MF_NODES | MF_EM_UV;
+        }
+        put3(g.emission, get3(em00));
+        if (luminance(em00) > 1e-6f) g.flags |= MF_EMISSIVE;
+        if (g.flags & MF_NODES) ps.any_nodes = true;
         if (d.has_volume) {
             GVolume v;
             std::memset(&v, 0, sizeof(v));
@@ -408,7 +430,7 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
             F3 fn = f3(0, 1, 0);
             unit(fcross(fsub(b, a), fsub(c, a)), fn);
             put3(l.normal, fn);
-            put3(l.emission, get3(d.emission));
+            put3(l.emission, get3(ps.mats[m].emission));
             l.area = area;
             put3(l.pos, fdiv(fadd(fadd(a, b), c), 3.0f));
             tri_light[m][t] = (int32_t)ps.lights.size();
@@ -435,10 +457,12 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
     }
 
     // ---- triangles in BVH order ----
+    if (ps.any_nodes) ps.tri_uv.assign(6 * total, 0.0f);
     for (int m = 0; m < num_meshes; ++m) {
         const lucid_mesh_desc& d = meshes[m];
         const MeshBvh& bvh = bvhs[m];
         bool smooth_mesh = d.smooth && d.tri_normals;
+        const bool uvs = (ps.mats[m].flags & MF_NODES) && d.tri_uvs;
         for (int p = 0; p < d.num_triangles; ++p) {
             int t = bvh.order[p];
             size_t g = (size_t)tri_off[m] + p;
@@ -465,6 +489,8 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
                 const F3 up = f3(0.0f, 0.0f, 1.0f);
                 for (int k = 0; k < 3; ++k) put3(&ps.tri_vn[12 * g + 4 * k], unit_or(get3(tn + 3 * k), up));
             }
+            // a triangle without "triangle_uvs" reads uv (0, 0)
+            if (uvs && t < d.num_tri_uvs) std::memcpy(&ps.tri_uv[6 * g], d.tri_uvs + 6 * (size_t)t, 6 * sizeof(float));
         }
     }
 

@@ -37,7 +37,14 @@ struct alignas(16) GNode {
 };
 static_assert(sizeof(GNode) == 64, "GNode is 64 bytes");
 
-enum MatFlags : int32_t { MF_EMISSIVE = 1, MF_VOLUME_BOUNDARY = 2, MF_VOLUME = 4 };
+// MF_EMISSIVE: the material's emission at uv (0, 0) is emissive, so its
+// triangles are in the light list (scene_lights.hpp:59-70).  MF_NODES: the
+// albedo or the emission is a node-tree program of the hit's uv (a Checker
+// texture below the Surface link); MF_EM_UV: the emission is such a program,
+// so is_emissive is decided per hit.
+enum MatFlags : int32_t {
+    MF_EMISSIVE = 1, MF_VOLUME_BOUNDARY = 2, MF_VOLUME = 4, MF_NODES = 8, MF_EM_UV = 16
+};
 
 struct alignas(16) GMat {
     float albedo[3];
@@ -48,8 +55,30 @@ struct alignas(16) GMat {
     int32_t flags;
     float emission[3];
     int32_t volume;       // index into volumes, -1 if none
+    int32_t prog_albedo;  // first GNodeOp of the albedo program, -1: albedo above
+    int32_t prog_emission;// first GNodeOp of the emission program, -1: emission above
+    int32_t pad[2];
 };
-static_assert(sizeof(GMat) == 48, "GMat is 48 bytes");
+static_assert(sizeof(GMat) == 64, "GMat is 64 bytes");
+
+// Node-tree programs: postfix code over RGB values, one 16-byte op each.
+// A program leaves one RGB on its stack at NOP_END.  The host constant-folds
+// every subtree that does not depend on uv, so programs remain only for
+// trees with a Checker texture (node_evaluator.cpp:352-401).
+enum NodeOpcode : int32_t {
+    NOP_END = 0,
+    NOP_CONST = 1,    // push (a, b, c)
+    NOP_CHECKER = 2,  // pop color2, color1; push the checker colour at uv * a
+    NOP_MIX = 3,      // pop B, A; push A * (1 - a) + B * a
+    NOP_SCALE = 4,    // pop C; push C * a
+};
+constexpr int32_t kNopSwapped = 0x100;  // binary op: the operands were pushed B first
+constexpr int kNodeStack = 4;           // evaluation slots a program may use
+struct alignas(16) GNodeOp {
+    int32_t op;
+    float a, b, c;
+};
+static_assert(sizeof(GNodeOp) == 16, "GNodeOp is 16 bytes");
 
 struct alignas(16) GLight {
     int32_t type;         // 0 emissive triangle, 1 point, 2 sun, 3 spot, 4 area
@@ -93,6 +122,8 @@ struct PackedScene {
     std::vector<GNode> nodes;
     std::vector<float> tri_v0, tri_e1, tri_e2, tri_fn, tri_vn;  // 4 floats per entry
     std::vector<GMat> mats;
+    std::vector<GNodeOp> node_ops;  // node-tree programs of all materials
+    std::vector<float> tri_uv;      // 6 floats per triangle in BVH order (only with MF_NODES materials)
     std::vector<GVolume> volumes;
     std::vector<float> grid;     // all heterogeneous grids, concatenated
     std::vector<GLight> lights;
@@ -107,12 +138,25 @@ struct PackedScene {
     int32_t max_depth = 0;                   // deepest node path (for the traversal stack)
     bool any_smooth = false;
     bool any_volume = false;
+    bool any_nodes = false;                  // a material with MF_NODES (tri_uv filled)
 };
 
 // Packs the scene; returns false and sets err on invalid input.
 bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_light_desc* lights,
                 int32_t num_lights, const float env_color[3], float env_strength, PackedScene& out,
                 std::string& err);
+
+// A material as getMaterialParams / getEmission see it at one uv
+// (include/integrator/path_tracer.hpp:45-72), compiled from a mesh
+// descriptor: the constant parameters plus programs for albedo and emission.
+struct CompiledMaterial {
+    float albedo[3], emission[3];      // values when the program is empty
+    float metallic, roughness, transmission, ior;  // roughness not yet clamped
+    std::vector<GNodeOp> albedo_prog, emission_prog;  // empty: constant
+};
+bool compile_material(const lucid_mesh_desc& d, CompiledMaterial& out, std::string& err);
+// Runs a program (ending in NOP_END) at uv, as the device does.
+void run_node_program(const GNodeOp* ops, float u, float v, float rgb[3]);
 
 // Camera basis and ray-generation constants of render_tile, computed on the
 // host in IEEE float exactly as set_camera / render_tile do

@@ -400,6 +400,73 @@ __device__ __forceinline__ bool ray_aabb_segment(v3f o, v3f d, float tmax, const
     tout = t1;
     return tout > tin;
 }
+// luminance of an RGB radiance (units/render_units.hpp:1192-1194)
+__device__ __forceinline__ float luminance(v3f c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// ---- node-tree programs (GNodeOp; the host compiler folds everything
+// but Checker textures, node_evaluator.cpp:352-401) ----
+
+// Low bit of floor(x): the reference's floor(u * scale) % 2 with its
+// negative fix-up.  |floor(x)| >= 2^24 is even, and NaN reads as even as the
+// reference's x86 float-to-int conversion (INT_MIN) makes it.
+__device__ __forceinline__ int checker_parity(float x) {
+    const float f = floorf(x);
+    return (fabsf(f) < 16777216.0f) ? ((int)f & 1) : 0;
+}
+
+// Evaluation slots are registers: every access is a per-component select
+// on the slot index, never a dynamically indexed (scratch) array.
+__device__ __forceinline__ v3f pick(bool c, v3f a, v3f b) {
+    return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+struct NodeStack {
+    v3f s[kNodeStack];
+    __device__ __forceinline__ v3f get(int i) const {
+        v3f r = s[0];
+#pragma unroll
+        for (int k = 1; k < kNodeStack; ++k) r = pick(i == k, s[k], r);
+        return r;
+    }
+    __device__ __forceinline__ void put(int i, v3f v) {
+#pragma unroll
+        for (int k = 0; k < kNodeStack; ++k) s[k] = pick(i == k, v, s[k]);
+    }
+};
+
+// The RGB a program leaves at NOP_END, at texture coordinate (u, v).
+__device__ __forceinline__ v3f run_node_program(const GNodeOp* ops, float u, float v) {
+    NodeStack st;
+#pragma unroll
+    for (int k = 0; k < kNodeStack; ++k) st.s[k] = mk(0.0f, 0.0f, 0.0f);
+    int sp = 0;
+    for (;; ++ops) {
+        const GNodeOp o = *ops;
+        const int code = o.op & 0xff;
+        if (code == NOP_END) break;
+        if (code == NOP_CONST) {
+            st.put(sp, mk(o.a, o.b, o.c));
+            ++sp;
+        } else if (code == NOP_SCALE) {
+            st.put(sp - 1, st.get(sp - 1) * o.a);
+        } else {
+            const bool sw = (o.op & kNopSwapped) != 0;
+            const v3f top = st.get(sp - 1), below = st.get(sp - 2);
+            const v3f x = pick(sw, top, below);  // color1 / A
+            const v3f y = pick(sw, below, top);  // color2 / B
+            v3f r;
+            if (code == NOP_CHECKER) {
+                r = pick(checker_parity(u * o.a) == checker_parity(v * o.a), x, y);
+            } else {
+                const float f1 = 1.0f - o.a;
+                r = mk(x.x * f1 + y.x * o.a, x.y * f1 + y.y * o.a, x.z * f1 + y.z * o.a);
+            }
+            --sp;
+            st.put(sp - 1, r);
+        }
+    }
+    return st.s[0];
+}
+
 __device__ __forceinline__ float grid_f32(float x) { return x; }
 __device__ __forceinline__ float grid_f32(__half x) { return __half2float(x); }
 

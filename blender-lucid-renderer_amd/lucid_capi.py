@@ -25,6 +25,19 @@ F3 = C.c_float * 3
 I3 = C.c_int32 * 3
 
 
+SOCK_NONE, SOCK_FLOAT, SOCK_VEC3, SOCK_VEC4 = 0, 1, 2, 3
+
+
+class NodeSocket(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("kind", C.c_int32), ("value", C.c_float * 4), ("linked_node", C.c_char_p),
+                ("linked_socket", C.c_char_p)]
+
+
+class Node(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_char_p), ("inputs", C.POINTER(NodeSocket)),
+                ("num_inputs", C.c_int32), ("outputs", C.POINTER(NodeSocket)), ("num_outputs", C.c_int32)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [("num_vertices", C.c_int32), ("vertices", C.POINTER(C.c_float)), ("num_triangles", C.c_int32),
                 ("indices", C.POINTER(C.c_int32)), ("tri_normals", C.POINTER(C.c_float)),
@@ -32,7 +45,9 @@ class MeshDesc(C.Structure):
                 ("roughness", C.c_float), ("emission", F3), ("transmission", C.c_float), ("ior", C.c_float),
                 ("has_volume", C.c_int32), ("vol_color", F3), ("vol_absorption", F3), ("vol_density", C.c_float),
                 ("vol_anisotropy", C.c_float), ("grid_dims", I3), ("grid_min", F3), ("grid_max", F3),
-                ("grid", C.POINTER(C.c_float)), ("surface_linked", C.c_int32)]
+                ("grid", C.POINTER(C.c_float)), ("surface_linked", C.c_int32), ("use_nodes", C.c_int32),
+                ("nodes", C.POINTER(Node)), ("num_nodes", C.c_int32), ("tri_uvs", C.POINTER(C.c_float)),
+                ("num_tri_uvs", C.c_int32)]
 
 
 class LightDesc(C.Structure):
@@ -153,6 +168,8 @@ SIGNATURES = {
                                            C.POINTER(C.c_uint32)]),
     "lucid_session_async_running": (C.c_int32, [C.c_void_p]),
     "lucid_session_error": (C.c_char_p, [C.c_void_p]),
+    "lucid_material_eval": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.c_char_p, C.c_int32]),
     "lucid_debug_host_trace": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
@@ -203,6 +220,77 @@ def surface_linked(material: dict) -> bool:
     return True
 
 
+def _socket_value(v, sock: NodeSocket):
+    """parseSocketValue (node_evaluator.cpp:47-69): a number is FLOAT, a
+    numeric 3- or 4-array VEC3 / VEC4, anything else NONE."""
+    sock.kind = SOCK_NONE
+    if isinstance(v, bool) or v is None:
+        return
+    if isinstance(v, (int, float)):
+        sock.kind, sock.value[0] = SOCK_FLOAT, float(v)
+    elif isinstance(v, (list, tuple)) and len(v) in (3, 4) and all(
+            isinstance(x, (int, float)) and not isinstance(x, bool) for x in v):
+        sock.kind = SOCK_VEC3 if len(v) == 3 else SOCK_VEC4
+        for k, x in enumerate(v):
+            sock.value[k] = float(x)
+
+
+def node_tree(material: dict, keep: list):
+    """parseNodeTree (node_evaluator.cpp:82-149) of a material dict into a
+    Node array: (array, count, use_nodes).  Each input socket is linked from
+    the last "links" entry that targets it; useNodes = "use_nodes" and a tree
+    with nodes (pybind_renderer.hpp:1381-1384).  `keep` receives every
+    buffer the array points into."""
+    nt = material.get("node_tree")
+    if not isinstance(nt, dict) or not isinstance(nt.get("nodes"), list) or not nt["nodes"]:
+        return None, 0, 0
+    conn = {}
+    for l in nt.get("links") or []:
+        conn[(l["to_node"], l["to_socket"])] = (l["from_node"], l["from_socket"])
+
+    def b(x):
+        bs = str(x).encode()
+        keep.append(bs)
+        return bs
+
+    nodes = (Node * len(nt["nodes"]))()
+    keep.append(nodes)
+    for i, nj in enumerate(nt["nodes"]):
+        name = nj.get("name", "") if isinstance(nj.get("name", ""), str) else ""
+        ntype = nj.get("type", "") if isinstance(nj.get("type", ""), str) else ""
+        nodes[i].name, nodes[i].type = b(name), b(ntype)
+        for field, count in (("inputs", "num_inputs"), ("outputs", "num_outputs")):
+            socks = nj.get(field) if isinstance(nj.get(field), list) else []
+            arr = (NodeSocket * max(1, len(socks)))()
+            keep.append(arr)
+            for k, sj in enumerate(socks):
+                sn = sj.get("name", "") if isinstance(sj.get("name", ""), str) else ""
+                arr[k].name = b(sn)
+                _socket_value(sj.get("default_value"), arr[k])
+                if field == "inputs" and (name, sn) in conn:
+                    fn, fs = conn[(name, sn)]
+                    arr[k].linked_node, arr[k].linked_socket = b(fn), b(fs)
+            setattr(nodes[i], field, C.cast(arr, C.POINTER(NodeSocket)))
+            setattr(nodes[i], count, len(socks))
+    return nodes, len(nt["nodes"]), 1 if material.get("use_nodes", False) else 0
+
+
+def material_eval(mesh_or_material: dict, uv) -> np.ndarray:
+    """lucid_material_eval: (n, 10) floats -- albedo RGB, emission RGB,
+    metallic, roughness, transmission, ior -- of a mesh's material (or a bare
+    material dict) at the (n, 2) texture coordinates uv.  Host only."""
+    mesh = mesh_or_material if "material" in mesh_or_material else {"material": mesh_or_material}
+    d = SceneDescs({"meshes": [dict(mesh, vertices=[], triangles=[])]})
+    uv = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1, 2))
+    out = np.zeros((uv.shape[0], 10), np.float32)
+    err = C.create_string_buffer(256)
+    rc = load().lucid_material_eval(C.byref(d.meshes[0]), uv.shape[0], uv.ctypes.data_as(C.POINTER(C.c_float)),
+                                    out.ctypes.data_as(C.POINTER(C.c_float)), err, 256)
+    if rc != 0:
+        raise ValueError(err.value.decode())
+    return out
+
+
 class SceneDescs:
     """POD descriptors of a reference-schema scene dict, owning the arrays."""
 
@@ -232,6 +320,14 @@ class SceneDescs:
             d.transmission = float(mat.get("transmission", 0.0))
             d.ior = float(mat.get("ior", 1.45))
             d.surface_linked = 1 if surface_linked(mat) else 0
+            nodes, n_nodes, use = node_tree(mat, self._keep)
+            if n_nodes:
+                d.nodes, d.num_nodes, d.use_nodes = nodes, n_nodes, use
+            tu = m.get("triangle_uvs")
+            if tu is not None and len(tu):
+                tu = np.ascontiguousarray(np.asarray(tu, np.float32).reshape(-1, 6))
+                self._keep.append(tu)
+                d.tri_uvs, d.num_tri_uvs = tu.ctypes.data_as(C.POINTER(C.c_float)), tu.shape[0]
             vol = mat.get("volume")
             if isinstance(vol, dict):
                 d.has_volume = 1

@@ -213,6 +213,103 @@ def boundary_node_tree():
                        "to_node": "Material Output", "to_socket": "Volume"}]}
 
 
+# ---- node-tree materials (the scene-JSON node_tree schema of
+# scene_export.py:104-181; evaluated as src/node_evaluator.cpp does) ----
+
+def _rgba(c):
+    return [float(x) for x in c] + [1.0]
+
+
+def _node(name, typ, inputs=(), outputs=()):
+    return {"name": name, "type": typ, "inputs": [dict(x) for x in inputs], "outputs": [dict(x) for x in outputs]}
+
+
+def _link(from_node, from_socket, to_node, to_socket):
+    return {"from_node": from_node, "from_socket": from_socket, "to_node": to_node, "to_socket": to_socket}
+
+
+def _output_node():
+    return _node("Material Output", "ShaderNodeOutputMaterial",
+                 [{"name": "Surface"}, {"name": "Volume"}, {"name": "Displacement"}])
+
+
+def _principled_node(name="Principled BSDF", base=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5,
+                     emission=(0.0, 0.0, 0.0), strength=0.0):
+    return _node(name, "ShaderNodeBsdfPrincipled", [
+        {"name": "Base Color", "default_value": _rgba(base)}, {"name": "Metallic", "default_value": metallic},
+        {"name": "Roughness", "default_value": roughness}, {"name": "IOR", "default_value": 1.45},
+        {"name": "Transmission Weight", "default_value": 0.0},
+        {"name": "Emission Color", "default_value": _rgba(emission)},
+        {"name": "Emission Strength", "default_value": strength}], [{"name": "BSDF"}])
+
+
+def _checker_node(name, c1, c2, scale):
+    return _node(name, "ShaderNodeTexChecker", [
+        {"name": "Vector"}, {"name": "Color1", "default_value": _rgba(c1)},
+        {"name": "Color2", "default_value": _rgba(c2)}, {"name": "Scale", "default_value": float(scale)}],
+        [{"name": "Color"}, {"name": "Fac"}])
+
+
+def _quad_uvs(q):
+    """Per-triangle UVs of _quad's two triangles from its corner UVs q[0..3]."""
+    return [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+
+
+def textured_box(seed: int = 6, panel: bool = True) -> dict:
+    """The closed box with node-tree materials: a checker floor (negative UVs
+    too), a back wall mixing a checker (one colour from an RGB node) with a
+    constant, a left wall whose Principled tree folds to constants, a box
+    whose tree is present but unused (use_nodes false), and (with `panel`)
+    an emissive checker panel on the right wall, lit by the ceiling area
+    light."""
+    rng = np.random.default_rng(seed)
+    scene = closed_box(seed)
+    m = scene["meshes"]
+    floor = {"use_nodes": True, "node_tree": {
+        "nodes": [_output_node(), _principled_node(roughness=0.6),
+                  _checker_node("Checker", (0.85, 0.8, 0.7), (0.1, 0.25, 0.15), 3.0)],
+        "links": [_link("Principled BSDF", "BSDF", "Material Output", "Surface"),
+                  _link("Checker", "Color", "Principled BSDF", "Base Color")]}}
+    m[0]["material"] = dict(m[0]["material"], **floor)
+    m[0]["triangle_uvs"] = _quad_uvs([[-1.0, -1.0], [-1.0, 1.0], [1.0, 1.0], [1.0, -1.0]])
+    back = {"use_nodes": True, "node_tree": {
+        "nodes": [_output_node(),
+                  _node("Diffuse BSDF", "ShaderNodeBsdfDiffuse", [{"name": "Color", "default_value": _rgba((0.5,) * 3)},
+                                                                  {"name": "Roughness", "default_value": 0.0}],
+                        [{"name": "BSDF"}]),
+                  _node("Mix", "ShaderNodeMixRGB", [{"name": "Fac", "default_value": 0.35},
+                                                    {"name": "A", "default_value": _rgba((0, 0, 0))},
+                                                    {"name": "B", "default_value": _rgba((0.2, 0.4, 0.8))}],
+                        [{"name": "Color"}]),
+                  _checker_node("Checker", (0.8, 0.8, 0.8), (0.05, 0.05, 0.05), 5.0),
+                  _node("RGB", "ShaderNodeRGB", [], [{"name": "Color", "default_value": _rgba((0.9, 0.6, 0.2))}])],
+        "links": [_link("Diffuse BSDF", "BSDF", "Material Output", "Surface"),
+                  _link("Mix", "Color", "Diffuse BSDF", "Color"), _link("Checker", "Color", "Mix", "A"),
+                  _link("RGB", "Color", "Checker", "Color1")]}}
+    m[2]["material"] = dict(m[2]["material"], **back)
+    m[2]["triangle_uvs"] = _quad_uvs([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    left = {"use_nodes": True, "node_tree": {
+        "nodes": [_output_node(), _principled_node(base=tuple(rng.uniform(0.2, 0.8, 3)), metallic=0.3, roughness=0.25)],
+        "links": [_link("Principled BSDF", "BSDF", "Material Output", "Surface")]}}
+    m[3]["material"] = dict(m[3]["material"], **left)
+    unused = {"use_nodes": False, "node_tree": left["node_tree"]}
+    m[5]["material"] = dict(m[5]["material"], **unused)
+    panel_v, panel_t = _quad([0.99, 0.5, -0.6], [0.99, 0.5, 0.2], [0.99, 1.3, 0.2], [0.99, 1.3, -0.6])
+    panel_mat = {"base_color": [0.8, 0.8, 0.8], "use_nodes": True, "node_tree": {
+        "nodes": [_output_node(),
+                  _node("Emission", "ShaderNodeEmission", [{"name": "Color", "default_value": _rgba((1, 1, 1))},
+                                                           {"name": "Strength", "default_value": 2.0}],
+                        [{"name": "Emission"}]),
+                  _checker_node("Checker", (6.0, 5.0, 4.0), (0.0, 0.0, 0.0), 2.0)],
+        "links": [_link("Emission", "Emission", "Material Output", "Surface"),
+                  _link("Checker", "Color", "Emission", "Color")]}}
+    pm = _mesh(panel_v, panel_t, panel_mat)
+    pm["triangle_uvs"] = _quad_uvs([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    if panel:
+        m.append(pm)
+    return scene
+
+
 def smoke_box(sidecar_path: str, grid_n: int = 32, seed: int = 4, density: float = 6.0, anisotropy: float = 0.3,
               albedo: float = 0.9) -> dict:
     """The closed box with a gridded smoke medium (fBm density, albedo 0.9)

@@ -25,6 +25,9 @@
  *                        bit-exact primary-hit check
  *   lucid_render_samples traceMIS & co. per sample (include/integrator/
  *                        path_tracer.hpp:128-835), per-sample radiance out
+ *   lucid_material_eval  getMaterialParams / getEmission with the node-tree
+ *                        evaluator (include/integrator/path_tracer.hpp:45-72,
+ *                        src/node_evaluator.cpp:168-740), on the host
  */
 #ifndef LUCID_HIP_H
 #define LUCID_HIP_H
@@ -66,6 +69,37 @@ extern "C" {
 typedef struct lucid_ctx lucid_ctx;
 typedef struct lucid_scene lucid_scene;
 
+/* Node-tree socket values (include/core/material.hpp:36-70, parsed by
+   parseSocketValue, src/node_evaluator.cpp:47-69): a JSON number is FLOAT, a
+   3-array VEC3, a 4-array VEC4; booleans, strings, other arrays and absent
+   values are NONE (the evaluator reads none of them). */
+#define LUCID_SOCK_NONE 0
+#define LUCID_SOCK_FLOAT 1
+#define LUCID_SOCK_VEC3 2
+#define LUCID_SOCK_VEC4 3
+
+/* One socket of a node (NodeSocket, include/core/material.hpp:75-84). The
+   caller resolves the tree's "links" as parseNodeTree does
+   (src/node_evaluator.cpp:87-96, the last link to a socket wins): an input
+   linked from (node, socket) carries both names, an unlinked one NULL. */
+typedef struct lucid_node_socket {
+    const char* name;
+    int32_t kind;              /* LUCID_SOCK_* */
+    float value[4];            /* FLOAT: value[0]; VEC3: [0..2]; VEC4: [0..3] */
+    const char* linked_node;   /* NULL when not linked */
+    const char* linked_socket;
+} lucid_node_socket;
+
+/* One node of a material's "node_tree" (MaterialNode, material.hpp:89-105). */
+typedef struct lucid_node {
+    const char* name;
+    const char* type;          /* Blender bl_idname, e.g. "ShaderNodeBsdfPrincipled" */
+    const lucid_node_socket* inputs;
+    int32_t num_inputs;
+    const lucid_node_socket* outputs;
+    int32_t num_outputs;
+} lucid_node;
+
 /* One mesh of the scene JSON "meshes" array (pybind_renderer.hpp:1266-1507). */
 typedef struct lucid_mesh_desc {
     int32_t num_vertices;
@@ -86,6 +120,15 @@ typedef struct lucid_mesh_desc {
     float grid_min[3], grid_max[3];
     const float* grid;        /* dims product floats, or NULL */
     int32_t surface_linked;   /* Material Output Surface linked (0 + volume => boundary-only) */
+    /* Node-tree material (pybind_renderer.hpp:1381-1384): when use_nodes is
+       set and num_nodes > 0 the tree replaces base_color, metallic,
+       roughness, transmission, ior and emission as getMaterialParams /
+       getEmission do (include/integrator/path_tracer.hpp:45-72). */
+    int32_t use_nodes;        /* "use_nodes" */
+    const lucid_node* nodes;  /* "node_tree" "nodes", in JSON order */
+    int32_t num_nodes;
+    const float* tri_uvs;     /* "triangle_uvs": 6 floats (u0 v0 u1 v1 u2 v2) per triangle, or NULL */
+    int32_t num_tri_uvs;      /* triangles [0, num_tri_uvs) carry UVs; the rest read uv (0, 0) */
 } lucid_mesh_desc;
 
 /* One entry of "native_lights" (pybind_renderer.hpp:1544-1746); the library
@@ -184,6 +227,16 @@ int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera*
 int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
                        const lucid_render_params* params, int32_t* mesh, int32_t* tri, float* u, float* v,
                        float* t);
+
+/* getMaterialParams + getEmission (include/integrator/path_tracer.hpp:45-72,
+   with the node tree of src/node_evaluator.cpp:168-740) of one mesh's
+   material at `count` texture coordinates (uv[2i], uv[2i+1]), through the
+   same compiled node programs the device runs.  Writes 10 floats per
+   coordinate: albedo RGB, emission RGB, metallic, roughness (clamped to
+   >= 0.01), transmission, ior.  Host only: no device is needed.  On
+   LUCID_ERR_INVALID the reason is written to err (err_len bytes). */
+int lucid_material_eval(const lucid_mesh_desc* mesh, int32_t count, const float* uv, float* out, char* err,
+                        int32_t err_len);
 
 /* Cooperative cancel, callable from any thread; polled between wavefront
    batches.  render entry points clear the flag on entry. */

@@ -259,6 +259,8 @@ struct Tri {
     V3 fn = v3(0, 1, 0), n0 = v3(0, 1, 0), n1 = v3(0, 1, 0), n2 = v3(0, 1, 0);
     V3 centroid;
     bool smooth = false;
+    float uv0[2] = {0, 0}, uv1[2] = {0, 0}, uv2[2] = {0, 0};  // core/scene.hpp:36
+    bool has_uv = false;
 };
 struct Node {
     V3 bmin, bmax;
@@ -274,6 +276,45 @@ struct Volume {
     bool present() const { return density > 0.0f || has_grid(); }
     bool has_grid() const { return !grid.empty() && dims[0] > 0 && dims[1] > 0 && dims[2] > 0; }
 };
+// Shader-node graph, core/material.hpp:29-127 (SocketValue, NodeSocket,
+// MaterialNode, NodeTree), filled as parseNodeTree does
+// (src/node_evaluator.cpp:82-149).
+enum SockType { SV_FLOAT, SV_VEC3, SV_VEC4, SV_NONE };
+struct SockVal {
+    SockType type = SV_NONE;
+    float f = 0.0f;
+    V3 v3_ = v3(0, 0, 0);   // VEC3
+    V3 v4 = v3(0, 0, 0);    // VEC4 colour part
+};
+struct NodeSocket {
+    std::string name;
+    SockVal dv;
+    bool is_linked = false;
+    std::string linked_node, linked_socket;
+};
+struct MatNode {
+    std::string name, type;
+    std::vector<NodeSocket> inputs, outputs;
+    const NodeSocket* find_input(const std::string& n) const {
+        for (const NodeSocket& x : inputs)
+            if (x.name == n) return &x;
+        return nullptr;
+    }
+};
+struct NodeTree {
+    std::vector<MatNode> nodes;
+    bool valid = false;
+    const MatNode* find_node(const std::string& n) const {
+        for (const MatNode& x : nodes)
+            if (x.name == n) return &x;
+        return nullptr;
+    }
+    const MatNode* find_output() const {
+        for (const MatNode& x : nodes)
+            if (x.type == "ShaderNodeOutputMaterial") return &x;
+        return nullptr;
+    }
+};
 struct Material {
     V3 albedo = v3(0.8f, 0.8f, 0.8f);
     float metallic = 0.0f, roughness = 0.5f;
@@ -281,6 +322,8 @@ struct Material {
     float transmission = 0.0f, ior = 1.45f;
     Volume vol;
     bool volume_boundary_only = false;
+    NodeTree tree;
+    bool use_nodes = false;   // "use_nodes" && tree.valid
 };
 struct Mesh {
     std::vector<V3> verts;
@@ -414,6 +457,8 @@ struct Hit {
     int mesh = -1, tri = -1;   // tri = original triangle index
     int bvh_pos = -1;          // position in the mesh's triIndices (tie key)
     float u = 0.0f, v = 0.0f;
+    float uv[2] = {0.0f, 0.0f};  // texture coordinate; kept from an earlier
+    bool has_uv = false;         // candidate when the closest triangle has none
 };
 
 // traverseBVH, geometry/bvh.hpp:34-142
@@ -449,6 +494,15 @@ static void traverse_bvh(const Mesh& m, const Ray& r, V3 id, Hit& res, int mesh_
                     res.bvh_pos = nd.tri_start + i;
                     res.u = u;
                     res.v = v;
+                    // bvh.hpp:121-128: uv0 * w + uv1 * u + uv2 * v, left unchanged
+                    // for a triangle without UVs
+                    if (tr.has_uv) {
+                        res.uv[0] = tr.uv0[0] * w + tr.uv1[0] * u + tr.uv2[0] * v;
+                        res.uv[1] = tr.uv0[1] * w + tr.uv1[1] * u + tr.uv2[1] * v;
+                        res.has_uv = true;
+                    } else {
+                        res.has_uv = false;
+                    }
                 }
             }
         } else {
@@ -558,16 +612,182 @@ struct MatParams {  // bsdf/bsdf.hpp:24-37
     V3 albedo;
     float metallic, roughness, transmission, ior;
 };
-// getMaterialParams, path_tracer.hpp:45-65 (constant materials, no node trees)
-static inline MatParams mat_params(const Material& m) {
+// getMaterialParams, path_tracer.hpp:45-65 (defined after the node-tree
+// evaluator below)
+static inline MatParams mat_params(const Material& m, float u = 0.0f, float v = 0.0f);
+static inline float lum_f(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+static inline bool is_emissive(V3 e) { return lum_f(e) > 1e-6f; }  // render_units.hpp:1197-1202
+
+// ---- node-tree evaluation, src/node_evaluator.cpp:168-740 ----
+static const V3 NODE_GRAY = {0.8f, 0.8f, 0.8f};
+// A socket's colour: VEC4 -> v4, VEC3 -> v3; false for other types.
+static bool sock_color(const NodeSocket* s, V3& c) {
+    if (s->dv.type == SV_VEC4) { c = s->dv.v4; return true; }
+    if (s->dv.type == SV_VEC3) { c = s->dv.v3_; return true; }
+    return false;
+}
+// evaluateNode, node_evaluator.cpp:168-415.  The reference recurses without
+// a bound; the depth guard only keeps a cyclic test tree from overflowing
+// this process's stack.
+static V3 eval_node(const NodeTree& t, const std::string& name, const std::string& sock, float u, float v,
+                    int depth = 0) {
+    const MatNode* n = t.find_node(name);
+    if (!n || depth > 4096) return NODE_GRAY;
+    V3 c;
+    if (n->type == "ShaderNodeBsdfPrincipled") {
+        if (sock == "BSDF") {
+            const NodeSocket* bc = n->find_input("Base Color");
+            if (!bc) return NODE_GRAY;
+            if (bc->is_linked) return eval_node(t, bc->linked_node, bc->linked_socket, u, v, depth + 1);
+            return sock_color(bc, c) ? c : NODE_GRAY;
+        } else if (sock == "Base Color") {
+            const NodeSocket* bc = n->find_input("Base Color");
+            if (bc && !bc->is_linked && sock_color(bc, c)) return c;
+            return NODE_GRAY;
+        }
+    } else if (n->type == "ShaderNodeEmission") {
+        if (sock == "Emission") {
+            const NodeSocket* cs = n->find_input("Color");
+            const NodeSocket* ss = n->find_input("Strength");
+            V3 color = v3(1.0f, 1.0f, 1.0f);
+            float strength = 1.0f;
+            if (cs) {
+                if (cs->is_linked) color = eval_node(t, cs->linked_node, cs->linked_socket, u, v, depth + 1);
+                else if (sock_color(cs, c)) color = c;
+            }
+            if (ss && !ss->is_linked && ss->dv.type == SV_FLOAT) strength = ss->dv.f;
+            return mul(color, strength);
+        }
+    } else if (n->type == "ShaderNodeBsdfDiffuse") {
+        if (sock == "BSDF") {
+            const NodeSocket* cs = n->find_input("Color");
+            if (!cs) return NODE_GRAY;
+            if (cs->is_linked) return eval_node(t, cs->linked_node, cs->linked_socket, u, v, depth + 1);
+            return sock_color(cs, c) ? c : NODE_GRAY;
+        }
+    } else if (n->type == "ShaderNodeRGB") {
+        for (const NodeSocket& o : n->outputs)
+            if (o.name == "Color" && o.dv.type != SV_NONE && sock_color(&o, c)) return c;
+        return v3(1.0f, 1.0f, 1.0f);
+    } else if (n->type == "ShaderNodeMix" || n->type == "ShaderNodeMixRGB") {
+        const NodeSocket* fs = n->find_input("Fac");
+        const NodeSocket* as = n->find_input("A");
+        const NodeSocket* bs = n->find_input("B");
+        float fac = 0.5f;
+        V3 a = v3(0, 0, 0), b = v3(1.0f, 1.0f, 1.0f);
+        if (fs && !fs->is_linked && fs->dv.type == SV_FLOAT) fac = fs->dv.f;
+        if (as) {
+            if (as->is_linked) a = eval_node(t, as->linked_node, as->linked_socket, u, v, depth + 1);
+            else if (sock_color(as, c)) a = c;
+        }
+        if (bs) {
+            if (bs->is_linked) b = eval_node(t, bs->linked_node, bs->linked_socket, u, v, depth + 1);
+            else if (sock_color(bs, c)) b = c;
+        }
+        return add(mul(a, 1.0f - fac), mul(b, fac));
+    } else if (n->type == "ShaderNodeTexChecker") {
+        const NodeSocket* c1s = n->find_input("Color1");
+        const NodeSocket* c2s = n->find_input("Color2");
+        const NodeSocket* ss = n->find_input("Scale");
+        V3 c1 = v3(0.8f, 0.8f, 0.8f), c2 = v3(0.2f, 0.2f, 0.2f);
+        float scale = 5.0f;
+        if (c1s) {
+            if (c1s->is_linked) c1 = eval_node(t, c1s->linked_node, c1s->linked_socket, u, v, depth + 1);
+            else if (sock_color(c1s, c)) c1 = c;
+        }
+        if (c2s) {
+            if (c2s->is_linked) c2 = eval_node(t, c2s->linked_node, c2s->linked_socket, u, v, depth + 1);
+            else if (sock_color(c2s, c)) c2 = c;
+        }
+        if (ss && !ss->is_linked && ss->dv.type == SV_FLOAT) scale = ss->dv.f;
+        float fu = u * scale, fv = v * scale;
+        // static_cast<int>(std::floor(x)) as x86 computes it: out-of-range
+        // and NaN convert to INT_MIN
+        auto to_int = [](float x) -> long long {
+            float f = std::floor(x);
+            return (f >= -2147483648.0f && f < 2147483648.0f) ? (long long)f : -2147483648LL;
+        };
+        long long cu = to_int(fu) % 2, cv = to_int(fv) % 2;
+        if (cu < 0) cu += 2;
+        if (cv < 0) cv += 2;
+        return ((cu + cv) % 2 == 0) ? c1 : c2;
+    } else if (n->type == "ShaderNodeOutputMaterial") {
+        const NodeSocket* ss = n->find_input("Surface");
+        if (ss && ss->is_linked) return eval_node(t, ss->linked_node, ss->linked_socket, u, v, depth + 1);
+    }
+    return NODE_GRAY;
+}
+// The Principled node the Material Output's linked Surface comes from.
+static const MatNode* surface_principled(const NodeTree& t, const NodeSocket** surf = nullptr) {
+    const MatNode* out = t.find_output();
+    if (!out) return nullptr;
+    const NodeSocket* s = out->find_input("Surface");
+    if (!s || !s->is_linked) return nullptr;
+    if (surf) *surf = s;
+    const MatNode* sh = t.find_node(s->linked_node);
+    return (sh && sh->type == "ShaderNodeBsdfPrincipled") ? sh : nullptr;
+}
+// getAlbedoFromNodeTree, :436-478
+static V3 tree_albedo(const NodeTree& t, float u, float v) {
+    if (!t.valid) return NODE_GRAY;
+    const MatNode* out = t.find_output();
+    if (!out) return NODE_GRAY;
+    const NodeSocket* s = out->find_input("Surface");
+    if (!s || !s->is_linked) return NODE_GRAY;
+    return eval_node(t, s->linked_node, s->linked_socket, u, v);
+}
+// getEmissionFromNodeTree, :501-565
+static V3 tree_emission(const NodeTree& t, float u, float v) {
+    if (!t.valid) return v3(0, 0, 0);
+    const MatNode* out = t.find_output();
+    if (!out) return v3(0, 0, 0);
+    const NodeSocket* s = out->find_input("Surface");
+    if (!s || !s->is_linked) return v3(0, 0, 0);
+    const MatNode* sh = t.find_node(s->linked_node);
+    if (sh && sh->type == "ShaderNodeEmission") return eval_node(t, s->linked_node, "Emission", u, v);
+    if (sh && sh->type == "ShaderNodeBsdfPrincipled") {
+        const NodeSocket* es = sh->find_input("Emission Color");
+        if (!es) es = sh->find_input("Emission");
+        if (es) {
+            const NodeSocket* st = sh->find_input("Emission Strength");
+            float strength = (st && st->dv.type == SV_FLOAT) ? st->dv.f : 0.0f;
+            if (es->is_linked) return mul(eval_node(t, es->linked_node, es->linked_socket, u, v), strength);
+            if (es->dv.type == SV_VEC4) return mul(es->dv.v4, strength);
+            if (es->dv.type == SV_VEC3) return mul(es->dv.v3_, strength);
+        }
+    }
+    return v3(0, 0, 0);
+}
+// getEmission, path_tracer.hpp:67-72
+static inline V3 mat_emission(const Material& m, float u = 0.0f, float v = 0.0f);
+// get{Transmission,IOR,Metallic,Roughness}FromNodeTree, :579-740
+static float tree_scalar(const NodeTree& t, const char* name, const char* alt, float dflt) {
+    if (!t.valid) return dflt;
+    const MatNode* sh = surface_principled(t);
+    if (!sh) return dflt;
+    const NodeSocket* s = sh->find_input(name);
+    if (!s && alt) s = sh->find_input(alt);
+    if (s && !s->is_linked && s->dv.type == SV_FLOAT) return s->dv.f;
+    return dflt;
+}
+static inline MatParams mat_params(const Material& m, float u, float v) {
     MatParams p;
     p.albedo = m.albedo; p.metallic = m.metallic; p.roughness = m.roughness;
     p.transmission = m.transmission; p.ior = m.ior;
+    if (m.use_nodes && m.tree.valid) {
+        p.albedo = tree_albedo(m.tree, u, v);
+        p.metallic = tree_scalar(m.tree, "Metallic", nullptr, 0.0f);
+        p.roughness = tree_scalar(m.tree, "Roughness", nullptr, 0.5f);
+        p.transmission = tree_scalar(m.tree, "Transmission", "Transmission Weight", 0.0f);
+        p.ior = tree_scalar(m.tree, "IOR", nullptr, 1.45f);
+    }
     p.roughness = fmax_std(p.roughness, MIN_ROUGH);
     return p;
 }
-static inline float lum_f(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
-static inline bool is_emissive(V3 e) { return lum_f(e) > 1e-6f; }  // render_units.hpp:1197-1202
+static inline V3 mat_emission(const Material& m, float u, float v) {
+    if (m.use_nodes && m.tree.valid) return tree_emission(m.tree, u, v);
+    return m.emission;
+}
 
 // fresnel.hpp:40-48
 static inline V3 fresnel_schlick_color(float c, V3 f0) {
@@ -977,13 +1197,15 @@ static void build_scene_lights(Scene& s) {
     for (size_t mi = 0; mi < s.meshes.size(); ++mi) {
         const Mesh& m = s.meshes[mi];
         s.tri_light[mi].assign(m.tris.size(), -1);
-        if (!is_emissive(m.mat.emission)) continue;
+        // node-tree emission at uv (0, 0), scene_lights.hpp:64-67
+        const V3 em = mat_emission(m.mat, 0.0f, 0.0f);
+        if (!is_emissive(em)) continue;
         for (size_t ti = 0; ti < m.tris.size(); ++ti) {
             const Tri& t = m.tris[ti];
             Light l;
             l.v0 = m.verts[t.i0]; l.v1 = m.verts[t.i1]; l.v2 = m.verts[t.i2];
             l.normal = t.fn;
-            l.emission = m.mat.emission;
+            l.emission = em;
             l.area = tri_area(l.v0, l.v1, l.v2);
             l.mesh_index = (int)mi;
             l.tri_index = (int)ti;
@@ -1216,8 +1438,8 @@ static V3 trace_simple(const Scene& s, Ray ray, int max_depth, Rng& rng, SampleO
         so.rays_closest++;
         if (lh.hit && (!hit.hit || lh.t < hit.t)) { V3 c = cmul(lh.emission, thr); acc(result, c); acc(so.strat[0], c); break; }
         if (!hit.hit) { V3 c = cmul(env_color(s), thr); acc(result, c); acc(so.strat[0], c); break; }
-        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
-        V3 em = s.meshes[hit.mesh].mat.emission;
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
+        V3 em = mat_emission(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
         { V3 c = cmul(em, thr); acc(result, c); acc(so.strat[0], c); }
         V3 n = hit.normal, wo = neg(ray.d);
         bool front = dot(wo, n) > 0;
@@ -1254,8 +1476,8 @@ static V3 trace_nee(const Scene& s, Ray ray, int max_depth, Rng& rng, SampleOut&
         so.rays_closest++;
         if (lh.hit && (!hit.hit || lh.t < hit.t)) { V3 c = cmul(lh.emission, thr); acc(result, c); acc(so.strat[0], c); break; }
         if (!hit.hit) { V3 c = cmul(env_color(s), thr); acc(result, c); acc(so.strat[0], c); break; }
-        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
-        V3 em = s.meshes[hit.mesh].mat.emission;
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
+        V3 em = mat_emission(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
         V3 n = hit.normal, wo = neg(ray.d);
         bool front = dot(wo, n) > 0;
         V3 sn = n;
@@ -1391,8 +1613,8 @@ static V3 trace_mis(const Scene& s, Ray ray, int max_depth, Rng& rng, SampleOut&
             break;
         }
         if (!hit.hit) { V3 c = cmul(env_color(s), thr); acc(result, c); acc(so.strat[0], c); break; }
-        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
-        V3 em = s.meshes[hit.mesh].mat.emission;
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
+        V3 em = mat_emission(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
         V3 n = hit.normal, wo = neg(ray.d);
         bool front = dot(wo, n) > 0;
         V3 sn = n;
@@ -1481,8 +1703,8 @@ static V3 trace_volume_simple(const Scene& s, Ray ray, int max_depth, Rng& rng, 
         thr = cmul(thr, transmittance_seg(s, ray, ts));
         if (lh.hit && (!hit.hit || lh.t < hit.t)) { V3 c = cmul(lh.emission, thr); acc(result, c); acc(so.strat[0], c); break; }
         if (!hit.hit) { V3 c = cmul(env_color(s), thr); acc(result, c); acc(so.strat[0], c); break; }
-        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
-        V3 em = s.meshes[hit.mesh].mat.emission;
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
+        V3 em = mat_emission(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
         { V3 c = cmul(em, thr); acc(result, c); acc(so.strat[0], c); }
         V3 n = hit.normal, wo = neg(ray.d);
         bool front = dot(wo, n) > 0;
@@ -1676,8 +1898,8 @@ static V3 trace_diag(const Scene& s, int kind, Ray ray, int max_depth, Rng& rng,
             acc(result, c);
             break;
         }
-        MatParams mat = mat_params(s.meshes[hit.mesh].mat);
-        V3 em = s.meshes[hit.mesh].mat.emission;
+        MatParams mat = mat_params(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
+        V3 em = mat_emission(s.meshes[hit.mesh].mat, hit.uv[0], hit.uv[1]);
         uint8_t bt = d_classify(mat);
         bool dl = d_is_delta(mat);
         bool has_em = is_emissive(em);
@@ -1946,6 +2168,28 @@ struct orc_mesh_desc {
     float grid_min[3], grid_max[3];
     const float* grid;           // dims[0]*dims[1]*dims[2] floats, x fastest, or NULL
     int32_t surface_linked;      // 0 => volume_boundary_only when a volume is present
+    int32_t use_nodes;           // "use_nodes"
+    const struct orc_node* nodes;  // "node_tree" nodes, links resolved per input
+    int32_t num_nodes;
+    const float* tri_uvs;        // 6 per triangle or NULL
+    int32_t num_tri_uvs;
+};
+// A node-tree socket: kind 0 none, 1 float, 2 vec3, 3 vec4 (parseSocketValue,
+// node_evaluator.cpp:47-69); linked_node NULL when not linked.
+struct orc_node_socket {
+    const char* name;
+    int32_t kind;
+    float value[4];
+    const char* linked_node;
+    const char* linked_socket;
+};
+struct orc_node {
+    const char* name;
+    const char* type;
+    const orc_node_socket* inputs;
+    int32_t num_inputs;
+    const orc_node_socket* outputs;
+    int32_t num_outputs;
 };
 
 struct orc_light_desc {
@@ -1991,8 +2235,48 @@ int orc_scene_add_mesh(void* p, const orc_mesh_desc* d) {
         } else {
             t.smooth = false;
         }
+        if (d->tri_uvs && i < d->num_tri_uvs) {  // pybind_renderer.hpp:1330-1337
+            const float* q = d->tri_uvs + 6 * (size_t)i;
+            t.uv0[0] = q[0]; t.uv0[1] = q[1]; t.uv1[0] = q[2]; t.uv1[1] = q[3]; t.uv2[0] = q[4]; t.uv2[1] = q[5];
+            t.has_uv = true;
+        }
     }
     Material& mt = m.mat;
+    // parseNodeTree (node_evaluator.cpp:82-149), useNodes (pybind_renderer.hpp:1381-1384)
+    auto str = [](const char* c) { return std::string(c ? c : ""); };
+    auto sockval = [](const orc_node_socket& q) {
+        SockVal sv;
+        if (q.kind == 1) { sv.type = SV_FLOAT; sv.f = q.value[0]; }
+        else if (q.kind == 2) { sv.type = SV_VEC3; sv.v3_ = v3(q.value[0], q.value[1], q.value[2]); }
+        else if (q.kind == 3) { sv.type = SV_VEC4; sv.v4 = v3(q.value[0], q.value[1], q.value[2]); }
+        return sv;
+    };
+    for (int i = 0; d->nodes && i < d->num_nodes; ++i) {
+        const orc_node& nj = d->nodes[i];
+        MatNode nd;
+        nd.name = str(nj.name);
+        nd.type = str(nj.type);
+        for (int k = 0; k < nj.num_inputs; ++k) {
+            NodeSocket so;
+            so.name = str(nj.inputs[k].name);
+            so.dv = sockval(nj.inputs[k]);
+            if (nj.inputs[k].linked_node) {
+                so.is_linked = true;
+                so.linked_node = str(nj.inputs[k].linked_node);
+                so.linked_socket = str(nj.inputs[k].linked_socket);
+            }
+            nd.inputs.push_back(so);
+        }
+        for (int k = 0; k < nj.num_outputs; ++k) {
+            NodeSocket so;
+            so.name = str(nj.outputs[k].name);
+            so.dv = sockval(nj.outputs[k]);
+            nd.outputs.push_back(so);
+        }
+        mt.tree.nodes.push_back(nd);
+    }
+    mt.tree.valid = !mt.tree.nodes.empty();
+    mt.use_nodes = d->use_nodes != 0 && mt.tree.valid;
     mt.albedo = v3(d->base_color[0], d->base_color[1], d->base_color[2]);
     mt.metallic = d->metallic; mt.roughness = d->roughness;
     mt.emission = v3(d->emission[0], d->emission[1], d->emission[2]);
@@ -2246,6 +2530,24 @@ int orc_primary_hits(void* p, int tx, int ty, int tw, int th, int fw, int fh, in
     return 0;
 }
 
+// getMaterialParams + getEmission (path_tracer.hpp:45-72) of mesh `mi` at n
+// texture coordinates: albedo RGB, emission RGB, metallic, roughness,
+// transmission, ior per coordinate.
+int orc_material_eval(void* p, int mi, int n, const float* uv, float* out) {
+    const Scene& s = *static_cast<Scene*>(p);
+    if (mi < 0 || mi >= (int)s.meshes.size()) return -1;
+    const Material& m = s.meshes[mi].mat;
+    for (int i = 0; i < n; ++i) {
+        MatParams mp = mat_params(m, uv[2 * i], uv[2 * i + 1]);
+        V3 em = mat_emission(m, uv[2 * i], uv[2 * i + 1]);
+        float* o = out + 10 * (size_t)i;
+        o[0] = mp.albedo.x; o[1] = mp.albedo.y; o[2] = mp.albedo.z;
+        o[3] = em.x; o[4] = em.y; o[5] = em.z;
+        o[6] = mp.metallic; o[7] = mp.roughness; o[8] = mp.transmission; o[9] = mp.ior;
+    }
+    return 0;
+}
+
 // intersectScene on explicit rays (closest hit); occluded-only when any_hit.
 int orc_intersect(void* p, int n, const float* o, const float* d, const float* tmin, const float* tmax,
                   int32_t* mesh, int32_t* tri, int32_t* pos, float* u, float* v, float* t) {
@@ -2260,6 +2562,21 @@ int orc_intersect(void* p, int n, const float* o, const float* d, const float* t
         tri[i] = h.hit ? h.tri : -1;
         pos[i] = h.hit ? h.bvh_pos : -1;
         u[i] = h.u; v[i] = h.v; t[i] = h.hit ? h.t : 0.0f;
+    }
+    return 0;
+}
+
+// Texture coordinate of the closest hit (Hit::uv, bvh.hpp:121-128): 2
+// floats per ray, (0, 0) on a miss.
+int orc_intersect_uv(void* p, int n, const float* o, const float* d, float* uv) {
+    Scene& s = *static_cast<Scene*>(p);
+    for (int i = 0; i < n; ++i) {
+        Ray r;
+        r.o = v3(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
+        r.d = v3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        Hit h = intersect_scene(s, r, RAY_T_MIN, RAY_T_MAX);
+        uv[2 * i] = h.hit ? h.uv[0] : 0.0f;
+        uv[2 * i + 1] = h.hit ? h.uv[1] : 0.0f;
     }
     return 0;
 }

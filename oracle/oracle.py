@@ -33,6 +33,11 @@ How each part is pinned (checks live in tests/test_oracle_*.py):
   exactly (:132-152), white furnace bound (:314-361), closed-form point light
   f*I/d^2 (test_gpu_path_tracer.cpp:281-349), and a direct-lighting integral
   computed by numerical quadrature.
+* node-tree materials (node_evaluator.cpp) -- checkerboard parity against
+  floor(u * scale) mod 2 computed in numpy, Mix against the float32 lerp,
+  colour x strength for Emission / Principled emission, the reference's
+  defaults and fallbacks case by case, and the hit's texture coordinate
+  against the affine UV map of a planar quad (tests/test_node_materials.py).
 * traceVolumeMIS -- homogeneous-slab transmittance against exp(-sigma*d)
   closed form; scatter vs no-scatter energy bound.  Per-sample agreement with
   the reference binary is "parity unpinned" (the reference cannot be built
@@ -53,6 +58,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 LIGHT_TYPES = {"POINT": 1, "SUN": 2, "SPOT": 3, "AREA": 4}
 AREA_SHAPES = {"SQUARE": 0, "RECTANGLE": 1, "DISK": 2, "ELLIPSE": 3}
+
+
+class _NodeSocket(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("kind", C.c_int32), ("value", C.c_float * 4), ("linked_node", C.c_char_p),
+                ("linked_socket", C.c_char_p)]
+
+
+class _Node(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_char_p), ("inputs", C.POINTER(_NodeSocket)),
+                ("num_inputs", C.c_int32), ("outputs", C.POINTER(_NodeSocket)), ("num_outputs", C.c_int32)]
 
 
 class _MeshDesc(C.Structure):
@@ -80,6 +95,11 @@ class _MeshDesc(C.Structure):
         ("grid_max", C.c_float * 3),
         ("grid", C.POINTER(C.c_float)),
         ("surface_linked", C.c_int32),
+        ("use_nodes", C.c_int32),
+        ("nodes", C.POINTER(_Node)),
+        ("num_nodes", C.c_int32),
+        ("tri_uvs", C.POINTER(C.c_float)),
+        ("num_tri_uvs", C.c_int32),
     ]
 
 
@@ -132,6 +152,8 @@ def lib(fast: bool = False) -> C.CDLL:
     L.orc_light_total_area.argtypes = [vp]
     L.orc_light_total_area.restype = C.c_float
     L.orc_light_emission.argtypes = [vp, ip, fp]
+    L.orc_material_eval.argtypes = [vp, C.c_int, C.c_int, fp, fp]
+    L.orc_intersect_uv.argtypes = [vp, C.c_int, fp, fp, fp]
     L.orc_render_tile.argtypes = [vp, C.c_char_p] + [ip] * 11 + [fp, dp, dp, i64p]
     L.orc_trace_pixel.argtypes = [vp, C.c_char_p] + [ip] * 8 + [fp]
     L.orc_primary_hits.argtypes = [vp] + [ip] * 8 + [i32p, i32p, i32p, fp, fp, fp]
@@ -202,6 +224,58 @@ def surface_linked(material: dict) -> bool:
     return True
 
 
+def _node_tree(material: dict, keep: list):
+    """parseNodeTree (node_evaluator.cpp:82-149) of a material dict: the
+    nodes with their sockets' parseSocketValue kinds (:47-69; 1 float,
+    2 vec3, 3 vec4, 0 anything else) and each input's link from the last
+    "links" entry that targets it.  Returns (array, count, use_nodes)."""
+    nt = material.get("node_tree")
+    if not isinstance(nt, dict) or not isinstance(nt.get("nodes"), list) or not nt["nodes"]:
+        return None, 0, 0
+    links = {}
+    for l in nt.get("links") or []:
+        links[(l["to_node"], l["to_socket"])] = (l["from_node"], l["from_socket"])
+
+    def enc(x):
+        b = str(x).encode()
+        keep.append(b)
+        return b
+
+    def kind(v):
+        if isinstance(v, bool) or v is None:
+            return 0, ()
+        if isinstance(v, (int, float)):
+            return 1, (float(v),)
+        if isinstance(v, list) and len(v) in (3, 4) and all(
+                isinstance(x, (int, float)) and not isinstance(x, bool) for x in v):
+            return (2 if len(v) == 3 else 3), tuple(float(x) for x in v)
+        return 0, ()
+
+    arr = (_Node * len(nt["nodes"]))()
+    keep.append(arr)
+    for i, n in enumerate(nt["nodes"]):
+        name = n.get("name", "")
+        name = name if isinstance(name, str) else ""
+        typ = n.get("type", "")
+        arr[i].name, arr[i].type = enc(name), enc(typ if isinstance(typ, str) else "")
+        for side in ("inputs", "outputs"):
+            socks = n.get(side) if isinstance(n.get(side), list) else []
+            sa = (_NodeSocket * max(1, len(socks)))()
+            keep.append(sa)
+            for k, sj in enumerate(socks):
+                sn = sj.get("name", "")
+                sn = sn if isinstance(sn, str) else ""
+                sa[k].name = enc(sn)
+                sa[k].kind, vals = kind(sj.get("default_value"))
+                for j, x in enumerate(vals):
+                    sa[k].value[j] = x
+                if side == "inputs" and (name, sn) in links:
+                    sa[k].linked_node, sa[k].linked_socket = (enc(x) for x in links[(name, sn)])
+            setattr(arr[i], side, C.cast(sa, C.POINTER(_NodeSocket)))
+            setattr(arr[i], "num_" + side, len(socks))
+    return arr, len(nt["nodes"]), 1 if material.get("use_nodes", False) else 0
+
+
 class OracleScene:
     """A scene loaded into the oracle, from a reference-schema scene dict."""
 
@@ -251,6 +325,15 @@ class OracleScene:
         d.transmission = float(mat.get("transmission", 0.0))
         d.ior = float(mat.get("ior", 1.45))
         d.surface_linked = 1 if surface_linked(mat) else 0
+        nodes, n_nodes, use = _node_tree(mat, keep)
+        if n_nodes:
+            d.nodes, d.num_nodes, d.use_nodes = nodes, n_nodes, use
+        tu = m.get("triangle_uvs")
+        if tu is not None and len(tu) > 0:
+            tu = np.ascontiguousarray(np.asarray(tu, dtype=np.float32).reshape(-1, 6))
+            d.tri_uvs = _fptr(tu)
+            d.num_tri_uvs = tu.shape[0]
+            keep.append(tu)
         vol = mat.get("volume")
         if isinstance(vol, dict):
             d.has_volume = 1
@@ -386,6 +469,14 @@ class OracleScene:
                              _iptr(pos), _fptr(u), _fptr(v), _fptr(t))
         return {"mesh": mesh, "tri": tri, "pos": pos, "u": u, "v": v, "t": t}
 
+    def intersect_uv(self, o, d):
+        """(n, 2) texture coordinates of the closest hits of rays o + t d."""
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        uv = np.zeros((o.shape[0], 2), np.float32)
+        self.L.orc_intersect_uv(self.h, o.shape[0], _fptr(o), _fptr(d), _fptr(uv))
+        return uv
+
     def mesh_bvh(self, mi):
         """(boxes (n,6), links (n,4) = left,right,tri_start,tri_count, tri_idx)."""
         n = C.c_int(0)
@@ -395,6 +486,15 @@ class OracleScene:
         tri_idx = np.zeros(self._ntri[mi], np.int32)
         self.L.orc_mesh_bvh(self.h, mi, C.byref(n), _fptr(boxes), _iptr(links), _iptr(tri_idx))
         return boxes, links, tri_idx
+
+    def material_eval(self, mesh: int, uv) -> np.ndarray:
+        """(n, 10): albedo RGB, emission RGB, metallic, roughness,
+        transmission, ior of mesh `mesh` at the (n, 2) texture coordinates."""
+        uv = np.ascontiguousarray(np.asarray(uv, np.float32).reshape(-1, 2))
+        out = np.zeros((uv.shape[0], 10), np.float32)
+        if self.L.orc_material_eval(self.h, mesh, uv.shape[0], _fptr(uv), _fptr(out)) != 0:
+            raise IndexError(mesh)
+        return out
 
     def lights_info(self):
         n = self.L.orc_num_lights(self.h)
