@@ -276,7 +276,7 @@ def main():
         tune_pr = lucid_capi.params(0, 0, W, H, W, H, spp, 0, depth, algo)  # one pass of the measured size
         ctx.check(L.lucid_autotune(ctx.h, sc, C.byref(camd), C.byref(tune_pr), C.byref(ls)))
         shape = {"traverse_groups_per_cu": ls.traverse_groups, "refill_idle_lanes": ls.refill,
-                 "shade_groups_per_cu": ls.shade_groups}
+                 "shade_groups_per_cu": ls.shade_groups, "ray_order": ls.ray_order}
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {shape}, {ls.ms:.1f} ms per pass")
 
     # ---- timed plain passes: Mrays/s ----

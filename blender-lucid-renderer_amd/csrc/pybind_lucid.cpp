@@ -592,6 +592,7 @@ public:
         d["traverse_groups"] = b.traverse_groups;
         d["refill"] = b.refill;
         d["shade_groups"] = b.shade_groups;
+        d["ray_order"] = b.ray_order;
         d["ms"] = b.ms;
         return d;
     }

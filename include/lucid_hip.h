@@ -261,16 +261,26 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
 /* Autotuner (no counterpart in the reference): times launch shapes on
    renders of `params` (discarded; guided requests are tuned on plain MIS)
-   by coordinate descent -- traversal groups per CU in {8, 12, 16, 24}, then
+   by coordinate descent -- the ray order (both values), traversal groups
+   per CU in {8, 12, 16, 24}, then
    the refill threshold in {32, 48, 60}, then shading groups per CU in
    {4, 8, 16}, two renders each -- and keeps the fastest on the context.
    Launch shapes do not change results.  Not with diagnostics enabled. */
 typedef struct lucid_launch_shape {
     int32_t traverse_groups, refill, shade_groups;
-    double ms;  /* device time of the chosen shape's render */
+    int32_t ray_order;  /* lucid_set_ray_order */
+    double ms;          /* device time of the chosen shape's render */
 } lucid_launch_shape;
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);
+
+/* Ray order (no counterpart in the reference): 1 (default) regroups, from
+   the second bounce on, the bounce rays of each depth by direction octant
+   and origin cell (8x8x8 over the scene box) before the closest-hit pass,
+   and the shadow rays by origin cell (16x16x16) before the shadow pass;
+   0 keeps the order in which shading queued them.  Results do not depend
+   on it. */
+int lucid_set_ray_order(lucid_ctx* ctx, int32_t order);
 
 /* Persistent traversal: a wavefront fetches new rays once this many of its
    64 lanes are idle (1..64, default 48).  A tuning knob, no counterpart in

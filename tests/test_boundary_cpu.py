@@ -70,6 +70,11 @@ def test_capi_rejects_bad_arguments_without_device():
     assert L.lucid_set_grid_format(None, 16) == 1 and L.lucid_scene_grid_bytes(None) == 0
     assert L.lucid_set_refill_threshold(None, 48) == 1
     assert L.lucid_set_launch_shape(None, 16, 8) == 1
+    assert L.lucid_set_ray_order(None, 1) == 1
+    uv = np.zeros(2, np.float32)
+    out = np.zeros(10, np.float32)
+    assert L.lucid_material_eval(None, 1, uv.ctypes.data_as(C.POINTER(C.c_float)),
+                                 out.ctypes.data_as(C.POINTER(C.c_float)), None, 0) == 1
 
 
 def test_async_api_without_scene_stays_idle(lucid):
