@@ -307,7 +307,8 @@ def main():
     per_launch = bytes_closest / n_launch
     achieved = per_launch / (avg_ms / 1e3) / 1e9
     log(f"intersect: {n_launch} launches, avg {avg_ms:.3f} ms, nodes/query {p['nodes_closest'] / max(q, 1):.1f}, "
-        f"tris/query {p['tris_closest'] / max(q, 1):.1f}; shade {p['ms_shade']:.1f} ms, shadow {p['ms_shadow']:.1f} ms")
+        f"tris/query {p['tris_closest'] / max(q, 1):.1f}; shade {p['ms_shade']:.1f} ms, shadow {p['ms_shadow']:.1f} ms, "
+        f"ray binning {p['ms_ray_binning']:.1f} ms")
 
     traffic = None
     if args.workload == "occluded_room" and spp == 16 and os.path.exists(TRAFFIC_PROFILE):
@@ -333,8 +334,10 @@ def main():
                                f"{algo}, {lights}",
                    "rays_per_step": rays // max(args.steps, 1), "paths_per_step": W * H * spp,
                    "launch_shape": shape,
+                   "pass_ms": {"intersect": round(p["ms_intersect"], 1), "shade": round(p["ms_shade"], 1),
+                               "shadow": round(p["ms_shadow"], 1), "ray_binning": round(p["ms_ray_binning"], 1)},
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
-        "roofline": {"bound": "hbm", "kernel": "k_trace_refill<false> (closest-hit BVH traversal, persistent with ray refill)",
+        "roofline": {"bound": "hbm", "kernel": "k_trace_refill<0> (closest-hit BVH traversal, persistent with ray refill)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/k_trace_refill_occluded_room_1080p_16spp.md (rocprofv3 PMC)"
@@ -405,6 +408,7 @@ def main():
             "reference": {"spp": rspp, "self_relmse": round(ref_err, 5), "kind": "oracle FP64, stratified subset"},
             "guided_pass_ms": {"render": round(gp["ms_render"], 1), "intersect": round(gp["ms_intersect"], 1),
                                "shade": round(gp["ms_shade"], 1), "shadow": round(gp["ms_shadow"], 1),
+                               "ray_binning": round(gp["ms_ray_binning"], 1),
                                "guide_infer": round(gp["ms_guide_infer"], 1),
                                "guide_train": round(gp["ms_guide_train"], 1)},
             "mfma": {"bound": "mfma", "peak": 2500.0, "unit": "TFLOP/s",

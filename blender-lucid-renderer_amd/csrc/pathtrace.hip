@@ -2239,15 +2239,17 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         k_raygen<<<blocks(n, kBlock), kBlock, 0, st>>>(W, cd, (int32_t)n);
         int q = 0;
         for (int depth = 0; depth < max_depth; ++depth) {
-            int ks = kev_begin(ctx, 0, st);
             const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_trav);
             const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
-            // this depth's kernels read the live queue through Wd
+            // the closest-hit pass reads the live queue through Wd
             WaveDev Wd = W;
             if (ctx->ray_order && depth > 0) {
+                const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinBounce>(W, BinSrc{W.q[q], W.cnt + q, W.ray_o, W.ray_d}, bin_box, n, ctx->num_cu, st);
+                kev_end(ctx, kb, st);
                 Wd.q[q] = W.sq;
             }
+            int ks = kev_begin(ctx, 0, st);
             if (refill)
                 k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, ctx->refill);
             else
@@ -2299,13 +2301,15 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                 k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, ctx->refill);
             }
             kev_end(ctx, ks, st);
-            ks = kev_begin(ctx, 2, st);
             // the shadow pass reads the NEE queue through Ws
             WaveDev Ws = W;
             if (ctx->ray_order && depth > 0) {
+                const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinShadow>(W, BinSrc{W.qs, W.cnt + 2, W.sh_o, W.sh_d}, bin_box, n, ctx->num_cu, st);
+                kev_end(ctx, kb, st);
                 Ws.qs = W.sq;
             }
+            ks = kev_begin(ctx, 2, st);
             if (refill && SD.any_vbo)
                 k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, ctx->refill);
             else if (refill)
@@ -2386,8 +2390,8 @@ int finish_stats(lucid_ctx* ctx, int nlanes = 1) {
     ctx->stats.tris_shadow = rays[5];
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->stats.ms_render = ms;
-    double kms[5] = {0, 0, 0, 0, 0};
-    uint64_t kn[5] = {0, 0, 0, 0, 0};
+    double kms[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t kn[6] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < ctx->kev_used; ++i) {
         float e = 0.0f;
         if (hipEventElapsedTime(&e, ctx->kev[2 * i], ctx->kev[2 * i + 1]) == hipSuccess) {
@@ -2398,6 +2402,7 @@ int finish_stats(lucid_ctx* ctx, int nlanes = 1) {
     ctx->stats.ms_intersect = kms[0]; ctx->stats.ms_shade = kms[1]; ctx->stats.ms_shadow = kms[2];
     ctx->stats.launches_intersect = kn[0]; ctx->stats.launches_shade = kn[1]; ctx->stats.launches_shadow = kn[2];
     ctx->stats.ms_guide_infer = kms[3]; ctx->stats.ms_guide_train = kms[4];
+    ctx->stats.ms_ray_binning = kms[5];
     if (cnt[3] != 0) return fail(ctx, LUCID_ERR_DEVICE, "BVH traversal stack overflow");
     return LUCID_OK;
 }

@@ -728,6 +728,7 @@ public:
         lucid_get_stats(ctx_, &rs);
         d["ms_guide_infer"] = rs.ms_guide_infer;
         d["ms_guide_train"] = rs.ms_guide_train;
+        d["ms_ray_binning"] = rs.ms_ray_binning;
         return d;
     }
     py::array_t<float> get_guiding_params() {

@@ -73,7 +73,8 @@ class Stats(C.Structure):
                 ("nodes_closest", C.c_uint64), ("tris_closest", C.c_uint64), ("nodes_shadow", C.c_uint64),
                 ("tris_shadow", C.c_uint64), ("ms_intersect", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("launches_intersect", C.c_uint64), ("launches_shade", C.c_uint64),
-                ("launches_shadow", C.c_uint64), ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double)]
+                ("launches_shadow", C.c_uint64), ("ms_guide_infer", C.c_double), ("ms_guide_train", C.c_double),
+                ("ms_ray_binning", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -185,6 +185,8 @@ typedef struct lucid_stats {
     /* with LUCID_PROFILE_KERNELS, guided renders: guiding-network inference
        and the training after the render */
     double ms_guide_infer, ms_guide_train;
+    /* with LUCID_PROFILE_KERNELS: regrouping the ray queues (lucid_set_ray_order) */
+    double ms_ray_binning;
 } lucid_stats;
 
 #define LUCID_PROFILE_WORK 1    /* count node / triangle fetches */
