@@ -420,6 +420,9 @@ def main():
                         "samples": int(gst.samples_total),
                         "adam_steps": int(gst.train_steps), "last_loss": round(float(gst.last_loss), 4)},
         }
+    # the session legs are other workloads than the one the autotuner ran on:
+    # they use the library's automatic ray order (regrouping waves of >= 2^24 paths)
+    ctx.check(L.lucid_set_ray_order(ctx.h, 2))
     if not args.no_sessions:
         result["sessions"] = sessions_leg(ctx, args.session_passes)
     if args.volumes:

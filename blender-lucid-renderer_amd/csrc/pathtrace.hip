@@ -1753,10 +1753,12 @@ struct lucid_ctx {
     int32_t traverse_groups = 16, shade_groups = 8;
     int32_t refill = 48;  // idle lanes that make a wavefront fetch new rays
     // 1: bounce and shadow rays regrouped by ray bin before their traversal
-    // passes (depth >= 1).  The 1080p occluded room renders 7 % faster with
-    // it (shadow pass -18 %, closest hit -5 % with the binning included); the
-    // 1M-triangle spheres are unchanged.
-    int32_t ray_order = 1;
+    // passes (depth >= 1); 0: never; 2: for waves of at least 2^24 paths.
+    // The 1080p occluded room (33M paths a wave) renders 8 % faster with it
+    // (shadow pass -22 %, closest hit -5 % with the binning included), the
+    // 1M-triangle spheres (67M) 1 % slower; batched sessions of small scenes
+    // (waves of 0.1-4M) lose 9 % to the binning launches.
+    int32_t ray_order = 2;
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
     int64_t film_cap = 0;
@@ -2218,6 +2220,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         W.sh_e = static_cast<float4*>(L.vallocs[1]);
         W.sh_t = static_cast<float4*>(L.vallocs[2]);
     }
+    const bool bin = ctx->ray_order == 1 || (ctx->ray_order == 2 && P * B >= ((int64_t)1 << 24));
     BinBox bin_box;
     for (int k = 0; k < 3; ++k) {
         bin_box.lo[k] = scene->bmin[k];
@@ -2243,7 +2246,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
             // the closest-hit pass reads the live queue through Wd
             WaveDev Wd = W;
-            if (ctx->ray_order && depth > 0) {
+            if (bin && depth > 0) {
                 const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinBounce>(W, BinSrc{W.q[q], W.cnt + q, W.ray_o, W.ray_d}, bin_box, n, ctx->num_cu, st);
                 kev_end(ctx, kb, st);
@@ -2303,7 +2306,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             kev_end(ctx, ks, st);
             // the shadow pass reads the NEE queue through Ws
             WaveDev Ws = W;
-            if (ctx->ray_order && depth > 0) {
+            if (bin && depth > 0) {
                 const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinShadow>(W, BinSrc{W.qs, W.cnt + 2, W.sh_o, W.sh_d}, bin_box, n, ctx->num_cu, st);
                 kev_end(ctx, kb, st);
@@ -2814,12 +2817,12 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
         return r;
     };
     // coordinate descent from the current shape over measured-good ranges
-    int32_t ro = ctx->ray_order, tg = ctx->traverse_groups, rf = ctx->refill, sg = ctx->shade_groups;
+    int32_t ro = 0, tg = ctx->traverse_groups, rf = ctx->refill, sg = ctx->shade_groups;
     double t_best = 0.0;
     rc = measure(ro, tg, rf, sg, &t_best);
     {
         double t = 0.0;
-        if (!rc && !(rc = measure(1 - ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; ro = 1 - ro; }
+        if (!rc && !(rc = measure(1, tg, rf, sg, &t)) && t < t_best) { t_best = t; ro = 1; }
     }
     const int32_t tgs[] = {8, 12, 16, 24}, rfs[] = {32, 48, 60}, sgs[] = {4, 8, 16};
     for (int32_t v : tgs) {
@@ -2856,7 +2859,7 @@ int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
 }
 
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order) {
-    if (!ctx || order < 0 || order > 1) return LUCID_ERR_INVALID;
+    if (!ctx || order < 0 || order > 2) return LUCID_ERR_INVALID;
     ctx->ray_order = order;
     return LUCID_OK;
 }

@@ -270,18 +270,18 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
    Launch shapes do not change results.  Not with diagnostics enabled. */
 typedef struct lucid_launch_shape {
     int32_t traverse_groups, refill, shade_groups;
-    int32_t ray_order;  /* lucid_set_ray_order */
+    int32_t ray_order;  /* lucid_set_ray_order: 0 or 1 */
     double ms;          /* device time of the chosen shape's render */
 } lucid_launch_shape;
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);
 
-/* Ray order (no counterpart in the reference): 1 (default) regroups, from
-   the second bounce on, the bounce rays of each depth by direction octant
-   and origin cell (8x8x8 over the scene box) before the closest-hit pass,
-   and the shadow rays by origin cell (16x16x16) before the shadow pass;
-   0 keeps the order in which shading queued them.  Results do not depend
-   on it. */
+/* Ray order (no counterpart in the reference): 1 regroups, from the second
+   bounce on, the bounce rays of each depth by direction octant and origin
+   cell (8x8x8 over the scene box) before the closest-hit pass, and the
+   shadow rays by origin cell (16x16x16) before the shadow pass; 0 keeps the
+   order in which shading queued them; 2 (default) regroups in waves of at
+   least 2^24 paths only.  Results do not depend on it. */
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order);
 
 /* Persistent traversal: a wavefront fetches new rays once this many of its

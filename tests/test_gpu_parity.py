@@ -166,7 +166,7 @@ def test_traversal_launch_shapes_agree(case, tmp_path):
     spp = 3
     prm = lucid_capi.params(0, 0, W, H, W, H, spp, 7, 8, algo)
     outs = []
-    for shape, refill, order in ((0, 48, 0), (16, 48, 0), (16, 1, 1), (3, 64, 1), (0, 48, 1)):
+    for shape, refill, order in ((0, 48, 0), (16, 48, 0), (16, 1, 1), (3, 64, 1), (0, 48, 1), (16, 48, 2)):
         ctx.check(ctx.L.lucid_set_launch_shape(ctx.h, shape, 16))
         ctx.check(ctx.L.lucid_set_refill_threshold(ctx.h, refill))
         ctx.check(ctx.L.lucid_set_ray_order(ctx.h, order))
