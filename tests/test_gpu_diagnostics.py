@@ -24,8 +24,9 @@ from conftest import make_oracle, make_renderer
 pytestmark = pytest.mark.gpu
 
 
-def _render_diag(lucid, algo, W, H, spp, depth, sub=1, offset=0):
-    r = make_renderer(lucid, workloads.closed_box(), workloads.CLOSED_BOX_CAMERA, algo)
+def _render_diag(lucid, algo, W, H, spp, depth, sub=1, offset=0, scene=None):
+    scene = workloads.closed_box() if scene is None else scene
+    r = make_renderer(lucid, scene, workloads.CLOSED_BOX_CAMERA, algo)
     cfg = lucid.PathRecordingConfig()
     cfg.subsample_factor = sub
     r.enable_diagnostics(cfg)
@@ -40,13 +41,16 @@ def _live_groups(px):
     return {int(g["path_hash"]): g for g in px["groups"][:n]}
 
 
-# every algorithm twice, so a later render runs on storage an earlier one used
-@pytest.mark.parametrize("algo", ["mis", "nee", "simple", "mis", "simple"])
-def test_film_matches_oracle(lucid, algo):
+# every algorithm twice, so a later render runs on storage an earlier one used;
+# then the box with node-tree materials (checker floor, mixed checker wall)
+@pytest.mark.parametrize("algo, textured", [("mis", False), ("nee", False), ("simple", False), ("mis", False),
+                                            ("simple", False), ("mis", True), ("nee", True), ("simple", True)])
+def test_film_matches_oracle(lucid, algo, textured):
     W = H = 16
     spp, depth = 8, 6
-    r, img, film, strat = _render_diag(lucid, algo, W, H, spp, depth)
-    o = make_oracle(workloads.closed_box(), workloads.CLOSED_BOX_CAMERA)
+    scene = workloads.textured_box(panel=False) if textured else workloads.closed_box()
+    r, img, film, strat = _render_diag(lucid, algo, W, H, spp, depth, scene=scene)
+    o = make_oracle(scene, workloads.CLOSED_BOX_CAMERA)
     ref_img, ref_film, ref_strat, ref_tot = o.render_tile_diag(algo, 0, 0, W, H, W, H, samples=spp, max_depth=depth)
     rel = np.abs(img - ref_img).sum() / max(np.abs(ref_img).sum(), 1e-12)
     assert rel < 2e-2, f"radiance rel L1 {rel:.3g}"

@@ -23,11 +23,22 @@ def smoke(tmp_path_factory):
     return workloads.smoke_box(os.path.join(str(d), "density.bin"), grid_n=24)
 
 
-@pytest.mark.parametrize("algorithm", ["volume_mis", "volume_simple"])
-def test_per_sample_radiance_smoke(lucid, smoke, algorithm):
+def _with_checker_floor(scene):
+    """The smoke box with the textured box's checker floor (a node-tree
+    material with triangle UVs) in place of its constant floor."""
+    floor = workloads.textured_box()["meshes"][0]
+    out = dict(scene, meshes=list(scene["meshes"]))
+    out["meshes"][0] = dict(out["meshes"][0], material=floor["material"], triangle_uvs=floor["triangle_uvs"])
+    return out
+
+
+@pytest.mark.parametrize("algorithm, textured", [("volume_mis", False), ("volume_simple", False),
+                                                 ("volume_mis", True)])
+def test_per_sample_radiance_smoke(lucid, smoke, algorithm, textured):
     cam = workloads.SMOKE_BOX_CAMERA
-    r = make_renderer(lucid, smoke, cam, algorithm)
-    o = make_oracle(smoke, cam)
+    scene = _with_checker_floor(smoke) if textured else smoke
+    r = make_renderer(lucid, scene, cam, algorithm)
+    o = make_oracle(scene, cam)
     W = H = 40
     spp = 4
     g = r.debug_render_samples(0, 0, W, H, W, H, spp, 0, 8)
