@@ -81,17 +81,24 @@ __device__ __forceinline__ void mlp_forward(const GuideDev& G, bf16x8 x0, bf16x8
 // operand is loaded once and applied to every tile.  No activations kept.
 template <int NT>
 __device__ __forceinline__ void mlp_forward_tiles(const GuideDev& G, bf16x8 (&b0)[NT], bf16x8 (&b1)[NT],
-                                                  f32x4 (&out)[NT][3]) {
+                                                  f32x4 (&out)[NT][3]);
+
+// The same with the packed weights at `fwd` and the biases of layer l at
+// bias_of(l) (both may live in LDS; see stage_weights).
+constexpr int kBiasStride = 64;
+template <int NT, class BiasOf>
+__device__ __forceinline__ void mlp_forward_tiles_at(const uint4* fwd, BiasOf bias_of, bf16x8 (&b0)[NT],
+                                                     bf16x8 (&b1)[NT], f32x4 (&out)[NT][3]) {
     const int lane = threadIdx.x & 63, g = lane >> 4;
 #pragma unroll
     for (int l = 0; l < kLayers; ++l) {
         const int mt = layer_mtiles(l);
-        const float* bias = G.params + b_offset(l);
+        const float* bias = bias_of(l);
         f32x4 acc[NT][4];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (m >= mt) break;
-            const uint4* a = G.fwd + (fwd_offset(l) + (m * 2) * 512) / 8 + lane;
+            const uint4* a = fwd + (fwd_offset(l) + (m * 2) * 512) / 8 + lane;
             const bf16x8 a0 = as_bf16x8(a[0]), a1 = as_bf16x8(a[64]);
             f32x4 bv;
             for (int j = 0; j < 4; ++j) bv[j] = bias[16 * m + 4 * g + j];
@@ -115,6 +122,23 @@ __device__ __forceinline__ void mlp_forward_tiles(const GuideDev& G, bf16x8 (&b0
             }
         }
     }
+}
+
+// Copies the packed forward operands and the biases (kBiasStride apart per
+// layer) into the workgroup's LDS arrays; the caller synchronises after.
+__device__ __forceinline__ void stage_weights(const GuideDev& G, uint4* fwd, float* bias) {
+    for (int i = threadIdx.x; i < kPackedFwd / 8; i += blockDim.x) fwd[i] = G.fwd[i];
+    for (int i = threadIdx.x; i < kLayers * kBiasStride; i += blockDim.x) {
+        const int l = i / kBiasStride, o = i % kBiasStride;
+        bias[i] = o < layer_out(l) ? G.params[b_offset(l) + o] : 0.0f;
+    }
+}
+
+// Forward pass with the weights read from global memory (tests).
+template <int NT>
+__device__ __forceinline__ void mlp_forward_tiles(const GuideDev& G, bf16x8 (&b0)[NT], bf16x8 (&b1)[NT],
+                                                  f32x4 (&out)[NT][3]) {
+    mlp_forward_tiles_at<NT>(G.fwd, [&](int l) { return G.params + b_offset(l); }, b0, b1, out);
 }
 
 }  // namespace guide
