@@ -115,3 +115,15 @@ def test_one_voxel_grid_axis_is_refused(tmp_path):
     d = np.float32([[0, 0, 1]])
     with pytest.raises(RuntimeError, match="LUCID_ERR_INVALID"):
         lucid_capi.host_trace(scene, o, d, 1e-4, 1e30)
+
+
+def test_occluded_room_rays():
+    """The 1080p workload's scene: camera rays and random rays inside the
+    room against the oracle."""
+    scene = workloads.occluded_room()
+    o, d = _camera_rays(workloads.OCCLUDED_ROOM_CAMERA, 64, 36)
+    _compare(scene, o, d, 0.0, 1e30)
+    rng = np.random.default_rng(4)
+    o, d = _rays(rng, 4000, -1.5, 1.5)
+    o[:, 1] += 1.5
+    _compare(scene, o, d, 1e-4, 1e30)
