@@ -120,9 +120,10 @@ def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
     tris = sum(int(ctx.L.lucid_scene_num_triangles(h)) for h in scenes)
     pixels = sum(w * hh for _, _, w, hh in specs)
     log(f"sessions: {count} scenes, {tris} triangles, {pixels} pixels, set up in {time.perf_counter() - t0:.1f} s")
-    lucid_capi.render_sessions(ctx, sessions, spp)  # warm-up: allocates every lane's buffers
-    for s in sessions:  # and the serial path's
-        lucid_capi.render_sessions(ctx, [s], 1)
+    # warm-up: allocates every lane's buffers, and runs each pass shape twice
+    # (once launch by launch, once captured), so the timed passes replay graphs
+    for _ in range(2):
+        lucid_capi.render_sessions(ctx, sessions, spp)
 
     def timed(batched):
         rays, t = 0, 0.0
@@ -141,6 +142,9 @@ def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
         return rays / t / 1e6, t / passes * 1e3
 
     b_mrays, b_ms = timed(True)
+    for _ in range(2):  # the one-at-a-time passes' own warm-up
+        for s in sessions:
+            lucid_capi.render_sessions(ctx, [s], spp)
     s_mrays, s_ms = timed(False)
     log(f"sessions: batched {b_mrays:.0f} Mrays/s ({b_ms:.0f} ms/pass), one at a time {s_mrays:.0f} Mrays/s")
     for s in sessions:

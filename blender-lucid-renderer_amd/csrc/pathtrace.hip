@@ -137,6 +137,7 @@ struct CamDev {
     int32_t samples_in_call;  // render_tile's `samples` (jitter only when > 1)
     int32_t s0;               // absolute sample index of wave slot b = 0
     int32_t P, B, max_depth;
+    const int32_t* s0p;       // when set, s0 is read here (a session pass replayed from a graph)
 };
 
 // ---------------------------------------------------------------- helpers --
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
     int b = p / c.P, pix = p - b * c.P;
     int px = pix % c.tw, py = pix / c.tw;
     int gx = c.tx + px, gy = c.ty + py;
-    int s = c.s0 + b;
+    int s = (c.s0p ? *c.s0p : c.s0) + b;
     float ndc_x = (2.0f * (gx + 0.5f) / c.fw - 1.0f) * c.aspect;
     float ndc_y = 1.0f - 2.0f * (gy + 0.5f) / c.fh;
     Pcg rng = pcg_seed((uint32_t)gx, (uint32_t)gy, (uint32_t)s);
@@ -1551,6 +1552,11 @@ void bin_rays(const WaveDev& W, const BinSrc& src, const BinBox& box, int64_t n,
     k_bin_scatter<<<gb, 256, 0, st>>>(W, src);
 }
 
+// One int to device memory, in stream order (vector store from lane 0).
+__global__ __launch_bounds__(64) void k_set_i32(int32_t* p, int32_t v) {
+    if (threadIdx.x == 0) p[threadIdx.x] = v;
+}
+
 __global__ __launch_bounds__(256) void k_film_clear(float4* film, int32_t P) {
     int pix = blockIdx.x * 256 + threadIdx.x;
     if (pix < P) film[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1774,6 +1780,7 @@ struct lucid_ctx {
     // 1M-triangle spheres (67M) 1 % slower; batched sessions of small scenes
     // (waves of 0.1-4M) lose 9 % to the binning launches.
     int32_t ray_order = 2;
+    int32_t session_graphs = 1;  // replay plain one-wave session passes from captured graphs
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
     int64_t film_cap = 0;
@@ -1834,6 +1841,11 @@ struct lucid_session {
     float* snap_pinned = nullptr;
     std::atomic<bool> failed{false};
     std::string err;  // why the worker ended early: written before `failed`
+    // a pass replayed from a captured graph (plain passes of one wave)
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gsig;  // what the graph (or the last plain pass) was made for
+    bool gseen = false;         // a plain pass ran with gsig: capture on the next
+    int32_t* s0_dev = nullptr;  // first sample index of the replayed pass
 };
 
 namespace {
@@ -2073,7 +2085,8 @@ struct RunOut {
 
 int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
               int mode, hipStream_t st, RunOut* ro, bool* cancelled, float4* accum = nullptr, int active_lanes = 1,
-              const std::atomic<int>* cancel_flag = nullptr) {
+              const std::atomic<int>* cancel_flag = nullptr, const int32_t* s0_dev = nullptr,
+              bool capturing = false) {
     *cancelled = false;
     if (!cancel_flag) cancel_flag = &ctx->cancel;
     const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
@@ -2210,6 +2223,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     cd.samples_in_call = mode == 3 ? std::max(S, 2) : S;
     cd.P = (int32_t)P;
     cd.max_depth = pr->max_depth;
+    cd.s0p = s0_dev;
     uint64_t waves = 0;
     const int max_depth = mode == 2 ? 1 : pr->max_depth;
     // traceVolumeSimple / traceVolumeMIS reduce exactly to traceSimple /
@@ -2357,7 +2371,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             k_hits_out<<<blocks(n, 256), 256, 0, st>>>(SD, W, ro->mesh, ro->tri, ro->u, ro->v, ro->t, (int32_t)P, b,
                                                        S, s0);
         }
-        (void)hipEventRecord(L.wave_ev[waves & 1], st);
+        if (!capturing) (void)hipEventRecord(L.wave_ev[waves & 1], st);
         ++waves;
         if (ro) ro->samples_done = s0 + b;
     }
@@ -2874,6 +2888,13 @@ int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
     return LUCID_OK;
 }
 
+int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on) {
+    if (!ctx || on < 0 || on > 1) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->session_graphs = on;
+    return LUCID_OK;
+}
+
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order) {
     if (!ctx || order < 0 || order > 2) return LUCID_ERR_INVALID;
     ctx->ray_order = order;
@@ -3103,14 +3124,111 @@ int session_clear(lucid_session* s) {
 
 // Adds up to `spp` samples to the session on lane L (caller holds ctx->mu);
 // the samples of every wave issued are counted, also when cancelled.
+// What a captured session pass depends on besides its first sample index:
+// a replay is valid only while every field is unchanged.
+struct PassKey {
+    const void* lane;
+    hipStream_t stream;
+    int64_t cap_paths, vcap, wave_paths;
+    WaveDev W;
+    const lucid_scene* scene;
+    lucid_camera cam;
+    const float4* accum;
+    int32_t w, h, max_depth, algorithm, spp, active_lanes;
+    int32_t traverse_groups, refill, shade_groups, ray_order, num_cu;
+};
+std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp, int active_lanes) {
+    PassKey k;
+    std::memset(&k, 0, sizeof(k));
+    const lucid_ctx* ctx = s->ctx;
+    k.lane = &L; k.stream = L.stream; k.cap_paths = L.cap_paths; k.vcap = L.vcap; k.wave_paths = ctx->wave_paths;
+    k.W = L.W; k.scene = s->scene; k.cam = s->cam; k.accum = s->accum;
+    k.w = s->w; k.h = s->h; k.max_depth = s->max_depth; k.algorithm = s->algorithm; k.spp = spp;
+    k.active_lanes = active_lanes;
+    k.traverse_groups = ctx->traverse_groups; k.refill = ctx->refill; k.shade_groups = ctx->shade_groups;
+    k.ray_order = ctx->ray_order; k.num_cu = ctx->num_cu;
+    const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
+    return std::vector<uint8_t>(b, b + sizeof(k));
+}
+
+void drop_graph(lucid_session* s) {
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+    s->gseen = false;
+    s->gsig.clear();
+}
+
+// A plain session pass (MIS, NEE, simple, volume) of one wave is launched as
+// one graph: the first pass with a new key runs launch by launch, the second
+// is captured, and later ones replay it with the new first sample index.
+// The kernels are those of run_waves, so results are the same bit for bit.
 int session_pass(lucid_session* s, Lane& L, int32_t spp, int active_lanes, const std::atomic<int>* cancel_flag,
                  bool* cancelled) {
-    if (s->samples + spp > INT32_MAX) return fail(s->ctx, LUCID_ERR_INVALID, "session sample count overflows");
+    lucid_ctx* ctx = s->ctx;
+    if (s->samples + spp > INT32_MAX) return fail(ctx, LUCID_ERR_INVALID, "session sample count overflows");
     lucid_render_params pr{0, 0, s->w, s->h, s->w, s->h, spp, (int32_t)s->samples, s->max_depth, s->algorithm};
+    const int64_t P = (int64_t)s->w * s->h;
+    // one wave for sure: the lane already holds P * spp paths (see run_waves' batch size)
+    const bool graphable = ctx->session_graphs && s->algorithm != LUCID_ALGO_GUIDED_MIS && !ctx->diag_on &&
+                           ctx->profiling == 0 && P * spp <= L.cap_paths && P * spp <= ((int64_t)1 << 26) &&
+                           (ctx->wave_paths <= 0 || ctx->wave_paths >= P * spp);
+    std::vector<uint8_t> key;
+    if (graphable) {
+        key = pass_key(s, L, spp, active_lanes);
+        if (key != s->gsig) drop_graph(s);
+    }
+    if (graphable && s->gseen && !s->gexec) {
+        if (!s->s0_dev) HIP_OK(hipMalloc((void**)&s->s0_dev, 64 * sizeof(int32_t)));
+        if (cancel_flag->load(std::memory_order_relaxed)) {
+            *cancelled = true;
+            return LUCID_OK;
+        }
+        HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed));
+        RunOut ro;
+        const int rc = run_waves(ctx, L, s->scene, &s->cam, &pr, 3, L.stream, &ro, cancelled, s->accum,
+                                 active_lanes, cancel_flag, s->s0_dev, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(L.stream, &g);
+        hipGraphExec_t ex = nullptr;
+        const bool ok = !rc && e == hipSuccess && g && !*cancelled && ro.samples_done == spp &&
+                        hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            (void)hipGetLastError();
+            drop_graph(s);
+            ctx->session_graphs = 0;  // capture failed: plain launches from now on
+            if (rc) return rc;
+            if (*cancelled) return LUCID_OK;
+            return session_pass(s, L, spp, active_lanes, cancel_flag, cancelled);
+        }
+        s->gexec = ex;
+        s->gsig = key;
+        // run_waves counted this pass's paths while it was captured
+        k_set_i32<<<1, 64, 0, L.stream>>>(s->s0_dev, (int32_t)s->samples);
+        HIP_OK(hipGraphLaunch(s->gexec, L.stream));
+        s->samples += spp;
+        return LUCID_OK;
+    }
+    if (graphable && s->gexec) {
+        if (cancel_flag->load(std::memory_order_relaxed)) {
+            *cancelled = true;
+            return LUCID_OK;
+        }
+        k_set_i32<<<1, 64, 0, L.stream>>>(s->s0_dev, (int32_t)s->samples);
+        HIP_OK(hipGraphLaunch(s->gexec, L.stream));
+        s->samples += spp;
+        ctx->stats.paths += (uint64_t)P * spp;
+        ctx->stats.waves += 1;
+        return LUCID_OK;
+    }
     RunOut ro;
-    int rc = run_waves(s->ctx, L, s->scene, &s->cam, &pr, 3, L.stream, &ro, cancelled, s->accum, active_lanes,
+    int rc = run_waves(ctx, L, s->scene, &s->cam, &pr, 3, L.stream, &ro, cancelled, s->accum, active_lanes,
                        cancel_flag);
     if (!rc) s->samples += ro.samples_done;
+    if (!rc && graphable && !*cancelled) {
+        s->gsig = key;
+        s->gseen = true;
+    }
     return rc;
 }
 
@@ -3259,6 +3377,8 @@ void lucid_session_destroy(lucid_session* s) {
         if (s->accum) (void)hipFree(s->accum);
         if (s->snap_dev) (void)hipFree(s->snap_dev);
         if (s->snap_pinned) (void)hipHostFree(s->snap_pinned);
+        drop_graph(s);
+        if (s->s0_dev) (void)hipFree(s->s0_dev);
     }
     lucid_scene_release(s->scene);
     delete s;

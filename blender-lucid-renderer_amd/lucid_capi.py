@@ -134,6 +134,7 @@ SIGNATURES = {
     "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lucid_set_refill_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_ray_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_set_session_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                  C.POINTER(LaunchShape)]),
     "lucid_set_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(DiagConfig)]),

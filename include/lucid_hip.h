@@ -276,6 +276,13 @@ typedef struct lucid_launch_shape {
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);
 
+/* Session passes as graphs (no counterpart in the reference): 1 (default)
+   launches a plain (not guided) one-wave session pass as one captured HIP
+   graph once the same pass (session, lane, size, samples per pass, launch
+   shape) has run twice, replaying it with the new first sample index; 0
+   launches every kernel on its own.  Results do not depend on it. */
+int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on);
+
 /* Ray order (no counterpart in the reference): 1 regroups, from the second
    bounce on, the bounce rays of each depth by direction octant and origin
    cell (8x8x8 over the scene box) before the closest-hit pass, and the

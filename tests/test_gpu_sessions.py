@@ -54,6 +54,28 @@ def test_session_passes_equal_one_render_tile(ctx, algo):
     ctx.L.lucid_scene_release(sh)
 
 
+@pytest.mark.parametrize("algo", ["mis", "volume_mis"])
+def test_graph_replayed_passes_equal_render_tile(ctx, algo, tmp_path):
+    """Equal passes run plain, are captured, then replay from the graph:
+    after every pass, with eager renders in between, the session holds
+    exactly render_tile of its sample count; with graphs off, the same."""
+    scene = workloads.closed_box() if algo == "mis" else workloads.smoke_box(str(tmp_path / "d.bin"), grid_n=16)
+    sh = ctx.scene(scene)
+    cam = lucid_capi.camera(workloads.CLOSED_BOX_CAMERA)
+    W, H, D = 40, 28, 6
+    for graphs in (1, 0):
+        ctx.check(ctx.L.lucid_set_session_graphs(ctx.h, graphs))
+        s = lucid_capi.Session(ctx, sh, cam, W, H, D, algo)
+        for k in range(6):
+            lucid_capi.render_sessions(ctx, [s], 3)
+            assert s.samples == 3 * (k + 1)
+            ref = _render_tile(ctx, sh, cam, W, H, s.samples, D, algo)  # eager work between replays
+            assert np.array_equal(s.read(average=False), ref), (graphs, k)
+        s.close()
+    ctx.check(ctx.L.lucid_set_session_graphs(ctx.h, 1))
+    ctx.L.lucid_scene_release(sh)
+
+
 def test_batched_sessions_equal_their_own_renders(ctx):
     """Eight sessions of different scenes, sizes, depths and algorithms in
     one batch (spread over the streams), two passes."""
@@ -71,10 +93,14 @@ def test_batched_sessions_equal_their_own_renders(ctx):
     st = ctx.stats()
     assert st["paths"] == sum(w * h * 10 for _, _, w, h, _, _, _ in cases)
     assert st["rays_closest"] >= st["paths"] and st["stack_overflows"] == 0
+    for _ in range(3):  # equal passes: plain, captured, replayed
+        lucid_capi.render_sessions(ctx, sessions, 4)
+        st = ctx.stats()
+        assert st["paths"] == sum(w * h * 4 for _, _, w, h, _, _, _ in cases)
     for sh, c, w, h, depth, algo, s in cases:
-        assert s.samples == 16
+        assert s.samples == 28
         got = s.read(average=False)
-        ref = _render_tile(ctx, sh, c, w, h, 16, depth, algo)
+        ref = _render_tile(ctx, sh, c, w, h, 28, depth, algo)
         assert np.array_equal(got, ref), (w, h, algo, np.abs(got - ref).max())
         assert got[0::4].sum() > 0
     for sh, *_, s in cases:
