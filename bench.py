@@ -219,9 +219,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
     ap.add_argument("--alpha", type=float, default=0.7)
-    ap.add_argument("--train-steps", type=int, default=32)
+    ap.add_argument("--train-steps", type=int, default=16)
     ap.add_argument("--batch", type=int, default=32768)
-    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--record-stride", type=int, default=32)
     ap.add_argument("--guide-depth", type=int, default=0, help="guided bounces: depth < N (0: all)")
     args = ap.parse_args()

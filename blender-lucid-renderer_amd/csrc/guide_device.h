@@ -59,6 +59,7 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
 // features() written straight into a row of BF16 (no 64-float array in
 // registers): the same values.
 __device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, __bf16* row) {
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         float sn, cs;
         sincospif(pn[c], &sn, &cs);

@@ -1277,8 +1277,10 @@ __device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float
 // staged once per workgroup in LDS (read there by every MFMA tile instead of
 // through the cache hierarchy: 16.0 -> 12.7 ms of a 1080p 16-spp guided
 // pass, bit-identical outputs).
-// Each lane encodes one path's 64 input features into LDS; the MFMA operands
-// are then gathered from LDS as 4 tiles of 16 that share every weight load.
+// Each lane encodes one path's 64 input features into its LDS row (8 16-byte
+// stores: rows are padded to 144 bytes, so they stay 16-byte aligned); the
+// MFMA operands are then gathered from LDS, two 8-byte runs of 4 features
+// each, as 4 tiles of 16 that share every weight load.
 // Paths without a surface hit run on zero input and are not written.
 constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 // Wavefronts per workgroup: they share one LDS copy of the weights (31 KB);
@@ -1287,41 +1289,74 @@ constexpr int kInferWaves = 4;
 __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, WaveDev W, guide::GuideDev G, int qin) {
     __shared__ uint4 wfwd[guide::kPackedFwd / 8];
     __shared__ float wbias[guide::kLayers * guide::kBiasStride];
-    __shared__ __bf16 feats[kInferWaves][64][guide::kIn + 2];
+    constexpr int kRow = guide::kIn + 8;  // bf16 per row: 144 bytes
+    __shared__ __attribute__((aligned(16))) __bf16 feats[kInferWaves][64][kRow];
     __shared__ int32_t paths[kInferWaves][64];
     guide::stage_weights(G, wfwd, wbias);
     __syncthreads();
     const int n = W.cnt[qin];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, s = lane & 15;
-    __bf16 (*feat)[guide::kIn + 2] = feats[wave];
+    __bf16 (*feat)[kRow] = feats[wave];
     int32_t* path = paths[wave];
     const auto bias_of = [&](int l) { return wbias + guide::kBiasStride * l; };
+    // Loads are software-pipelined over the trip loop: the queue entry two
+    // trips ahead and the hit and ray of the next trip are issued before this
+    // trip's MLP, so their latency overlaps the matrix work; only the hit's
+    // normal gathers stay on the critical path.
+    const int stride = gridDim.x * kInferWaves * 64, lane0 = wave * 64 + lane;
+    const auto load_q = [&](int b0) { return b0 + lane0 < n ? W.q[qin][b0 + lane0] : -1; };
+    // Branch-free, and only the words that are used: a load whose register
+    // is dead or copied at once would be waited for right away.  Slot -1
+    // reads path 0, which every launched wave has.
+    struct PathIn {
+        int pp;
+        float4 hv;
+        v3f o, d;
+    };
+    const auto load_path = [&](int pp) {
+        const int i = pp >= 0 ? pp : 0;
+        const float* po = reinterpret_cast<const float*>(W.ray_o + i);
+        const float* pd = reinterpret_cast<const float*>(W.ray_d + i);
+        PathIn r;
+        r.pp = pp;
+        r.hv = W.hit[i];
+        r.o = mk(po[0], po[1], po[2]);
+        r.d = mk(pd[0], pd[1], pd[2]);
+        return r;
+    };
+    int base0 = blockIdx.x * kInferWaves * 64;
+    PathIn cur = load_path(load_q(base0));
+    int q_next = load_q(base0 + stride);
     // trip counts are uniform over the workgroup (its waves take 64 paths each)
-    for (int base0 = blockIdx.x * kInferWaves * 64; base0 < n; base0 += gridDim.x * kInferWaves * 64) {
-        const int base = base0 + wave * 64;
+    for (; base0 < n; base0 += stride) {
         {
-            const int qi = base + lane;
             int p = -1;
             float pn[3] = {0.0f, 0.0f, 0.0f};
             v3f nrm = mk(0.0f, 0.0f, 1.0f), wo = mk(0.0f, 0.0f, 1.0f);
-            if (qi < n) {
-                const int pp = W.q[qin][qi];
-                const float4 hv = W.hit[pp];
-                if (__float_as_int(hv.y) >= 0) {
-                    const v3f o = mk(W.ray_o[pp]), d = mk(W.ray_d[pp]);
-                    v3f point;
-                    hit_frame(S, o, d, hv, point, nrm);
-                    wo = -d;
-                    pn[0] = (point.x - G.bmin[0]) * G.binv[0];
-                    pn[1] = (point.y - G.bmin[1]) * G.binv[1];
-                    pn[2] = (point.z - G.bmin[2]) * G.binv[2];
-                    p = pp;
-                }
+            if (cur.pp >= 0 && __float_as_int(cur.hv.y) >= 0) {
+                const v3f o = cur.o, d = cur.d;
+                v3f point;
+                hit_frame(S, o, d, cur.hv, point, nrm);
+                wo = -d;
+                pn[0] = (point.x - G.bmin[0]) * G.binv[0];
+                pn[1] = (point.y - G.bmin[1]) * G.binv[1];
+                pn[2] = (point.z - G.bmin[2]) * G.binv[2];
+                p = cur.pp;
             }
-            guide::features_bf16(pn, nrm, wo, &feat[lane][0]);
+            __bf16 row[guide::kIn];
+            guide::features_bf16(pn, nrm, wo, row);
+            uint4* dst = reinterpret_cast<uint4*>(&feat[lane][0]);
+#pragma unroll
+            for (int q = 0; q < guide::kIn / 8; ++q) {
+                uint4 v;
+                __builtin_memcpy(&v, row + 8 * q, 16);
+                dst[q] = v;
+            }
             path[lane] = p;
         }
         __syncthreads();
+        const PathIn nxt = load_path(q_next);
+        q_next = load_q(base0 + 2 * stride);
         // the 4 tiles of 16 samples in passes of kInferTiles tiles (fewer
         // live accumulators per pass, more wavefronts per SIMD)
 #pragma unroll
@@ -1330,10 +1365,13 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
             int pt[kInferTiles];
 #pragma unroll
             for (int t = 0; t < kInferTiles; ++t) {
-                for (int i = 0; i < 8; ++i) {
-                    x0[t][i] = feat[16 * (t0 + t) + s][guide::perm(0, g, i)];
-                    x1[t][i] = feat[16 * (t0 + t) + s][guide::perm(1, g, i)];
-                }
+                // perm(kb, g, i): features 32 kb + 4 g + i (i < 4) and
+                // 32 kb + 16 + 4 g + i - 4, i.e. 8-byte runs 8 kb + g and 8 kb + 4 + g
+                const uint2* r = reinterpret_cast<const uint2*>(&feat[16 * (t0 + t) + s][0]);
+                const uint4 a = {r[g].x, r[g].y, r[4 + g].x, r[4 + g].y};
+                const uint4 b = {r[8 + g].x, r[8 + g].y, r[12 + g].x, r[12 + g].y};
+                __builtin_memcpy(&x0[t], &a, 16);
+                __builtin_memcpy(&x1[t], &b, 16);
                 pt[t] = path[16 * (t0 + t) + s];
             }
             guide::f32x4 o[kInferTiles][3];
@@ -1355,6 +1393,7 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
             }
         }
         __syncthreads();  // the next iteration overwrites feat / path
+        cur = nxt;
     }
 }
 
@@ -1790,7 +1829,7 @@ struct lucid_ctx {
     int32_t profiling = 0;
     // learned path guiding
     guide::State* gstate = nullptr;
-    lucid_guiding_config gcfg{0.7f, 32, 32768, 3e-3f, 32, 1u, 0};
+    lucid_guiding_config gcfg{0.7f, 16, 32768, 1e-2f, 32, 1u, 0};
     int64_t gsteps = 0;
     __half* gout = nullptr;  // FP16 network outputs, 40 per path
     int64_t gout_cap = 0;

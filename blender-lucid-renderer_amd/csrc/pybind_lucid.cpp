@@ -1110,8 +1110,8 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
-        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 32,
-             py::arg("batch") = 32768, py::arg("learning_rate") = 3e-3f, py::arg("record_stride") = 32,
+        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 16,
+             py::arg("batch") = 32768, py::arg("learning_rate") = 1e-2f, py::arg("record_stride") = 32,
              py::arg("seed") = 1u, py::arg("max_depth") = 0,
              "Extension: configure learned path guiding (algorithm 'guided') and start from a fresh network")
         .def("reset_guiding", &Renderer::reset_guiding, "Extension: fresh guiding network")
