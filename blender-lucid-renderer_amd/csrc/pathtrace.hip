@@ -1481,6 +1481,7 @@ __global__ __launch_bounds__(256) void k_film(WaveDev W, float4* film, int32_t P
 // order: every path keeps its own state and RNG.
 constexpr int kRayBinBits = 12;
 constexpr int kRayBins = 1 << kRayBinBits;
+constexpr int kBinBlock = 1024;  // threads per workgroup of the binning kernels
 struct BinBox {
     float lo[3], sc[3];  // (origin - lo) * sc is in [0, 1) inside the scene box
 };
@@ -1517,19 +1518,19 @@ struct BinSrc {
     const float4* dir;
 };
 template <int KEY>
-__global__ __launch_bounds__(256) void k_bin_count(WaveDev W, BinSrc src, BinBox b) {
+__global__ __launch_bounds__(kBinBlock) void k_bin_count(WaveDev W, BinSrc src, BinBox b) {
     __shared__ uint32_t h[kRayBins];
-    for (int i = threadIdx.x; i < kRayBins; i += 256) h[i] = 0u;
+    for (int i = threadIdx.x; i < kRayBins; i += kBinBlock) h[i] = 0u;
     __syncthreads();
     const int n = *src.count;
-    for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = blockIdx.x * kBinBlock + (int)threadIdx.x; i < n; i += gridDim.x * kBinBlock) {
         const int p = src.q[i];
         const uint32_t k = ray_bin<KEY>(src.org[p], src.dir[p], b);
         W.skey[i] = k;
         atomicAdd(&h[k], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kRayBins; i += 256)
+    for (int i = threadIdx.x; i < kRayBins; i += kBinBlock)
         if (h[i]) atomicAdd(&W.sbins[i], h[i]);
 }
 // exclusive prefix sum of the kRayBins counts (one workgroup, 4 bins a lane)
@@ -1559,22 +1560,22 @@ __global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* bins) {
 }
 // Each workgroup moves one contiguous chunk of the queue: it reserves a
 // range in every bin it holds entries of, then places them.
-__global__ __launch_bounds__(256) void k_bin_scatter(WaveDev W, BinSrc src) {
+__global__ __launch_bounds__(kBinBlock) void k_bin_scatter(WaveDev W, BinSrc src) {
     __shared__ uint32_t h[kRayBins];
     __shared__ uint32_t base[kRayBins];
-    for (int i = threadIdx.x; i < kRayBins; i += 256) h[i] = 0u;
+    for (int i = threadIdx.x; i < kRayBins; i += kBinBlock) h[i] = 0u;
     __syncthreads();
     const int n = *src.count;
     const int chunk = (n + (int)gridDim.x - 1) / (int)gridDim.x;
     const int i0 = (int)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) atomicAdd(&h[W.skey[i]], 1u);
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += kBinBlock) atomicAdd(&h[W.skey[i]], 1u);
     __syncthreads();
-    for (int i = threadIdx.x; i < kRayBins; i += 256) {
+    for (int i = threadIdx.x; i < kRayBins; i += kBinBlock) {
         base[i] = h[i] ? atomicAdd(&W.sbins[i], h[i]) : 0u;
         h[i] = 0u;
     }
     __syncthreads();
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += kBinBlock) {
         const uint32_t k = W.skey[i];
         const uint32_t r = atomicAdd(&h[k], 1u);
         W.sq[base[k] + r] = src.q[i];
@@ -1584,11 +1585,15 @@ __global__ __launch_bounds__(256) void k_bin_scatter(WaveDev W, BinSrc src) {
 // Regroups src into W.sq (stream-ordered; W.skey and W.sbins are scratch).
 template <int KEY>
 void bin_rays(const WaveDev& W, const BinSrc& src, const BinBox& box, int64_t n, int num_cu, hipStream_t st) {
-    const unsigned gb = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + 255) / 256), (int64_t)num_cu * 4);
+    // one workgroup per CU at most: each workgroup adds its LDS histogram
+    // into the 4096 global counters, so fewer, larger workgroups mean fewer
+    // contended atomics on them
+    const unsigned gb =
+        (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kBinBlock - 1) / kBinBlock), (int64_t)num_cu);
     k_bin_clear<<<1, 256, 0, st>>>(W.sbins);
-    k_bin_count<KEY><<<gb, 256, 0, st>>>(W, src, box);
+    k_bin_count<KEY><<<gb, kBinBlock, 0, st>>>(W, src, box);
     k_bin_scan<<<1, 1024, 0, st>>>(W.sbins);
-    k_bin_scatter<<<gb, 256, 0, st>>>(W, src);
+    k_bin_scatter<<<gb, kBinBlock, 0, st>>>(W, src);
 }
 
 // One int to device memory, in stream order (vector store from lane 0).
