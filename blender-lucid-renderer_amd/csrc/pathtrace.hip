@@ -2240,17 +2240,20 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // workgroups per CU walk the live queues (grid-stride).
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
     const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
+    // the launch shape of this render, read once (callers hold ctx->mu)
+    const int32_t traverse_groups = ctx->traverse_groups, shade_groups = ctx->shade_groups;
+    const int32_t refill_lanes = ctx->refill, ray_order = ctx->ray_order;
     // traverse_groups > 0: persistent traversal with ray refill on that many
     // workgroups per CU; 0: one lane per queued ray
-    const bool refill = ctx->traverse_groups > 0;
+    const bool refill = traverse_groups > 0;
     // Batched session passes run up to four lanes at once: each gets half
     // the groups (measured best on 32 sessions).  Small waves keep at least
     // kMinRaysPerLane rays per lane, so no wavefront is left near-idle.
-    const int tgroups = active_lanes > 1 ? std::max(4, ctx->traverse_groups / 2) : ctx->traverse_groups;
+    const int tgroups = active_lanes > 1 ? std::max(4, traverse_groups / 2) : traverse_groups;
     const int64_t cap_trav = refill ? std::min<int64_t>((int64_t)ctx->num_cu * tgroups,
                                                         blocks(P * B, kBlock * kMinRaysPerLane))
                                     : INT64_MAX;
-    const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
+    const int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
         ctx->kev_used = 0;
@@ -2293,7 +2296,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         W.sh_e = static_cast<float4*>(L.vallocs[1]);
         W.sh_t = static_cast<float4*>(L.vallocs[2]);
     }
-    const bool bin = ctx->ray_order == 1 || (ctx->ray_order == 2 && P * B >= ((int64_t)1 << 24));
+    const bool bin = ray_order == 1 || (ray_order == 2 && P * B >= ((int64_t)1 << 24));
     BinBox bin_box;
     for (int k = 0; k < 3; ++k) {
         bin_box.lo[k] = scene->bmin[k];
@@ -2327,7 +2330,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             int ks = kev_begin(ctx, 0, st);
             if (refill)
-                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, ctx->refill);
+                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, refill_lanes);
             else
                 k_intersect<<<g, kBlock, lds, st>>>(SD, Wd, q, sd);
             kev_end(ctx, ks, st);
@@ -2375,7 +2378,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             if (march) {
                 const unsigned gm = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
-                k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, ctx->refill);
+                k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, refill_lanes);
             }
             kev_end(ctx, ks, st);
             // the shadow pass reads the NEE queue through Ws
@@ -2388,9 +2391,9 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             ks = kev_begin(ctx, 2, st);
             if (refill && SD.any_vbo)
-                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, ctx->refill);
+                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes);
             else if (refill)
-                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, ctx->refill);
+                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes);
             else
                 k_shadow<<<g, kBlock, lds, st>>>(SD, Ws, sd);
             kev_end(ctx, ks, st);
@@ -2846,6 +2849,7 @@ int64_t lucid_scene_grid_bytes(const lucid_scene* s) { return s ? (int64_t)s->gr
 
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->wave_paths = paths;
     return LUCID_OK;
 }
@@ -2928,6 +2932,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
 
 int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
     if (!ctx || idle_lanes < 1 || idle_lanes > 64) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->refill = idle_lanes;
     return LUCID_OK;
 }
@@ -2941,6 +2946,7 @@ int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on) {
 
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order) {
     if (!ctx || order < 0 || order > 2) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->ray_order = order;
     return LUCID_OK;
 }
@@ -2949,6 +2955,7 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
     if (!ctx || traverse_groups_per_cu < 0 || traverse_groups_per_cu > 64 || shade_groups_per_cu < 0 ||
         shade_groups_per_cu > 64)
         return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->traverse_groups = traverse_groups_per_cu;
     ctx->shade_groups = shade_groups_per_cu;
     return LUCID_OK;

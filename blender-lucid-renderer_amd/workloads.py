@@ -103,6 +103,46 @@ def closed_box(seed: int = 1) -> dict:
 CLOSED_BOX_CAMERA = {"pos": (0.0, 1.0, 3.6), "dir": (0.0, 0.0, -1.0), "up": (0.0, 1.0, 0.0), "fov": 40.0}
 
 
+def _sphere_mesh(center, radius, material, level=3):
+    """Icosphere of `radius` at `center` with exact sphere normals per corner
+    (smooth shading)."""
+    if level not in _ICO_CACHE:
+        _ICO_CACHE[level] = _icosphere(level)
+    v, t = _ICO_CACHE[level]
+    return _mesh(v * radius + np.asarray(center, np.float64), t, material, v[t], smooth=True)
+
+
+def glass_lights_box(seed: int = 3) -> dict:
+    """The closed box's walls with three spheres -- smooth glass
+    (transmission 1, IOR 1.5, roughness 0), rough glass (roughness 0.3) and a
+    partly transmissive sphere (transmission 0.4) -- lit by a sphere point
+    light (radius 0.08), a DISK and an ELLIPSE area light under the ceiling
+    and a spot light with a blended edge.  Covers the glass branch of
+    sampleBSDF and the light shapes the closed box does not have."""
+    rng = np.random.default_rng(seed)
+    walls = closed_box(seed)["meshes"][:5]
+    glass = {"base_color": [1.0, 1.0, 1.0], "metallic": 0.0, "roughness": 0.0, "emission": [0.0, 0.0, 0.0],
+             "transmission": 1.0, "ior": 1.5}
+    rough = dict(glass, roughness=0.3)
+    mixed = {"base_color": rng.uniform(0.3, 0.8, 3).tolist(), "metallic": 0.0, "roughness": 0.2,
+             "emission": [0.0, 0.0, 0.0], "transmission": 0.4, "ior": 1.45}
+    meshes = walls + [_sphere_mesh([-0.45, 0.38, -0.2], 0.36, glass),
+                      _sphere_mesh([0.45, 0.32, 0.25], 0.3, rough),
+                      _sphere_mesh([0.05, 1.15, -0.45], 0.25, mixed)]
+    down = {"direction": [0.0, -1.0, 0.0], "right": [1.0, 0.0, 0.0], "up": [0.0, 0.0, 1.0]}
+    lights = [
+        {"type": "POINT", "name": "Bulb", "position": [0.55, 1.55, 0.45], "color": [1.0, 0.9, 0.8], "energy": 25.0,
+         "radius": 0.08},
+        dict(down, type="AREA", name="Disk", shape="DISK", position=[-0.45, 1.98, 0.2], size=0.4, size_y=0.4,
+             color=[0.9, 0.95, 1.0], energy=6.0),
+        dict(down, type="AREA", name="Ellipse", shape="ELLIPSE", position=[0.35, 1.98, -0.5], size=0.5, size_y=0.25,
+             color=[1.0, 1.0, 0.9], energy=5.0),
+        {"type": "SPOT", "name": "Spot", "position": [-0.75, 1.9, 0.8], "direction": [0.45, -1.0, -0.55],
+         "color": [1.0, 0.95, 0.85], "energy": 40.0, "spot_size": 0.9, "spot_blend": 0.3},
+    ]
+    return {"meshes": meshes, "native_lights": lights, "environment": {"color": [0.03, 0.03, 0.04], "strength": 1.0}}
+
+
 def _icosphere(level: int):
     """Unit icosphere: (vertices (n,3), triangles (m,3)), 20*4^level triangles."""
     p = (1.0 + math.sqrt(5.0)) / 2.0

@@ -12,12 +12,15 @@
 
 import ctypes as C
 
+import math
+
 import numpy as np
 import pytest
 
 import guide_ref as G
 import lucid_capi
 import workloads
+import sigma
 from conftest import make_renderer
 
 pytestmark = pytest.mark.gpu
@@ -98,11 +101,12 @@ def test_guided_is_unbiased(lucid):
     p = _box_renderer(lucid, "mis")
     sp = p.debug_render_samples(0, 0, W, H, W, H, n, 50000, 8).astype(np.float64)
     mg, mp = sg.mean(1), sp.mean(1)
-    se = np.sqrt(sg.var(1, ddof=1) / n + sp.var(1, ddof=1) / n)
-    z = np.abs(mg - mp) / np.maximum(se, 1e-12)
-    ok = (z <= 3.0) | ((se < 1e-9) & (np.abs(mg - mp) < 1e-6))
-    assert 1.0 - ok.mean() < 0.02, f"{1 - ok.mean():.3f} of pixel channels beyond 3 sigma"
-    assert abs(mg.mean() / mp.mean() - 1.0) < 0.02
+    # two different estimators of one image: the fraction bound per pixel, and
+    # the image mean within 3 standard errors of the plain one
+    sigma.check(mg, sg.var(1, ddof=1), n, mp, sp.var(1, ddof=1), n, "guided vs plain", mean_z=False)
+    se_img = np.sqrt(sg.sum(axis=(0, 2)).var(ddof=1) + sp.sum(axis=(0, 2)).var(ddof=1)) / math.sqrt(n)
+    diff = sg.sum(axis=(0, 2)).mean() - sp.sum(axis=(0, 2)).mean()
+    assert abs(diff) < 3.0 * se_img, (diff, se_img)
 
 
 def test_guiding_gain_occluded_room(lucid):

@@ -26,6 +26,7 @@ import pytest
 
 import lucid_capi
 import workloads
+import sigma
 from conftest import make_oracle, make_renderer
 
 pytestmark = pytest.mark.gpu
@@ -74,14 +75,8 @@ def test_textured_box_with_panel_statistics_3sigma(lucid, algorithm):
     g = r.debug_render_samples(0, 0, W, H, W, H, n, 1000, 8).astype(np.float64)
     _, ex = o.render_tile(algorithm, 0, 0, W, H, W, H, samples=n, sample_offset=0, max_depth=8, stats=True)
     st = ex["stats"]
-    m_c = st[:, :3] / n
-    v_c = np.maximum(st[:, 3:] / n - m_c ** 2, 0) * n / (n - 1)
-    m_g = g.mean(axis=1)
-    v_g = g.var(axis=1, ddof=1)
-    se = np.sqrt(v_c / n + v_g / n)
-    z = np.abs(m_g - m_c) / np.maximum(se, 1e-12)
-    ok = (z <= 3.0) | (se < 1e-9) & (np.abs(m_g - m_c) < 1e-6)
-    assert 1.0 - ok.mean() < 0.02, f"{1.0 - ok.mean():.4f} of pixel channels outside 3 sigma"
+    m_c, v_c = sigma.from_sums(st[:, :3], st[:, 3:], n)
+    sigma.check(g.mean(axis=1), g.var(axis=1, ddof=1), n, m_c, v_c, n, f"textured box {algorithm}")
 
 
 def test_render_tile_textured_matches_oracle(lucid):

@@ -7,13 +7,16 @@ Bars (the tolerances are written in each test):
   at least 99 % of samples agree to 1e-3 relative and the tile sums agree to
   1 % relative L1;
 * per-pixel means over independent seeds agree within 3 sigma of the Monte
-  Carlo error (fraction of |z| > 3 pixels at the Gaussian rate + slack).
+  Carlo error: the fraction of |z| > 3 within a binomial bound on the t
+  rate, and the image-mean signed z within 4 standard errors of 0
+  (tests/sigma.py).
 """
 
 import numpy as np
 import pytest
 
 import workloads
+import sigma
 from conftest import make_oracle, make_renderer
 
 pytestmark = pytest.mark.gpu
@@ -104,15 +107,8 @@ def test_spheres_statistics_3sigma(lucid):
     g = r.debug_render_samples(0, 0, W, H, W, H, n, 1000, 12).astype(np.float64)
     _, ex = o.render_tile("mis", 0, 0, W, H, W, H, samples=n, sample_offset=0, max_depth=12, stats=True)
     st = ex["stats"]
-    m_c = st[:, :3] / n
-    v_c = np.maximum(st[:, 3:] / n - m_c ** 2, 0) * n / (n - 1)
-    m_g = g.mean(axis=1)
-    v_g = g.var(axis=1, ddof=1)
-    se = np.sqrt(v_c / n + v_g / n)
-    z = np.abs(m_g - m_c) / np.maximum(se, 1e-12)
-    ok = (z <= 3.0) | (se < 1e-9) & (np.abs(m_g - m_c) < 1e-6)
-    frac_bad = 1.0 - ok.mean()
-    assert frac_bad < 0.02, f"{frac_bad:.4f} of pixel channels outside 3 sigma"
+    m_c, v_c = sigma.from_sums(st[:, :3], st[:, 3:], n)
+    sigma.check(g.mean(axis=1), g.var(axis=1, ddof=1), n, m_c, v_c, n, "spheres mis")
 
 
 def test_env_only_scene_exact(lucid):

@@ -4,7 +4,8 @@ path_tracer.hpp:209-283, 586-835) on the smoke-box workload.
 Bars: per-sample radiance on the same seed streams agrees to 1e-3 relative
 for at least 97 % of samples (delta tracking amplifies last-ulp differences of
 the device log() into occasional different collision points), tile sums to
-2 % relative L1, and independent-seed pixel means within 3 sigma."""
+2 % relative L1, and independent-seed pixel means within 3 sigma (the
+fraction and image-mean bounds of tests/sigma.py)."""
 
 import os
 
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 import workloads
+import sigma
 from conftest import make_oracle, make_renderer
 
 pytestmark = pytest.mark.gpu
@@ -58,13 +60,8 @@ def test_smoke_statistics_3sigma(lucid, smoke):
     g = r.debug_render_samples(0, 0, W, H, W, H, n, 5000, 8).astype(np.float64)
     _, ex = o.render_tile("volume_mis", 0, 0, W, H, W, H, samples=n, max_depth=8, stats=True)
     st = ex["stats"]
-    m_c = st[:, :3] / n
-    v_c = np.maximum(st[:, 3:] / n - m_c ** 2, 0) * n / (n - 1)
-    m_g, v_g = g.mean(axis=1), g.var(axis=1, ddof=1)
-    se = np.sqrt(v_c / n + v_g / n)
-    z = np.abs(m_g - m_c) / np.maximum(se, 1e-12)
-    ok = (z <= 3.0) | ((se < 1e-9) & (np.abs(m_g - m_c) < 1e-6))
-    assert 1.0 - ok.mean() < 0.02
+    m_c, v_c = sigma.from_sums(st[:, :3], st[:, 3:], n)
+    sigma.check(g.mean(axis=1), g.var(axis=1, ddof=1), n, m_c, v_c, n, "smoke volume_mis")
 
 
 def test_volume_algorithms_reduce_to_surface_ones_without_media(lucid):

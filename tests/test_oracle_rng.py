@@ -111,3 +111,16 @@ def test_bits_balanced_and_seeds_distinct():
             L.orc_seed(x, y, 0, C.byref(a), C.byref(b))
             firsts.add(_stream(a.value, b.value, 1)[0])
     assert len(firsts) == 64                                       # XYZSeeding_DifferentPixels
+
+
+def test_jenkins_matches_reference_vectors():
+    """jenkins_hash (random.hpp:49-57) on 328 inputs: the fixture was
+    evaluated from the reference's own statements, parsed from its text by
+    tests/golden/make_jenkins_vectors.py."""
+    import json
+    import os
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "jenkins32_vectors.json")))
+    assert len(fx["statements"]) == 6 and len(fx["pairs"]) > 300
+    bad = [(a, b) for a, b in fx["pairs"] if L.orc_jenkins(a) != b]
+    assert not bad, f"{len(bad)} mismatches, first {bad[:3]}"
