@@ -109,7 +109,14 @@ struct WaveDev {
     float4* tslot;
     int32_t tstride;
     int64_t tnrec;  // records (paths with slots); slot arrays are [k * 5 + field][record]
+    // render_tile: the context's cancel word in host memory the device reads
+    // (null elsewhere).  Set, it ends every path at its next shading; the
+    // film then skips the wave (cnt[7] latches what each wave saw).
+    const int32_t* stop;
 };
+__device__ __forceinline__ bool stop_requested(const WaveDev& W) {
+    return W.stop && *reinterpret_cast<const volatile int32_t*>(W.stop) != 0;
+}
 constexpr int kTrainSlots = 6;
 // Paths that keep training slots: one per block of `tstride` consecutive
 // paths, at a hashed offset so that every pixel column contributes.
@@ -890,7 +897,7 @@ __device__ __forceinline__ void put_record(const WaveDev& W, int p, int slot, v3
 // their registers out of the kernels of scenes without them).
 template <int ALGO, bool NODES>
 __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, const guide::GuideDev& G, int qin,
-                                          int32_t max_depth, int i, int n) {
+                                          int32_t max_depth, int i, int n, bool stop) {
     constexpr bool DIAG = ALGO >= kAlgoDiagBase;
     constexpr bool GUIDED = ALGO == LUCID_ALGO_GUIDED_MIS;
     constexpr int A = DIAG ? ALGO - kAlgoDiagBase : (GUIDED ? LUCID_ALGO_MIS : ALGO);
@@ -1232,6 +1239,9 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
         W.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
         if (DIAG) W.dcnt[p] = nvert | ((depth + 1) << 8);
     }
+    // a cancelled render ends its paths here (no next bounce, no shadow ray)
+    cont = cont && !stop;
+    nee = nee && !stop;
     int slot = wave_append(&W.cnt[1 - qin], cont);
     if (cont) W.q[1 - qin][slot] = p;
     int sslot = wave_append(&W.cnt[2], nee);
@@ -1246,8 +1256,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
     if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    // cancel requests are polled once per launch, i.e. between bounces;
+    // wavefront-uniform, so the ballots in shade_one stay convergent
+    const bool stop = __builtin_amdgcn_readfirstlane(stop_requested(W) ? 1 : 0) != 0;
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
-        shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n);
+        shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n, stop);
 }
 
 template <int ALGO>
@@ -1282,6 +1295,18 @@ __device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float
 // MFMA operands are then gathered from LDS, two 8-byte runs of 4 features
 // each, as 4 tiles of 16 that share every weight load.
 // Paths without a surface hit run on zero input and are not written.
+// Surface point and shading normal of every traced path (into sh_o / sh_d),
+// for the inference test hook.
+__global__ __launch_bounds__(256) void k_debug_frame(SceneDev S, WaveDev W, int32_t n) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= n) return;
+    const float4 hv = W.hit[p];
+    v3f point = mk(0.0f, 0.0f, 0.0f), nrm = point;
+    if (__float_as_int(hv.y) >= 0) hit_frame(S, mk(W.ray_o[p]), mk(W.ray_d[p]), hv, point, nrm);
+    W.sh_o[p] = make_float4(point.x, point.y, point.z, 0.0f);
+    W.sh_d[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+}
+
 constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 // Wavefronts per workgroup: they share one LDS copy of the weights (31 KB);
 // with their feature rows a workgroup takes 65 KB, two per CU.
@@ -1462,9 +1487,16 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W, int sd
 }
 
 // ------------------------------------------------------------------ film --
+// Latches the cancel word once per wave, after its last shading launch:
+// cnt[7] = 1 once a cancel has been seen.  Every later kernel of the render
+// reads the latch, so all of them agree on one value.
+__global__ __launch_bounds__(64) void k_latch_stop(WaveDev W) {
+    if (threadIdx.x == 0 && stop_requested(W)) W.cnt[7] = 1;
+}
 __global__ __launch_bounds__(256) void k_film(WaveDev W, float4* film, int32_t P, int32_t B) {
     int pix = blockIdx.x * 256 + threadIdx.x;
     if (pix >= P) return;
+    if (W.stop && W.cnt[7]) return;  // a cancelled wave: the film keeps the complete waves only
     float4 a = film[pix];
     for (int b = 0; b < B; ++b) {
         float4 r = W.rad[(size_t)b * P + pix];
@@ -1617,9 +1649,12 @@ __global__ __launch_bounds__(256) void k_samples_out(WaveDev W, float* dst, int3
 }
 // apply_camera_sensitivity + clamp + alpha + Y flip (pybind_renderer.hpp:331-360)
 __global__ __launch_bounds__(256) void k_output(const float4* film, float* out, int32_t tw, int32_t th, float sens,
-                                                int32_t flip) {
+                                                int32_t flip, const int32_t* latch) {
     int pix = blockIdx.x * 256 + threadIdx.x;
     if (pix >= tw * th) return;
+    // a cancelled render returns its partial sums in row order, unflipped
+    // (pybind_renderer.hpp:225-233 returns before the flip of :351-360)
+    if (latch && *latch) flip = 0;
     int px = pix % tw, py = pix / tw;
     float4 a = film[pix];
     float r = smax(a.x * sens, 0.0f), g = smax(a.y * sens, 0.0f), b = smax(a.z * sens, 0.0f);
@@ -1806,6 +1841,10 @@ struct lucid_ctx {
     int32_t lanes_ready = 1;  // lanes with a stream (lane 0 always)
     std::string err;
     std::atomic<int> cancel{0};
+    // the cancel word the kernels of render_tile read: pinned host memory
+    // mapped into the device's address space
+    int32_t* cancel_host = nullptr;
+    const int32_t* cancel_dev = nullptr;
     std::mutex mu;
     int64_t wave_paths = 0;
     // Launch shape: workgroups per CU of the persistent kernels, or 0 for
@@ -2255,6 +2294,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                                     : INT64_MAX;
     const int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
+    W.stop = (mode == 0 && cancel_flag == &ctx->cancel && !capturing) ? ctx->cancel_dev : nullptr;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
         ctx->kev_used = 0;
         k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
@@ -2407,6 +2447,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             kev_end(ctx, ks, st);
         }
         if (mode == 0 || mode == 3) {
+            if (W.stop) k_latch_stop<<<1, 64, 0, st>>>(W);
             k_film<<<blocks(P, 256), 256, 0, st>>>(W, film, (int32_t)P, b);
             if (diag && dsw > 0 && dsh > 0)
                 k_diag_film<<<blocks((int64_t)dsw * dsh, 64), 64, 0, st>>>(
@@ -2512,6 +2553,16 @@ int lucid_create(int32_t device, lucid_ctx** out) {
         return LUCID_ERR_DEVICE;
     }
     ctx->lanes[0].stream = ctx->stream;
+    void* dp = nullptr;
+    if (hipHostMalloc((void**)&ctx->cancel_host, 64, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer(&dp, ctx->cancel_host, 0) != hipSuccess) {
+        if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return LUCID_ERR_DEVICE;
+    }
+    __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_SEQ_CST);
+    ctx->cancel_dev = static_cast<const int32_t*>(dp);
     *out = ctx;
     return LUCID_OK;
 }
@@ -2543,6 +2594,7 @@ void lucid_destroy(lucid_ctx* ctx) {
     }
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
     delete ctx;
 }
 
@@ -2675,7 +2727,9 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
     if (!out_rgba_dev) return fail(ctx, LUCID_ERR_INVALID, "null output");
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_OK(hipSetDevice(ctx->device));
+    // a cancel that came before the render is dropped, as render_tile does
     ctx->cancel.store(0);
+    __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_SEQ_CST);
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     HIP_OK(hipEventRecord(ctx->ev0, st));
     bool cancelled = false;
@@ -2684,7 +2738,7 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
     CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
     int64_t P = (int64_t)pr->tile_w * pr->tile_h;
     k_output<<<blocks(P, 256), 256, 0, st>>>(ctx->film, out_rgba_dev, pr->tile_w, pr->tile_h, cb.sensitivity,
-                                             cancelled ? 0 : 1);
+                                             cancelled ? 0 : 1, ctx->lanes[0].W.cnt + 7);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(ctx->ev1, st));
     return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
@@ -2711,8 +2765,11 @@ int lucid_render_tile(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* ca
     int rrc = lucid_render_tile_device(ctx, scene, cam, pr, ctx->out_dev, nullptr);
     if (rrc != LUCID_OK && rrc != LUCID_ERR_CANCELLED) return rrc;
     std::lock_guard<std::mutex> lock(ctx->mu);
+    int32_t latch = 0;
     HIP_OK(hipMemcpyAsync(out_rgba, ctx->out_dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipMemcpyAsync(&latch, ctx->lanes[0].W.cnt + 7, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(hipStreamSynchronize(ctx->stream));
+    if (latch) rrc = LUCID_ERR_CANCELLED;  // the device saw the cancel: partial, unflipped
     rc = finish_stats(ctx);
     return rc ? rc : rrc;
 }
@@ -2781,11 +2838,15 @@ int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
 }
 
 void lucid_cancel(lucid_ctx* ctx) {
-    if (ctx) ctx->cancel.store(1);
+    if (!ctx) return;
+    ctx->cancel.store(1);
+    if (ctx->cancel_host) __atomic_store_n(ctx->cancel_host, 1, __ATOMIC_SEQ_CST);
 }
 int32_t lucid_is_cancelled(const lucid_ctx* ctx) { return ctx ? ctx->cancel.load() : 0; }
 void lucid_reset_cancel(lucid_ctx* ctx) {
-    if (ctx) ctx->cancel.store(0);
+    if (!ctx) return;
+    ctx->cancel.store(0);
+    if (ctx->cancel_host) __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_SEQ_CST);
 }
 
 int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out) {
@@ -3133,6 +3194,84 @@ int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* 
     guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
     ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
+    return LUCID_OK;
+}
+
+// Test hook of the render-path inference (k_guide_infer): n rays are traced
+// by the closest-hit kernel, then the network runs over the queue `order`
+// (m path indices, in that order) exactly as a guided bounce does.
+int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const float* params, int32_t n,
+                                     const float* rays, const int32_t* order, int32_t m, float* out, float* frame,
+                                     float* box) {
+    if (!ctx || !scene || !params || !rays || !order || !out || !frame || !box || n < 1 || m < 0 || m > n ||
+        n > (1 << 24))
+        return LUCID_ERR_INVALID;
+    for (int32_t i = 0; i < m; ++i)
+        if (order[i] < 0 || order[i] >= n) return fail(ctx, LUCID_ERR_INVALID, "queue entry out of range");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
+    int rc = ensure_guide(ctx);
+    if (!rc) rc = ensure_wave(ctx, ctx->lanes[0], n);
+    if (!rc) rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, (int64_t)n * guide::kOut);
+    if (rc) return rc;
+    if (guide::set_params(ctx->gstate, params, ctx->stream)) return fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed");
+    ctx->gsteps = std::max<int64_t>(ctx->gsteps, 1);
+    hipStream_t st = ctx->stream;
+    WaveDev W = ctx->lanes[0].W;
+    W.gout = ctx->gout;
+    W.dcnt = nullptr;
+    W.count_work = 0;
+    std::vector<float4> o4((size_t)n), d4((size_t)n);
+    for (int32_t i = 0; i < n; ++i) {
+        o4[i] = make_float4(rays[6 * (size_t)i], rays[6 * (size_t)i + 1], rays[6 * (size_t)i + 2], 0.0f);
+        d4[i] = make_float4(rays[6 * (size_t)i + 3], rays[6 * (size_t)i + 4], rays[6 * (size_t)i + 5], 0.0f);
+    }
+    HIP_OK(hipMemcpyAsync(W.ray_o, o4.data(), (size_t)n * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(W.ray_d, d4.data(), (size_t)n * 16, hipMemcpyHostToDevice, st));
+    k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
+    // every path traced (queue 1 = 0..n-1), then the network over `order` (queue 0)
+    std::vector<int32_t> all((size_t)n);
+    for (int32_t i = 0; i < n; ++i) all[i] = i;
+    HIP_OK(hipMemcpyAsync(W.q[1], all.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    k_set_i32<<<1, 64, 0, st>>>(W.cnt + 1, n);
+    const int sd = std::min(kStackMax, scene->host.max_depth + 2);
+    const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
+    k_trace_refill<0><<<g, kBlock, (size_t)sd * kBlock * sizeof(int32_t), st>>>(scene->dev, W, 1, sd, 48);
+    if (m > 0) HIP_OK(hipMemcpyAsync(W.q[0], order, (size_t)m * 4, hipMemcpyHostToDevice, st));
+    k_set_i32<<<1, 64, 0, st>>>(W.cnt, m);
+    // outputs start as NaN: entries the kernel must not write stay NaN
+    HIP_OK(hipMemsetAsync(W.gout, 0xff, (size_t)n * guide::kOut * sizeof(__half), st));
+    const guide::GuideDev G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gcfg.alpha);
+    const unsigned gg = (unsigned)std::min<int64_t>(blocks(std::max<int32_t>(m, 1), kInferWaves * 64),
+                                                    (int64_t)ctx->num_cu * 2);
+    k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(scene->dev, W, G, 0);
+    k_debug_frame<<<blocks(n, 256), 256, 0, st>>>(scene->dev, W, n);
+    HIP_OK(hipGetLastError());
+    std::vector<__half> h((size_t)n * guide::kOut);
+    std::vector<float4> hit((size_t)n);
+    HIP_OK(hipMemcpyAsync(h.data(), W.gout, h.size() * sizeof(__half), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hit.data(), W.hit, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(o4.data(), W.sh_o, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(d4.data(), W.sh_d, (size_t)n * 16, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < h.size(); ++i) out[i] = __half2float(h[i]);
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t prim;
+        std::memcpy(&prim, &hit[i].y, 4);
+        const bool hit_any = prim >= 0;
+        const float nan = std::nanf("");
+        frame[6 * (size_t)i + 0] = hit_any ? o4[i].x : nan;
+        frame[6 * (size_t)i + 1] = hit_any ? o4[i].y : nan;
+        frame[6 * (size_t)i + 2] = hit_any ? o4[i].z : nan;
+        frame[6 * (size_t)i + 3] = hit_any ? d4[i].x : nan;
+        frame[6 * (size_t)i + 4] = hit_any ? d4[i].y : nan;
+        frame[6 * (size_t)i + 5] = hit_any ? d4[i].z : nan;
+    }
+    for (int k = 0; k < 3; ++k) {
+        box[k] = scene->bmin[k];
+        box[3 + k] = scene->bmax[k];
+    }
     return LUCID_OK;
 }
 

@@ -150,6 +150,10 @@ SIGNATURES = {
     "lucid_guiding_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32]),
     "lucid_guiding_debug_infer": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
                                             C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "lucid_guiding_debug_render_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_int32,
+                                                   C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32,
+                                                   C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                   C.POINTER(C.c_float)]),
     "lucid_guiding_debug_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32,
                                            C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                            C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -240,8 +240,14 @@ int lucid_primary_hits(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
 int lucid_material_eval(const lucid_mesh_desc* mesh, int32_t count, const float* uv, float* out, char* err,
                         int32_t err_len);
 
-/* Cooperative cancel, callable from any thread; polled between wavefront
-   batches.  render entry points clear the flag on entry. */
+/* Cooperative cancel, callable from any thread (PyRenderer::cancel,
+   include/pybind_renderer.hpp:225-245, which polls per 8x8 micro-tile).
+   The host polls it between wavefront batches; render_tile's kernels read it
+   from mapped host memory once per bounce, and a set flag ends every path at
+   its next shading, so a cancel lands within one bounce.  The film then
+   keeps the batches completed before the cancel was seen, and the rows are
+   left unflipped (top row first), as the reference's early return leaves
+   them.  render entry points clear the flag on entry. */
 void lucid_cancel(lucid_ctx* ctx);
 int32_t lucid_is_cancelled(const lucid_ctx* ctx);
 void lucid_reset_cancel(lucid_ctx* ctx);
@@ -394,6 +400,16 @@ int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* s
    wavefront) on the same sample format: raw outputs (n x 40).  For tests. */
 int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
                               const float bmin[3], const float bmax[3], float* out);
+/* The inference kernel of guided renders (k_guide_infer), run as a guided
+   bounce runs it: n rays (6 floats each: origin, direction) are traced by
+   the closest-hit kernel, then the network with `params` runs over the queue
+   `order` (m path indices, any order, each < n).  out: n x 40 raw outputs
+   (FP16 widened; NaN for paths the kernel must not write: not queued, or
+   missed), frame: n x 6 (hit point, shading normal; NaN on a miss), box: the
+   scene box the features are normalised by (min[3], max[3]).  For tests. */
+int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const float* params, int32_t n,
+                                     const float* rays, const int32_t* order, int32_t m, float* out, float* frame,
+                                     float* box);
 
 /* Runs the device BVH traversal code on the CPU over a scene packed from
    `meshes` (no device needed), checking every node and triangle index before

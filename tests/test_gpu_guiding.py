@@ -2,7 +2,8 @@
 
 * The training kernels' forward outputs, loss and parameter gradient equal
   the CPU reference (oracle/guide_ref.py, BF16 mode: same roundings, float64
-  sums) on the same parameters and samples.
+  sums) on the same parameters and samples; so do the outputs of the
+  render-path inference kernel on a traced, shuffled queue.
 * Online training lowers the loss of the guide on the closed box.
 * The guided estimator is unbiased: per-pixel means agree with plain MIS
   within 3 sigma (independent seeds).
@@ -71,6 +72,49 @@ def test_render_inference_matches_reference():
     x = G.features(smp[:, 0:3], smp[:, 3:6], smp[:, 6:9], box[0], box[1])
     _, o_ref = G.forward(params.astype(np.float64), x, bf=True)
     assert np.max(np.abs(out - o_ref[:, :40])) < 2e-2 * np.max(np.abs(o_ref))
+
+
+def test_render_path_inference_kernel_matches_reference():
+    """k_guide_infer itself (the kernel guided renders run, not the training
+    forward pass): 300,007 rays traced in the closed box, the network run over
+    a shuffled queue of 287,113 of them -- not a multiple of the 256-lane
+    workgroup, more than one grid stride so every workgroup makes several
+    pipelined trips, with misses among the entries (path 0 among them).
+    Queued hits match the float64 BF16-mode reference on the device's own
+    hit frames; every other output stays untouched (NaN)."""
+    rng = np.random.default_rng(31)
+    params = G.init_params(seed=32, scale_out=1.0).astype(np.float32)
+    n, m = 300007, 287113
+    o = rng.uniform([-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[: n // 10, 2] = np.abs(d[: n // 10, 2]) + 2.0  # out through the open front: misses
+    d[: n // 10] /= np.linalg.norm(d[: n // 10], axis=1, keepdims=True)
+    o[:50], d[:50] = [0.0, 1.0, 5.0], [0.0, 0.0, 1.0]  # from outside, away: misses, path 0 first
+    rays = np.ascontiguousarray(np.concatenate([o, d], axis=1), np.float32)
+    order = rng.permutation(n)[:m].astype(np.int32)
+    order[0] = 0
+    ctx = lucid_capi.Context(0)
+    sh = ctx.scene(workloads.closed_box())
+    out = np.zeros((n, 40), np.float32)
+    frame = np.zeros((n, 6), np.float32)
+    box = np.zeros(6, np.float32)
+    ctx.check(ctx.L.lucid_guiding_debug_render_infer(ctx.h, sh, F(params), n, F(rays),
+                                                     order.ctypes.data_as(C.POINTER(C.c_int32)), m, F(out), F(frame),
+                                                     F(box)))
+    ctx.L.lucid_scene_release(sh)
+    ctx.close()
+    hit = np.isfinite(frame[:, 0])
+    assert not hit[0] and 0.05 < 1 - hit.mean() < 0.2
+    queued = np.zeros(n, bool)
+    queued[order] = True
+    live = queued & hit
+    assert np.isnan(out[~live]).all(), "outputs written for paths not queued or missed"
+    assert np.isfinite(out[live]).all()
+    x = G.features(frame[live, :3], frame[live, 3:6], -rays[live, 3:6], box[:3], box[3:])
+    _, o_ref = G.forward(params.astype(np.float64), x, bf=True)
+    err = np.abs(out[live] - o_ref[:, :40])
+    assert err.max() < 2e-2 * np.max(np.abs(o_ref)), err.max()
 
 
 def _box_renderer(lucid, algo):

@@ -8,15 +8,21 @@ Per pixel channel z = (m_gpu - m_ref) / sqrt(v (1/n_gpu + 1/n_ref)), with v
 the reference's own per-sample variance (known to 0.3 %), so z has mean 0 and
 variance 1 under the null hypothesis whatever the shape of the pixel's
 sample distribution.  This scene's samples are heavy-tailed (light arrives
-through a 5 cm slit; per-sample relative variance up to ~170), so a single
-pixel's z is far from Gaussian at 1,024 spp and the bars are placed where
-the central limit theorem holds:
+through a 5 cm slit; per-sample relative variance up to ~170): a rare path
+through the slit carries a large value, so a pixel's z has a long upper tail
+at 1,024 spp (one such firefly can put a 256-pixel block at z = +5), while
+the lower tail of a mean of non-negative samples is no heavier than the
+Gaussian one.  The bars use what holds whatever the skew:
 
 * the image-mean z over all 16,384 pixels (mean of independent unit-variance
-  terms): within 4 / sqrt(16,384);
+  terms): within 3 / sqrt(16,384) -- a bias of 0.03 sigma per pixel fails it;
 * the z of the whole subset's sum (about 5e7 samples): within 4;
-* the z of each of 64 blocks of 256 pixels (2 subset rows each): the
-  fraction beyond 3 within the binomial bound of tests/sigma.py.
+* the z of each of 8 regions of 2,048 pixels (16 subset rows each): every
+  one within 4.5 (Bonferroni over 8 at about 5e-4);
+* per pixel, the lower tail: the fraction of channels with z < -3 within
+  four binomial standard errors of the one-sided Gaussian rate 0.00135 -- a
+  tracer that loses energy anywhere fails it; the fraction beyond +3 is
+  reported.
 """
 
 import math
@@ -56,19 +62,21 @@ def test_plain_tracer_within_3sigma_of_cpu_reference(lucid):
     m_pix = len(xs)
     # image-mean z: channels of a pixel are correlated, so the pixel count is the trial count
     zp = z.mean(axis=1)
-    assert abs(zp.mean()) < 4.0 / math.sqrt(m_pix) * max(1.0, zp.std()), f"image-mean z {zp.mean():+.4f}"
+    assert abs(zp.mean()) < 3.0 / math.sqrt(m_pix), f"image-mean z {zp.mean():+.4f}"
     # the whole subset's sum
     var_sum = (v_ref * (1.0 / n_gpu + 1.0 / n_ref)).sum()
     z_all = (m_gpu - m_ref).sum() / math.sqrt(var_sum)
     assert abs(z_all) < 4.0, f"subset-sum z {z_all:+.3f}"
-    # 64 blocks of 256 pixels
-    blk = np.arange(m_pix) // 256
-    zb = np.array([(m_gpu - m_ref)[blk == b].sum() / math.sqrt((se[blk == b] ** 2).sum()) for b in range(64)])
-    p0 = 2.0 * (1.0 - 0.5 * (1.0 + math.erf(3.0 / math.sqrt(2.0))))
-    bound = p0 + 4.0 * math.sqrt(p0 * (1.0 - p0) / 64) + 1.0 / 64
-    frac = float(np.mean(np.abs(zb) > 3.0))
-    assert frac <= bound, f"{frac:.3f} of blocks beyond 3 sigma (bound {bound:.3f}); block z {np.round(zb, 2)}"
+    # 8 regions of 2,048 pixels
+    reg = np.arange(m_pix) // 2048
+    zr = np.array([(m_gpu - m_ref)[reg == b].sum() / math.sqrt((se[reg == b] ** 2).sum()) for b in range(8)])
+    assert np.abs(zr).max() < 4.5, f"region z {np.round(zr, 2)}"
+    # per pixel: the lower tail
+    lo, hi = float(np.mean(z[live] < -3.0)), float(np.mean(z[live] > 3.0))
+    p1 = 0.00135
+    assert lo <= p1 + 4.0 * math.sqrt(p1 * (1.0 - p1) / m_pix), f"{lo:.4f} of channels below -3 sigma"
     # relMSE of the 1,024-spp estimate against the FP64 oracle (stated, not a bar)
     rel = workloads.relmse(m_gpu, m_ref)
-    print(f"occluded room 1024 spp: image-mean z {zp.mean():+.4f}, subset-sum z {z_all:+.3f}, "
-          f"blocks beyond 3 sigma {frac:.3f}, relMSE {rel:.4f} (oracle's own {float(d['self_relmse']):.5f})")
+    print(f"occluded room 1024 spp: image-mean z {zp.mean():+.4f}, subset-sum z {z_all:+.3f}, region z "
+          f"{np.round(zr, 2)}, channels below -3 sigma {lo:.4f}, above +3 sigma {hi:.4f}, relMSE {rel:.4f} "
+          f"(oracle's own {float(d['self_relmse']):.5f})")
