@@ -1698,6 +1698,107 @@ __global__ __launch_bounds__(256) void k_hits_out(SceneDev S, WaveDev W, int32_t
 }
 
 
+// ---------------------------------------------------------- debug passes --
+// render_debug (pybind_renderer.hpp:371-439): the closest hit of one
+// unjittered camera ray per pixel, found by intersectScene from t = 0 (no
+// volume-boundary skipping), shown as the shading normal mapped to
+// n * 0.5 + 0.5 (traceNormal, renderer.hpp:58-65), the material's albedo
+// (traceAlbedo :67-78), its emission (traceEmission :80-93) or a medium's
+// thickness (traceVolume :109-190); any other mode is magenta.
+constexpr int32_t kDebugNormal = 0, kDebugAlbedo = 1, kDebugEmission = 2, kDebugVolume = 3;
+__device__ __forceinline__ bool medium_present(const GVolume& v) { return v.density > 0.0f || v.has_grid; }
+// traceVolume's second query: a camera ray whose closest hit is a mesh with
+// a medium is traced again from 1e-4 past the entry point (the first hit is
+// kept in sh_o).  Queue 1 gets those paths.
+__global__ __launch_bounds__(256) void k_debug_vol_prep(SceneDev S, WaveDev W, int32_t n) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    bool again = false;
+    if (p < n) {
+        const float4 hv = W.hit[p];
+        W.sh_o[p] = hv;
+        const int32_t prim = __float_as_int(hv.y);
+        if (prim >= 0) {
+            const int32_t vi = S.mats[prim_mesh(S, prim)].volume;
+            if (vi >= 0 && medium_present(S.vols[vi])) {
+                const v3f o = mk(W.ray_o[p]), d = mk(W.ray_d[p]);
+                const v3f entry = o + d * hv.x;
+                const v3f in = entry + d * 1e-4f;
+                W.ray_o[p] = make_float4(in.x, in.y, in.z, 0.0f);
+                again = true;
+            }
+        }
+    }
+    const int slot = wave_append(&W.cnt[1], again);
+    if (again) W.q[1][slot] = p;
+}
+__global__ __launch_bounds__(256) void k_debug_out(SceneDev S, WaveDev W, float* out, int32_t tw, int32_t th,
+                                                   int32_t mode) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= tw * th) return;
+    const int px = p % tw, py = p / tw;
+    const float4 hv = mode == kDebugVolume ? W.sh_o[p] : W.hit[p];
+    const int32_t prim = __float_as_int(hv.y);
+    v3f c = mk(0.0f, 0.0f, 0.0f);
+    if (mode == kDebugNormal) {
+        v3f nrm = c;
+        if (prim >= 0) {
+            v3f point;
+            hit_frame(S, mk(W.ray_o[p]), mk(W.ray_d[p]), hv, point, nrm);
+        }
+        c = mk(nrm.x * 0.5f + 0.5f, nrm.y * 0.5f + 0.5f, nrm.z * 0.5f + 0.5f);
+    } else if ((mode == kDebugAlbedo || mode == kDebugEmission) && prim >= 0) {
+        const GMat& gm = S.mats[prim_mesh(S, prim)];
+        const int32_t prog = mode == kDebugAlbedo ? gm.prog_albedo : gm.prog_emission;
+        c = mode == kDebugAlbedo ? mk(gm.albedo[0], gm.albedo[1], gm.albedo[2])
+                                 : mk(gm.emission[0], gm.emission[1], gm.emission[2]);
+        if (S.prog && (gm.flags & MF_NODES) && prog >= 0) {
+            // the hit's texture coordinate (geometry/bvh.hpp:67-69)
+            const float2* tu = S.tuv + 3 * (size_t)prim;
+            const float w = 1.0f - hv.z - hv.w;
+            const float hu = tu[0].x * w + tu[1].x * hv.z + tu[2].x * hv.w;
+            const float hv2 = tu[0].y * w + tu[1].y * hv.z + tu[2].y * hv.w;
+            c = run_node_program(S.prog + prog, hu, hv2);
+        }
+    } else if (mode == kDebugVolume && prim >= 0) {
+        const int32_t vi = S.mats[prim_mesh(S, prim)].volume;
+        if (vi >= 0 && medium_present(S.vols[vi])) {
+            const GVolume& v = S.vols[vi];
+            const float4 ex = W.hit[p];  // the query from inside
+            const bool exit_hit = __float_as_int(ex.y) >= 0;
+            if (v.has_grid) {
+                // ray march at half a voxel, max sampled density vs opacity
+                const float t_inside = exit_hit ? ex.x : 0.0f;
+                if (t_inside > 0.0f) {
+                    const float extent = max3(v.wmax[0] - v.wmin[0], v.wmax[1] - v.wmin[1], v.wmax[2] - v.wmin[2]);
+                    const int max_dim = max(max(v.dims[0], v.dims[1]), v.dims[2]);
+                    const float voxel = extent / (float)max(1, max_dim);
+                    const float step = smax(1e-4f, voxel * 0.5f);
+                    const int n_steps = min(2048, (int)(t_inside / step) + 1);
+                    const v3f ro = mk(W.ray_o[p]), rd = mk(W.ray_d[p]);
+                    float integrated = 0.0f, max_sampled = 0.0f;
+                    for (int i = 0; i < n_steps; ++i) {
+                        const float t = ((float)i + 0.5f) * step;
+                        if (t > t_inside) break;
+                        const float d = grid_value(S, v, ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
+                        integrated += d * step;
+                        if (d > max_sampled) max_sampled = d;
+                    }
+                    const float opacity = 1.0f - expf(-v.density * integrated * 10.0f);
+                    const float vis = smax(max_sampled, opacity);
+                    c = mk(v.color[0] * vis, v.color[1] * vis, v.color[2] * vis);
+                }
+            } else {
+                const float t = exit_hit ? 1.0f - expf(-v.density * ex.x) : 1.0f - expf(-v.density);
+                c = mk(v.color[0] * t, v.color[1] * t, v.color[2] * t);
+            }
+        }
+    } else if (mode < kDebugNormal || mode > kDebugVolume) {
+        c = mk(1.0f, 0.0f, 1.0f);
+    }
+    const size_t k = ((size_t)(th - 1 - py) * tw + px) * 4;
+    *(float4*)(out + k) = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
 // ------------------------------------------------------------ diag film --
 // A recorded completed path, read from the wave's diagnostic storage: the
 // first `prefix` path vertices, then the light vertex when the path had room
@@ -2772,6 +2873,60 @@ int lucid_render_tile(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* ca
     if (latch) rrc = LUCID_ERR_CANCELLED;  // the device saw the cancel: partial, unflipped
     rc = finish_stats(ctx);
     return rc ? rc : rrc;
+}
+
+int lucid_render_debug(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
+                       int32_t mode, float* out_rgba) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!out_rgba) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_OK(hipSetDevice(ctx->device));
+    ctx->cancel.store(0);
+    const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
+    Lane& L = ctx->lanes[0];
+    rc = ensure_wave(ctx, L, P);
+    if (!rc) rc = ensure_buf(ctx, &ctx->out_dev, &ctx->out_cap, P * 4);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    SceneDev SD = scene->dev;
+    SD.any_vbo = 0;  // intersectScene itself
+    WaveDev W = L.W;
+    W.dcnt = nullptr;
+    W.tslot = nullptr;
+    W.tstride = 1;
+    W.gout = nullptr;
+    W.count_work = 0;
+    W.stop = nullptr;
+    CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
+    CamDev cd;
+    std::memcpy(cd.pos, cb.pos, 12); std::memcpy(cd.fwd, cb.fwd, 12);
+    std::memcpy(cd.right, cb.right, 12); std::memcpy(cd.up, cb.up, 12);
+    cd.aspect = cb.aspect; cd.scale = cb.scale;
+    cd.fw = pr->full_w; cd.fh = pr->full_h; cd.tx = pr->tile_x; cd.ty = pr->tile_y;
+    cd.tw = pr->tile_w; cd.th = pr->tile_h;
+    cd.samples_in_call = 1;  // no jitter
+    cd.s0 = 0;
+    cd.P = (int32_t)P;
+    cd.B = 1;
+    cd.max_depth = 1;
+    cd.s0p = nullptr;
+    const int sd = std::min(kStackMax, scene->host.max_depth + 2);
+    const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
+    const unsigned g = (unsigned)std::min<int64_t>(blocks(P, kBlock), (int64_t)ctx->num_cu * 16);
+    k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
+    k_raygen<<<blocks(P, kBlock), kBlock, 0, st>>>(W, cd, (int32_t)P);
+    k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48);
+    if (mode == kDebugVolume) {
+        k_debug_vol_prep<<<blocks(P, 256), 256, 0, st>>>(SD, W, (int32_t)P);
+        k_set_i32<<<1, 64, 0, st>>>(W.cnt + 4, 0);  // fetch counter of the second query
+        k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48);
+    }
+    k_debug_out<<<blocks(P, 256), 256, 0, st>>>(SD, W, ctx->out_dev, pr->tile_w, pr->tile_h, mode);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out_rgba, ctx->out_dev, (size_t)P * 16, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return LUCID_OK;
 }
 
 int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,

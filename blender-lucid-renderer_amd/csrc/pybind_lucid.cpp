@@ -514,6 +514,27 @@ public:
         return px;
     }
 
+    // render_debug / render_debug_gpu (pybind_renderer.hpp:371-520): one
+    // unjittered camera ray per pixel, coloured by `mode` ("normal",
+    // "albedo", "emission", "volume"; anything else magenta), rows bottom
+    // first.  Both run the same device pass here.
+    std::vector<float> render_debug(int tx, int ty, int tw, int th, int fw, int fh, const std::string& mode) {
+        reset_cancel();
+        if (tw < 0 || th < 0) throw std::length_error("negative tile size");
+        std::vector<float> px((size_t)tw * th * 4, 0.0f);
+        if (!scene_loaded_ || !camera_set_ || tw == 0 || th == 0) return px;
+        const int32_t m = mode == "normal" ? LUCID_DEBUG_NORMAL
+                          : mode == "albedo" ? LUCID_DEBUG_ALBEDO
+                          : mode == "emission" ? LUCID_DEBUG_EMISSION
+                          : mode == "volume" ? LUCID_DEBUG_VOLUME
+                                             : -1;
+        std::lock_guard<std::mutex> l(render_mu_);
+        ensure_scene();
+        lucid_render_params pr = params(tx, ty, tw, th, fw, fh, 1, 0, 1);
+        check(lucid_render_debug(ctx_, scene_, &cam_, &pr, m, px.data()));
+        return px;
+    }
+
     // -- async accumulator (pybind_renderer.hpp:689-855) --------------------
     // A session of the current scene, camera and algorithm accumulates on a
     // worker thread; poll returns its latest averaged snapshot.
@@ -1064,6 +1085,12 @@ PYBIND11_MODULE(lucidrenderer, m) {
              "Non-blocking in-place reset of the running session with the current camera (same size only)")
         .def("is_render_async_running", &Renderer::is_render_async_running,
              py::call_guard<py::gil_scoped_release>(), "True iff a background session exists")
+        .def("render_debug", &Renderer::render_debug, py::call_guard<py::gil_scoped_release>(), py::arg("tile_x"),
+             py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"), py::arg("full_h"),
+             py::arg("mode"), "Render debug visualization: 'normal', 'albedo', or 'emission'")
+        .def("render_debug_gpu", &Renderer::render_debug, py::call_guard<py::gil_scoped_release>(),
+             py::arg("tile_x"), py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"),
+             py::arg("full_h"), py::arg("mode"), "GPU debug visualization (the same device pass as render_debug)")
         .def("render_tile_gpu", &Renderer::render_tile, py::call_guard<py::gil_scoped_release>(),
              py::arg("tile_x"), py::arg("tile_y"), py::arg("tile_w"), py::arg("tile_h"), py::arg("full_w"),
              py::arg("full_h"), py::arg("samples") = 1, py::arg("sample_offset") = 0, py::arg("max_depth") = 8,

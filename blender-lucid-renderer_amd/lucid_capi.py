@@ -126,6 +126,8 @@ SIGNATURES = {
     "lucid_primary_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float),
                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "lucid_render_debug": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams), C.c_int32,
+                                     C.POINTER(C.c_float)]),
     "lucid_cancel": (None, [C.c_void_p]),
     "lucid_is_cancelled": (C.c_int32, [C.c_void_p]),
     "lucid_reset_cancel": (None, [C.c_void_p]),

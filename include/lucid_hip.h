@@ -219,10 +219,30 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
                              const lucid_render_params* params, float* out_rgba_dev, void* stream);
 int lucid_synchronize(lucid_ctx* ctx);
 
+
 /* Per-sample radiance (before exposure and clamp) of every pixel and sample:
    out_rgb[((py*tile_w + px)*samples + s)*3 + c], tile order (top row first). */
 int lucid_render_samples(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
                          const lucid_render_params* params, float* out_rgb);
+
+/* Debug render of a tile (PyRenderer::render_debug,
+   include/pybind_renderer.hpp:371-439, and render_debug_gpu :449-520): one
+   unjittered camera ray per pixel, its closest hit by intersectScene from
+   t = 0, shown per `mode` as
+     LUCID_DEBUG_NORMAL   shading normal * 0.5 + 0.5 (traceNormal,
+                          include/renderer.hpp:58-65; a miss gives 0.5),
+     LUCID_DEBUG_ALBEDO   the material's albedo at the hit (traceAlbedo :67-78),
+     LUCID_DEBUG_EMISSION its emission (traceEmission :80-93),
+     LUCID_DEBUG_VOLUME   a medium's thickness map (traceVolume :109-190);
+   any other mode paints every pixel magenta.  tile_w * tile_h * 4 floats,
+   RGBA, bottom row first, alpha 1.  params' samples, offset, depth and
+   algorithm are ignored. */
+#define LUCID_DEBUG_NORMAL 0
+#define LUCID_DEBUG_ALBEDO 1
+#define LUCID_DEBUG_EMISSION 2
+#define LUCID_DEBUG_VOLUME 3
+int lucid_render_debug(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                       const lucid_render_params* params, int32_t mode, float* out_rgba);
 
 /* Closest hit of every (jittered) camera ray of a tile: mesh index, original
    triangle index, raw barycentric u, v and t (t = 0, ids = -1 on a miss). */

@@ -2530,6 +2530,78 @@ int orc_primary_hits(void* p, int tx, int ty, int tw, int th, int fw, int fh, in
     return 0;
 }
 
+// render_debug (pybind_renderer.hpp:371-439): one unjittered camera ray per
+// pixel, intersectScene from t = 0, coloured by traceNormal (renderer.hpp:
+// 58-65, then n * 0.5 + 0.5), traceAlbedo (:67-78), traceEmission (:80-93)
+// or traceVolume (:109-190); any other mode magenta.  mode: 0 normal,
+// 1 albedo, 2 emission, 3 volume.  out: tw*th RGBA, rows bottom first.
+static V3 debug_volume(const Scene& s, const Ray& ray) {
+    Hit hit = intersect_scene(s, ray, 0.0f);
+    if (!hit.hit) return v3(0, 0, 0);
+    const Volume& vol = s.meshes[hit.mesh].mat.vol;
+    if (!vol.present()) return v3(0, 0, 0);
+    const float kEntryOffset = 1e-4f;
+    V3 entry = ray_at(ray, hit.t);
+    Ray in;
+    in.o = add(entry, mul(ray.d, kEntryOffset));
+    in.d = ray.d;
+    Hit ex = intersect_scene(s, in, 0.0f);
+    if (vol.has_grid()) {
+        const float t_inside = ex.hit ? ex.t : 0.0f;
+        if (t_inside <= 0.0f) return v3(0, 0, 0);
+        const float extent = std::max({vol.wmax[0] - vol.wmin[0], vol.wmax[1] - vol.wmin[1], vol.wmax[2] - vol.wmin[2]});
+        const int max_dim = std::max({vol.dims[0], vol.dims[1], vol.dims[2]});
+        const float voxel = extent / (float)std::max(1, max_dim);
+        const float step = fmax_std(1e-4f, voxel * 0.5f);
+        const int n_steps = std::min(2048, (int)(t_inside / step) + 1);
+        float integrated = 0.0f, max_sampled = 0.0f;
+        for (int i = 0; i < n_steps; ++i) {
+            const float t = ((float)i + 0.5f) * step;
+            if (t > t_inside) break;
+            const float d = grid_trilinear(vol, in.o.x + in.d.x * t, in.o.y + in.d.y * t, in.o.z + in.d.z * t);
+            integrated += d * step;
+            if (d > max_sampled) max_sampled = d;
+        }
+        const float opacity = 1.0f - std::exp(-vol.density * integrated * 10.0f);
+        const float vis = std::max(max_sampled, opacity);
+        return v3(vol.color.x * vis, vol.color.y * vis, vol.color.z * vis);
+    }
+    const float t = ex.hit ? 1.0f - std::exp(-vol.density * ex.t) : 1.0f - std::exp(-vol.density);
+    return v3(vol.color.x * t, vol.color.y * t, vol.color.z * t);
+}
+int orc_render_debug(void* p, int mode, int tx, int ty, int tw, int th, int fw, int fh, float* out) {
+    const Scene& s = *static_cast<Scene*>(p);
+    for (size_t i = 0; i < (size_t)tw * th * 4; ++i) out[i] = 0.0f;
+    if (!s.cam.set) return 1;
+    RayGen g = make_raygen(s, fw, fh);
+#pragma omp parallel for collapse(2) schedule(dynamic, 16)
+    for (int py = 0; py < th; ++py) {
+        for (int px = 0; px < tw; ++px) {
+            Rng rng;
+            Ray r = camera_ray(g, tx + px, ty + py, 0u, 1, rng);
+            V3 c = v3(0, 0, 0);
+            if (mode == 0) {
+                Hit h = intersect_scene(s, r, 0.0f);
+                V3 n = h.hit ? h.normal : v3(0, 0, 0);
+                c = v3(n.x * 0.5f + 0.5f, n.y * 0.5f + 0.5f, n.z * 0.5f + 0.5f);
+            } else if (mode == 1 || mode == 2) {
+                Hit h = intersect_scene(s, r, 0.0f);
+                if (h.hit) {
+                    const Material& m = s.meshes[h.mesh].mat;
+                    c = mode == 1 ? mat_params(m, h.uv[0], h.uv[1]).albedo : mat_emission(m, h.uv[0], h.uv[1]);
+                }
+            } else if (mode == 3) {
+                c = debug_volume(s, r);
+            } else {
+                c = v3(1, 0, 1);
+            }
+            float* o = out + ((size_t)(th - 1 - py) * tw + px) * 4;
+            o[0] = c.x; o[1] = c.y; o[2] = c.z; o[3] = 1.0f;
+        }
+    }
+    return 0;
+}
+
 // getMaterialParams + getEmission (path_tracer.hpp:45-72) of mesh `mi` at n
 // texture coordinates: albedo RGB, emission RGB, metallic, roughness,
 // transmission, ior per coordinate.

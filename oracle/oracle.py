@@ -157,6 +157,7 @@ def lib(fast: bool = False) -> C.CDLL:
     L.orc_render_tile.argtypes = [vp, C.c_char_p] + [ip] * 11 + [fp, dp, dp, i64p]
     L.orc_trace_pixel.argtypes = [vp, C.c_char_p] + [ip] * 8 + [fp]
     L.orc_primary_hits.argtypes = [vp] + [ip] * 8 + [i32p, i32p, i32p, fp, fp, fp]
+    L.orc_render_debug.argtypes = [vp] + [ip] * 7 + [fp]
     L.orc_intersect.argtypes = [vp, ip, fp, fp, fp, fp, i32p, i32p, i32p, fp, fp, fp]
     L.orc_mesh_bvh.argtypes = [vp, ip, C.POINTER(C.c_int), fp, i32p, i32p]
     L.orc_seed.argtypes = [C.c_uint32] * 3 + [C.POINTER(C.c_uint64)] * 2
@@ -456,6 +457,15 @@ class OracleScene:
         self.L.orc_primary_hits(self.h, tx, ty, tw, th, fw, fh, samples, sample_offset, _iptr(mesh), _iptr(tri),
                                 _iptr(pos), _fptr(u), _fptr(v), _fptr(t))
         return {"mesh": mesh, "tri": tri, "pos": pos, "u": u, "v": v, "t": t}
+
+    def render_debug(self, mode, tx, ty, tw, th, fw, fh):
+        """render_debug (pybind_renderer.hpp:371-439): tw*th RGBA floats,
+        bottom row first; mode "normal", "albedo", "emission", "volume"."""
+        code = {"normal": 0, "albedo": 1, "emission": 2, "volume": 3}.get(mode, -1)
+        out = np.zeros(tw * th * 4, np.float32)
+        if self.L.orc_render_debug(self.h, code, tx, ty, tw, th, fw, fh, _fptr(out)) != 0:
+            raise RuntimeError("camera not set")
+        return out
 
     def intersect(self, o, d, tmin, tmax):
         o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
