@@ -1902,6 +1902,21 @@ __global__ __launch_bounds__(256) void k_diag_init(uint64_t* film, const uint64_
 // ============================================================ host side ==
 using namespace lucid;
 
+// Device memory of lucid_render_tile_device_noalloc: the counters, bins,
+// film and wave buffers of that render are carved from this one file-scope
+// array, so the call allocates nothing.  Every GPU the library runs on holds
+// its own copy for the life of the process.  Host code never uses the name
+// (on the host it is a stand-in, not a device address): lucid_create looks
+// up each device's copy with hipGetSymbolAddress once, with that device
+// current, and the renders use that address.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -1946,6 +1961,7 @@ struct lucid_ctx {
     // mapped into the device's address space
     int32_t* cancel_host = nullptr;
     const int32_t* cancel_dev = nullptr;
+    void* scratch = nullptr;  // this device's `scratch` array (no-allocation renders), or null
     std::mutex mu;
     int64_t wave_paths = 0;
     // Launch shape: workgroups per CU of the persistent kernels, or 0 for
@@ -2267,10 +2283,19 @@ struct RunOut {
     float *u = nullptr, *v = nullptr, *t = nullptr;
 };
 
+// Wave buffers carved from caller-provided device memory instead of the
+// lane's allocations (the no-allocation render): W's path arrays hold
+// cap_paths paths, film holds the tile.
+struct FixedWave {
+    WaveDev W;
+    float4* film;
+    int64_t cap_paths;
+};
+
 int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* pr,
               int mode, hipStream_t st, RunOut* ro, bool* cancelled, float4* accum = nullptr, int active_lanes = 1,
               const std::atomic<int>* cancel_flag = nullptr, const int32_t* s0_dev = nullptr,
-              bool capturing = false) {
+              bool capturing = false, const FixedWave* fixed = nullptr) {
     *cancelled = false;
     if (!cancel_flag) cancel_flag = &ctx->cancel;
     const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
@@ -2279,7 +2304,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // at most 2^26.  Each depth iteration costs one launch per kernel whose
     // duration has a floor set by the slowest wavefront's traversal, so few
     // large batches beat many small ones.
-    int64_t target = ctx->wave_paths;
+    int64_t target = fixed ? fixed->cap_paths : ctx->wave_paths;
     if (target <= 0) {
         size_t free_b = 0, total_b = 0;
         target = (int64_t)1 << 26;
@@ -2300,16 +2325,22 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         target = std::min<int64_t>(target, std::max<int64_t>((int64_t)1 << 16, ((int64_t)2 << 30) / per_path));
     }
     int64_t B = std::max<int64_t>(1, std::min<int64_t>(S, target / P));
-    int rc = ensure_wave(ctx, L, P * B);
-    if (rc) return rc;
+    int rc = LUCID_OK;
     float4* film = accum;
-    if (mode != 3) {
-        rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
+    if (fixed) {
+        if (P * B > fixed->cap_paths || diag || guided) return fail(ctx, LUCID_ERR_INVALID, "fixed wave misused");
+        film = fixed->film;
+    } else {
+        rc = ensure_wave(ctx, L, P * B);
         if (rc) return rc;
-        film = ctx->film;
+        if (mode != 3) {
+            rc = ensure_buf(ctx, &ctx->film, &ctx->film_cap, P);
+            if (rc) return rc;
+            film = ctx->film;
+        }
     }
     SceneDev SD = scene->dev;
-    WaveDev W = L.W;
+    WaveDev W = fixed ? fixed->W : L.W;
     W.sh_a = W.sh_e = W.sh_t = nullptr;
     const int32_t dsub = ctx->diag_sub;
     const int32_t dsw = pr->tile_w / dsub, dsh = pr->tile_h / dsub;
@@ -2430,7 +2461,12 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // traceVolumeMIS through gridded media marches its NEE transmittance in
     // k_vol_march, between shading and the shadow rays
     const bool march = algo == LUCID_ALGO_VOLUME_MIS && SD.n_vols > 0;
-    if (march) {
+    if (march && fixed) {
+        if (!fixed->W.sh_a) return fail(ctx, LUCID_ERR_INVALID, "fixed wave without volume factors");
+        W.sh_a = fixed->W.sh_a;
+        W.sh_e = fixed->W.sh_e;
+        W.sh_t = fixed->W.sh_t;
+    } else if (march) {
         rc = ensure_vol_wave(ctx, L, P * B);
         if (rc) return rc;
         W.sh_a = static_cast<float4*>(L.vallocs[0]);
@@ -2664,6 +2700,17 @@ int lucid_create(int32_t device, lucid_ctx** out) {
     }
     __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_SEQ_CST);
     ctx->cancel_dev = static_cast<const int32_t*>(dp);
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch[device]) {  // once per device, with the device current
+            void* sp = nullptr;
+            if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess && sp)
+                g_scratch[device] = sp;
+            else
+                (void)hipGetLastError();  // no-allocation renders then report the failure
+        }
+        ctx->scratch = g_scratch[device];
+    }
     *out = ctx;
     return LUCID_OK;
 }
@@ -2842,6 +2889,89 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
                                              cancelled ? 0 : 1, ctx->lanes[0].W.cnt + 7);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(ctx->ev1, st));
+    return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
+}
+
+int lucid_render_tile_device_noalloc(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                                     const lucid_render_params* pr, float* out_rgba_dev, void* stream) {
+    int rc = validate(ctx, scene, cam, pr);
+    if (rc) return rc;
+    if (!out_rgba_dev) return fail(ctx, LUCID_ERR_INVALID, "null output");
+    if (pr->algorithm == LUCID_ALGO_GUIDED_MIS)
+        return fail(ctx, LUCID_ERR_INVALID, "no-allocation renders do not train the guiding network");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (ctx->diag_on) return fail(ctx, LUCID_ERR_INVALID, "no-allocation renders do not record diagnostics");
+    if (!ctx->scratch) return fail(ctx, LUCID_ERR_DEVICE, "no-allocation scratch unavailable on this device");
+    HIP_OK(hipSetDevice(ctx->device));
+    // layout, each piece 256-byte aligned: counters (cnt, rays), ray-bin
+    // counters, the tile's film, then the wave's path arrays for as many
+    // paths as fit (whole samples of the tile)
+    const int64_t P = (int64_t)pr->tile_w * pr->tile_h;
+    const bool vol = pr->algorithm == LUCID_ALGO_VOLUME_MIS;
+    const int64_t per_path = kWaveBytesPerPath + (vol ? 48 : 0);
+    const auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+    const int64_t head = 256 + up((int64_t)kRayBins * 4) + up(P * 16);
+    const int64_t slack = 18 * 256;  // alignment of the path arrays
+    const int64_t avail = (int64_t)kScratchBytes - head - slack;
+    const int64_t cap = avail > 0 ? avail / per_path : 0;
+    if (cap < P) {
+        const int64_t max_px = ((int64_t)kScratchBytes - 256 - up((int64_t)kRayBins * 4) - slack) / (per_path + 16);
+        return fail(ctx, LUCID_ERR_INVALID,
+                    "tile of " + std::to_string(P) + " pixels exceeds the no-allocation scratch (" +
+                        std::to_string(kScratchBytes >> 20) + " MiB: at most " + std::to_string(max_px) +
+                        " pixels for this algorithm)");
+    }
+    unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+    int64_t off = 0;
+    const auto take = [&](int64_t bytes) {
+        void* q = base + off;
+        off += up(bytes);
+        return q;
+    };
+    FixedWave fw{};
+    WaveDev& W = fw.W;
+    W.cnt = static_cast<int32_t*>(take(256));
+    W.rays = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(W.cnt) + 64);
+    W.sbins = static_cast<uint32_t*>(take((int64_t)kRayBins * 4));
+    fw.film = static_cast<float4*>(take(P * 16));
+    W.ray_o = static_cast<float4*>(take(cap * 16));
+    W.ray_d = static_cast<float4*>(take(cap * 16));
+    W.thr = static_cast<float4*>(take(cap * 16));
+    W.rad = static_cast<float4*>(take(cap * 16));
+    W.rng = static_cast<ulonglong2*>(take(cap * 16));
+    W.meta = static_cast<int4*>(take(cap * 16));
+    W.hit = static_cast<float4*>(take(cap * 16));
+    W.sh_o = static_cast<float4*>(take(cap * 16));
+    W.sh_d = static_cast<float4*>(take(cap * 16));
+    W.sh_c = static_cast<float4*>(take(cap * 16));
+    W.q[0] = static_cast<int32_t*>(take(cap * 4));
+    W.q[1] = static_cast<int32_t*>(take(cap * 4));
+    W.qs = static_cast<int32_t*>(take(cap * 4));
+    W.sq = static_cast<int32_t*>(take(cap * 4));
+    W.skey = static_cast<uint32_t*>(take(cap * 4));
+    if (vol) {
+        W.sh_a = static_cast<float4*>(take(cap * 16));
+        W.sh_e = static_cast<float4*>(take(cap * 16));
+        W.sh_t = static_cast<float4*>(take(cap * 16));
+    }
+    if (off > (int64_t)kScratchBytes) return fail(ctx, LUCID_ERR_INVALID, "no-allocation scratch layout overflow");
+    fw.cap_paths = cap;
+    // every buffer is written by a kernel before it is read (counters and
+    // film are cleared by the render's first kernels); no per-launch events
+    const int32_t prof = ctx->profiling;
+    ctx->profiling = 0;
+    ctx->cancel.store(0);
+    __atomic_store_n(ctx->cancel_host, 0, __ATOMIC_SEQ_CST);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    bool cancelled = false;
+    rc = run_waves(ctx, ctx->lanes[0], scene, cam, pr, 0, st, nullptr, &cancelled, nullptr, 1, nullptr, nullptr, false,
+                   &fw);
+    ctx->profiling = prof;
+    if (rc) return rc;
+    CameraBasis cb = camera_basis(*cam, pr->full_w, pr->full_h);
+    k_output<<<blocks(P, 256), 256, 0, st>>>(fw.film, out_rgba_dev, pr->tile_w, pr->tile_h, cb.sensitivity,
+                                             cancelled ? 0 : 1, W.cnt + 7);
+    HIP_OK(hipGetLastError());
     return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
 }
 

@@ -120,6 +120,8 @@ SIGNATURES = {
                                     C.POINTER(C.c_float)]),
     "lucid_render_tile_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                            C.c_void_p, C.c_void_p]),
+    "lucid_render_tile_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera),
+                                                   C.POINTER(RenderParams), C.c_void_p, C.c_void_p]),
     "lucid_synchronize": (C.c_int, [C.c_void_p]),
     "lucid_render_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                        C.POINTER(C.c_float)]),

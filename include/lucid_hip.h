@@ -219,6 +219,20 @@ int lucid_render_tile_device(lucid_ctx* ctx, lucid_scene* scene, const lucid_cam
                              const lucid_render_params* params, float* out_rgba_dev, void* stream);
 int lucid_synchronize(lucid_ctx* ctx);
 
+/* No counterpart in the reference.  lucid_render_tile_device for callers
+   that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc): it calls no hipMalloc,
+   hipFree or allocating library routine.  Its counters, film and wave
+   buffers live in a 256 MiB device array of the library that every GPU the
+   library runs on holds until the process ends; the scene, the camera and
+   the output are the caller's.  Same output as lucid_render_tile_device,
+   bit for bit.  Limits: tile_w * tile_h at most about 1.37M pixels (1.1M
+   for LUCID_ALGO_VOLUME_MIS), beyond which LUCID_ERR_INVALID says so; not
+   for LUCID_ALGO_GUIDED_MIS or with diagnostics enabled.  All calls on one
+   GPU share the array: calls that overlap on one GPU (other contexts,
+   other streams) race. */
+int lucid_render_tile_device_noalloc(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam,
+                                     const lucid_render_params* params, float* out_rgba_dev, void* stream);
 
 /* Per-sample radiance (before exposure and clamp) of every pixel and sample:
    out_rgb[((py*tile_w + px)*samples + s)*3 + c], tile order (top row first). */
