@@ -65,13 +65,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
-    """Times the oracle (fast build) on a strip of rows of the same frame."""
-    import oracle
+def _host_info():
+    """CPU model and core counts of the host the baseline runs on."""
+    import subprocess
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    o = oracle.OracleScene(scene, fast=True)
-    o.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"])
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except Exception:
+        info["nproc"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("Socket(s):"):
+                info["sockets"] = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return info
+
+
+def _cpu_rate(o, W, H, depth, threads, seconds_target):
+    """Mrays/s of the oracle on a strip of rows of the frame sized to about
+    `seconds_target` seconds; returns (Mrays/s, sample text)."""
     rows, samples = 8, 2
     t0 = time.perf_counter()
     o.render_tile("mis", 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
@@ -84,9 +101,41 @@ def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
     _, ex = o.render_tile("mis", 0, y0, W, rows2, W, H, samples=samples, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0
     rays = int(ex["counts"].sum())
-    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{W}x{rows2} rows {y0}..{y0 + rows2 - 1} of the {W}x{H} frame, {samples} spp, depth {depth}, "
-                      f"mis; {rays} queries in {dt:.1f} s (oracle -O3 -ffast-math, OpenMP)"}
+    return rays / dt / 1e6, (f"{W}x{rows2} rows {y0}..{y0 + rows2 - 1} of the {W}x{H} frame, {samples} spp, "
+                             f"depth {depth}, mis; {rays} queries in {dt:.1f} s")
+
+
+def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
+    """The oracle (the CPU restatement of the reference tracer) on the host:
+    the -O3 -ffast-math build (the reference module's flags) on a strip of
+    the measured frame, the IEEE parity build on a shorter strip, and the
+    34-triangle closed box (256 x 256, 16 spp, depth 8, mis) in full.
+    Threads: OMP_NUM_THREADS, which the GPU host sets to the CPU share of one
+    GPU (16); the host's own core counts and CPU model are recorded beside."""
+    import oracle
+    import workloads
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    o = oracle.OracleScene(scene, fast=True)
+    o.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"])
+    fast, sample = _cpu_rate(o, W, H, depth, threads, seconds_target)
+    oi = oracle.OracleScene(scene, fast=False)
+    oi.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"])
+    ieee, sample_i = _cpu_rate(oi, W, H, depth, threads, seconds_target / 3)
+    box = workloads.closed_box()
+    ob = oracle.OracleScene(box, fast=True)
+    bc = workloads.CLOSED_BOX_CAMERA
+    ob.set_camera(bc["pos"], bc["dir"], bc["up"], bc["fov"])
+    t0 = time.perf_counter()
+    _, ex = ob.render_tile("mis", 0, 0, 256, 256, 256, 256, samples=16, max_depth=8, threads=threads)
+    t_box = time.perf_counter() - t0
+    box_rays = int(ex["counts"].sum())
+    return {"value": round(fast, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": sample + " (oracle -O3 -ffast-math, OpenMP)",
+            "ieee_build": {"value": round(ieee, 3), "unit": "Mrays/s", "sample": sample_i + " (oracle -O2, IEEE)"},
+            "closed_box_full": {"seconds": round(t_box, 3), "mrays_s": round(box_rays / t_box / 1e6, 3),
+                                "queries": box_rays, "config": "34 triangles, 256x256, 16 spp, depth 8, mis, full"},
+            "host": _host_info()}
 
 
 def time_to_target(curve):

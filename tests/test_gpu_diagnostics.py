@@ -25,6 +25,7 @@ import pytest
 
 import oracle
 import workloads
+import filmcheck
 import sigma
 from conftest import make_oracle, make_renderer
 
@@ -41,11 +42,6 @@ def _render_diag(lucid, algo, W, H, spp, depth, sub=1, offset=0, scene=None):
     film = r.get_diagnostic_film_bytes()
     film = np.ascontiguousarray(film).view(oracle.PIXEL_DT)[..., 0]
     return r, img, film, r.get_diagnostic_strategy_stats()
-
-
-def _live_groups(px):
-    n = int(px["num_groups"])
-    return {int(g["path_hash"]): g for g in px["groups"][:n]}
 
 
 # every algorithm twice, so a later render runs on storage an earlier one used;
@@ -66,33 +62,8 @@ def test_film_matches_oracle(lucid, algo, textured):
     assert same.mean() >= 0.99, f"record counts agree in {same.mean():.3f} of pixels"
     st = r.get_diagnostic_stats()
     assert st.total_samples == int(film["total_samples"].sum()) and st.active_pixels > 0
-    bad = 0
-    for y, x in zip(*np.nonzero(same)):
-        a, b = _live_groups(film[y, x]), _live_groups(ref_film[y, x])
-        if a.keys() != b.keys():
-            bad += 1
-            continue
-        for k in a:
-            assert a[k]["stats"]["count"] == b[k]["stats"]["count"]
-            np.testing.assert_allclose(a[k]["stats"]["mean"], b[k]["stats"]["mean"], rtol=1e-3, atol=1e-6)
-            # group variance: float Welford M2 / (n - 1) (path_stats.hpp:63-98)
-            sc = max(float(np.abs(b[k]["stats"]["mean"]).max()) ** 2, 1e-6)
-            np.testing.assert_allclose(a[k]["stats"]["variance"], b[k]["stats"]["variance"], rtol=2e-3,
-                                       atol=1e-4 * sc)
-            assert a[k]["depth"] == b[k]["depth"] and a[k]["strategy"] == b[k]["strategy"]
-            assert a[k]["light_type"] == b[k]["light_type"]
-        tot, rtot = film[y, x]["total"], ref_film[y, x]["total"]
-        np.testing.assert_allclose(tot["mean"], rtot["mean"], rtol=1e-3, atol=1e-6)
-        sc = max(float(np.abs(rtot["mean"]).max()) ** 2, 1e-6)
-        np.testing.assert_allclose(tot["variance"], rtot["variance"], rtol=2e-3, atol=1e-4 * sc)
-    assert bad <= 0.01 * same.sum() + 1
-    # FP64 per-strategy Welford (n, mean[3], M2[3]): n exact, mean and M2 to
-    # the last-bit differences of the per-sample contributions
-    np.testing.assert_array_equal(strat[..., 0], ref_strat[..., 0])
-    np.testing.assert_allclose(strat[..., 1:4], ref_strat[..., 1:4], rtol=1e-4, atol=1e-6)
-    m2_scale = np.maximum(ref_strat[..., 1:4] ** 2 * np.maximum(ref_strat[..., :1], 1.0), 1e-9)
-    assert (np.abs(strat[..., 4:7] - ref_strat[..., 4:7]) <= 1e-3 * np.abs(ref_strat[..., 4:7]) +
-            1e-5 * m2_scale).mean() >= 0.99
+    filmcheck.compare(film, ref_film, strat, ref_strat, same)
+    np.testing.assert_array_equal(strat[..., 0], ref_strat[..., 0])  # strategy counts: every pixel
 
 
 def test_mis_diagnostic_render_equals_plain(lucid):
