@@ -265,6 +265,8 @@ def main():
                     help="also measure 16 volume sessions of the 4M-triangle smoke scene (FP16 vs FP32 grid)")
     ap.add_argument("--session-passes", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shape")
+    ap.add_argument("--shade-order", choices=("auto", "0", "1"), default="auto",
+                    help="shading order after tuning: auto (the autotuner's), 0 compaction, 1 by material class")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
     ap.add_argument("--alpha", type=float, default=0.7)
@@ -273,6 +275,10 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--record-stride", type=int, default=32)
     ap.add_argument("--guide-depth", type=int, default=0, help="guided bounces: depth < N (0: all)")
+    ap.add_argument("--crossing", action="store_true",
+                    help="time-to-relMSE only: render plain, then guided, until each crosses the target (measured)")
+    ap.add_argument("--seed", type=int, default=0, help="--crossing: sample-index and network seed set")
+    ap.add_argument("--max-seconds", type=float, default=540.0, help="--crossing: cap per tracer")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -329,8 +335,11 @@ def main():
         tune_pr = lucid_capi.params(0, 0, W, H, W, H, spp, 0, depth, algo)  # one pass of the measured size
         ctx.check(L.lucid_autotune(ctx.h, sc, C.byref(camd), C.byref(tune_pr), C.byref(ls)))
         shape = {"traverse_groups_per_cu": ls.traverse_groups, "refill_idle_lanes": ls.refill,
-                 "shade_groups_per_cu": ls.shade_groups, "ray_order": ls.ray_order}
+                 "shade_groups_per_cu": ls.shade_groups, "ray_order": ls.ray_order, "shade_order": ls.shade_order}
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {shape}, {ls.ms:.1f} ms per pass")
+    if args.shade_order != "auto":
+        ctx.check(L.lucid_set_shade_order(ctx.h, int(args.shade_order)))
+        shape = dict(shape or {}, shade_order=int(args.shade_order))
 
     # ---- timed plain passes: Mrays/s ----
     for i in range(args.warmup):
@@ -412,10 +421,12 @@ def main():
         idx = torch.tensor(((H - 1 - ys) * W + xs).astype(np.int64), device="cuda")  # rows are bottom-first
         Rt = torch.tensor(R, device="cuda", dtype=torch.float64)
 
-        def progressive(algorithm, seed_base):
+        def progressive(algorithm, seed_base, stop_at_target=False, budget=None):
             acc = torch.zeros(len(xs), 3, dtype=torch.float64, device="cuda")
             t_run, n, curve, i = 0.0, 0, [], 0
-            while t_run < args.budget:
+            budget = args.budget if budget is None else budget
+            t_log = 0.0
+            while t_run < budget:
                 torch.cuda.synchronize()
                 ts = time.perf_counter()
                 render(algorithm, spp, seed_base + i * spp)
@@ -427,8 +438,40 @@ def main():
                 x = acc / n
                 e = torch.mean((x - Rt) ** 2 / (Rt * Rt + 0.01)).item() - ref_err
                 curve.append((t_run, e))
+                if t_run - t_log >= 15.0:
+                    log(f"{algorithm}: {t_run:.0f} s, {n} spp, relMSE {e:.4f}")
+                    t_log = t_run
+                if stop_at_target and e <= TARGET:
+                    break
             log(f"{algorithm}: {n} spp in {t_run:.1f} s, relMSE {curve[-1][1]:.4f}")
             return curve, n
+
+        if args.crossing:
+            # both crossings measured: each tracer renders until its running
+            # estimate reaches the target (checkpoints every pass)
+            base = args.seed * 10_000_000
+            plain_curve, plain_spp = progressive("mis", base, True, args.max_seconds)
+            gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride,
+                                          7 + args.seed, args.guide_depth)
+            ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
+            ctx.check(L.lucid_guiding_reset(ctx.h))
+            guided_curve, guided_spp = progressive("guided", base, True, args.max_seconds)
+            tp, mp = time_to_target(plain_curve)
+            tg, mg = time_to_target(guided_curve)
+            print(json.dumps({
+                "metric": METRIC, "mode": "crossing", "workload": result["config"]["workload"], "seed": args.seed,
+                "sample_offset": base, "network_seed": 7 + args.seed, "target": TARGET, "pixels": int(len(xs)),
+                "pass_spp": spp, "plain_s": round(tp, 2), "plain_method": mp, "plain_spp": plain_spp,
+                "guided_s": round(tg, 2), "guided_method": mg, "guided_spp": guided_spp,
+                "speedup": round(tp / tg, 3), "mrays_s_plain_passes": result["value"],
+                "reference": {"spp": rspp, "self_relmse": round(ref_err, 5)},
+                "guiding": {"alpha": args.alpha, "train_steps_per_pass": args.train_steps, "batch": args.batch,
+                            "lr": args.lr, "record_stride": args.record_stride},
+                "plain_curve": [(round(t, 2), round(e, 5)) for t, e in plain_curve[::8]],
+                "guided_curve": [(round(t, 2), round(e, 5)) for t, e in guided_curve[::8]]}), flush=True)
+            L.lucid_scene_release(sc)
+            ctx.close()
+            return
 
         plain_curve, plain_spp = progressive("mis", 0)
         gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7,

@@ -1138,9 +1138,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                             float pl = ls.pdf * sp;
                             v3f cc;
                             if (A == LUCID_ALGO_MIS) {
+                                // (a guided bounce keeps the plain MIS weights: see the BSDF sampling below)
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
-                                // guided bounce: the BSDF strategy is the one-sample mixture
-                                if (GUIDED && guided) pb = G.alpha * guide::mixture_pdf(gmx, ls.dir) + (1.0f - G.alpha) * pb;
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
@@ -1180,8 +1179,19 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             v3f smp_n = (mat.transmission > 0.5f) ? nrm : sn;
             const v3f thr_in = thr;
             BSample bs;
+            // the density the throughput divides by (bs.pdf, the plain
+            // tracers' value, stays the MIS pdf)
+            float pw = 0.0f;
             if (GUIDED && guided) {
-                // one-sample MIS of the guide and the BSDF: pdf = a p_guide + (1 - a) p_bsdf
+                // One-sample MIS of the guide and the BSDF sampler.  The plain
+                // bounce draws wi with the sampler's true density pt and weighs
+                // it by f cos / pb (pb = pdf_bsdf, the reference's pdf, which
+                // differs from pt at low roughness: pdf_bsdf_sampled), so it
+                // estimates the integral of f (pt / pb) cos L.  The guided bounce
+                // draws wi with q = a p_guide + (1 - a) pt and weighs it by
+                // f (pt / pb) cos / q: the same integral, the same MIS weights
+                // (pb at the next emitter hit, pb in NEE), so guided and plain
+                // converge to the same image.
                 if (randf(rng) < G.alpha) {
                     const float g0 = randf(rng), g1 = randf(rng), g2 = randf(rng);
                     bs.wi = guide::mixture_sample(gmx, g0, g1, g2);
@@ -1193,17 +1203,20 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                     const float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
                     bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
                 }
-                bs.pdf = G.alpha * guide::mixture_pdf(gmx, bs.wi) + (1.0f - G.alpha) * bs.pdf;
+                const float pt = pdf_bsdf_sampled(mat, wo, bs.wi, smp_n);
+                const float q = G.alpha * guide::mixture_pdf(gmx, bs.wi) + (1.0f - G.alpha) * pt;
+                pw = (pt > 0.0f && q > 0.0f) ? q * (bs.pdf / pt) : 0.0f;
             } else {
                 const float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
                 bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
+                pw = bs.pdf;
             }
             bool ok = bs.kind == 1 || bs.pdf > MIN_PDF;
             v3f w = bs.fw;
             if (ok && bs.kind == 0) {
                 float cs = fabsf(dot(smp_n, bs.wi));
-                if (cs <= 1e-6f || bs.pdf < MIN_PDF) ok = false;
-                else w = bsdf_weight(bs.fw, cs, bs.pdf);
+                if (cs <= 1e-6f || bs.pdf < MIN_PDF || pw < MIN_PDF) ok = false;
+                else w = bsdf_weight(bs.fw, cs, pw);
             }
             // training slot of this bounce: where, how much arrived, which way it left
             if (GUIDED && trec >= 0 && depth < kTrainSlots) {
@@ -1628,6 +1641,106 @@ void bin_rays(const WaveDev& W, const BinSrc& src, const BinBox& box, int64_t n,
     k_bin_scatter<<<gb, kBinBlock, 0, st>>>(W, src);
 }
 
+// ---- shading order: a bounce's live queue regrouped by the shading class
+// of each path's hit (miss, emitter, dielectric, metal, partly transmissive,
+// glass, node-tree material), so that the wavefronts of the shading kernel
+// run paths that take the same branches.  A stable partition by wavefront
+// ballots and prefix sums: each workgroup owns a contiguous chunk of the
+// queue; per round of 256 entries every wavefront counts its lanes of each
+// class with one ballot per class, the counts are prefix-summed over the
+// workgroup's wavefronts, and one atomic per class reserves the round's
+// range in that class's segment.  The order within a class follows the
+// queue, so the shading kernel's per-path reads stay as coalesced as they
+// were.  Results do not depend on the order.
+constexpr int kClassBlock = 256;
+__device__ __forceinline__ int shade_class_of(const SceneDev& S, float4 hv) {
+    const int32_t prim = __float_as_int(hv.y);
+    return prim < 0 ? (int)kClassMiss : S.mats[prim_mesh(S, prim)].shade_class;
+}
+__device__ __forceinline__ int class_chunk(int n) {
+    return ((n + (int)gridDim.x - 1) / (int)gridDim.x + kClassBlock - 1) / kClassBlock * kClassBlock;
+}
+// per-class totals of the queue (added into totals[0, kShadeClasses)); each
+// entry's class is kept in skey
+__global__ __launch_bounds__(kClassBlock) void k_class_count(SceneDev S, WaveDev W, const int32_t* q, const int32_t* count,
+                                                             uint32_t* totals) {
+    __shared__ uint32_t h[kShadeClasses];
+    if (threadIdx.x < kShadeClasses) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int n = *count, chunk = class_chunk(n);
+    const int i0 = (int)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int base = i0; base < i1; base += kClassBlock) {  // uniform over the workgroup
+        const int i = base + (int)threadIdx.x;
+        int c = -1;
+        if (i < i1) {
+            c = shade_class_of(S, W.hit[q[i]]);
+            W.skey[i] = (uint32_t)c;
+        }
+#pragma unroll
+        for (int k = 0; k < kShadeClasses; ++k) {
+            const unsigned long long m = __ballot(c == k);
+            if (m && lane_id() == 0) atomicAdd(&h[k], (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kShadeClasses && h[threadIdx.x]) atomicAdd(&totals[threadIdx.x], h[threadIdx.x]);
+}
+// cursors[k] = start of class k's segment (exclusive prefix sum of totals)
+__global__ __launch_bounds__(64) void k_class_scan(const uint32_t* totals, uint32_t* cursors) {
+    if (threadIdx.x == 0) {
+        uint32_t run = 0u;
+        for (int k = 0; k < kShadeClasses; ++k) {
+            cursors[k] = run;
+            run += totals[k];
+        }
+    }
+}
+__global__ __launch_bounds__(kClassBlock) void k_class_scatter(WaveDev W, const int32_t* q, const int32_t* count,
+                                                               uint32_t* cursors, int32_t* out) {
+    constexpr int kWaves = kClassBlock / 64;
+    __shared__ uint32_t wc[kWaves][kShadeClasses];
+    __shared__ uint32_t rb[kShadeClasses];
+    const int n = *count, chunk = class_chunk(n);
+    const int i0 = (int)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    const int wave = (int)threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+    for (int base = i0; base < i1; base += kClassBlock) {
+        const int i = base + (int)threadIdx.x;
+        const int c = i < i1 ? (int)W.skey[i] : -1;
+        unsigned long long mine = 0ull;
+#pragma unroll
+        for (int k = 0; k < kShadeClasses; ++k) {
+            const unsigned long long m = __ballot(c == k);
+            if (lane_id() == 0) wc[wave][k] = (uint32_t)__popcll(m);
+            if (c == k) mine = m;
+        }
+        __syncthreads();
+        if (threadIdx.x < kShadeClasses) {
+            uint32_t t = 0u;
+            for (int w = 0; w < kWaves; ++w) t += wc[w][threadIdx.x];
+            rb[threadIdx.x] = t ? atomicAdd(&cursors[threadIdx.x], t) : 0u;
+        }
+        __syncthreads();
+        if (c >= 0) {
+            uint32_t off = rb[c];
+            for (int w = 0; w < wave; ++w) off += wc[w][c];
+            out[off + (uint32_t)__popcll(mine & below)] = q[i];
+        }
+        __syncthreads();  // wc / rb are rewritten by the next round
+    }
+}
+// Regroups queue q (count entries) by shading class into `out`; sbins and
+// skey are scratch (free between a traversal pass and the next binning).
+void class_order(const SceneDev& S, const WaveDev& W, const int32_t* q, const int32_t* count, int32_t* out, int64_t n,
+                 int num_cu, hipStream_t st) {
+    const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + kClassBlock - 1) / kClassBlock),
+                                                   (int64_t)num_cu * 4);
+    k_bin_clear<<<1, 256, 0, st>>>(W.sbins);
+    k_class_count<<<g, kClassBlock, 0, st>>>(S, W, q, count, W.sbins);
+    k_class_scan<<<1, 64, 0, st>>>(W.sbins, W.sbins + kShadeClasses);
+    k_class_scatter<<<g, kClassBlock, 0, st>>>(W, q, count, W.sbins + kShadeClasses, out);
+}
+
 // One int to device memory, in stream order (vector store from lane 0).
 __global__ __launch_bounds__(64) void k_set_i32(int32_t* p, int32_t v) {
     if (threadIdx.x == 0) p[threadIdx.x] = v;
@@ -1980,6 +2093,9 @@ struct lucid_ctx {
     // 1M-triangle spheres (67M) 1 % slower; batched sessions of small scenes
     // (waves of 0.1-4M) lose 9 % to the binning launches.
     int32_t ray_order = 2;
+    // 1: each bounce's live queue is regrouped by the shading class of its
+    // hits before the shading launch (class_order); 0: compaction order
+    int32_t shade_order = 0;
     int32_t session_graphs = 1;  // replay plain one-wave session passes from captured graphs
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     float4* film = nullptr;
@@ -2413,7 +2529,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
     // the launch shape of this render, read once (callers hold ctx->mu)
     const int32_t traverse_groups = ctx->traverse_groups, shade_groups = ctx->shade_groups;
-    const int32_t refill_lanes = ctx->refill, ray_order = ctx->ray_order;
+    const int32_t refill_lanes = ctx->refill, ray_order = ctx->ray_order, shade_order = ctx->shade_order;
     // traverse_groups > 0: persistent traversal with ray refill on that many
     // workgroups per CU; 0: one lane per queued ray
     const bool refill = traverse_groups > 0;
@@ -2512,45 +2628,53 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                 k_intersect<<<g, kBlock, lds, st>>>(SD, Wd, q, sd);
             kev_end(ctx, ks, st);
             if (mode == 2) break;
+            // the shading pass reads the live queue through Wsh
+            WaveDev Wsh = W;
+            if (shade_order == 1) {
+                const int kb = kev_begin(ctx, 5, st);
+                class_order(SD, W, W.q[q], W.cnt + q, W.sq, n, ctx->num_cu, st);
+                kev_end(ctx, kb, st);
+                Wsh.q[q] = W.sq;
+            }
             const bool guide_here = guided && G.alpha > 0.0f && (ctx->gcfg.max_depth <= 0 || depth < ctx->gcfg.max_depth);
             if (guide_here) {
                 ks = kev_begin(ctx, 3, st);
                 const unsigned gg =
                     (unsigned)std::min<int64_t>(blocks(n, kInferWaves * 64), (int64_t)ctx->num_cu * 2);
-                k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(SD, W, G, q);
+                k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(SD, Wsh, G, q);
                 kev_end(ctx, ks, st);
             }
             ks = kev_begin(ctx, 1, st);
             switch (algo) {
                 case LUCID_ALGO_SIMPLE:
-                    launch_shade<LUCID_ALGO_SIMPLE>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_SIMPLE>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case LUCID_ALGO_MIS:
-                    launch_shade<LUCID_ALGO_MIS>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_MIS>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_SIMPLE:
-                    launch_shade<LUCID_ALGO_VOLUME_SIMPLE>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_VOLUME_SIMPLE>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case LUCID_ALGO_VOLUME_MIS:
-                    launch_shade<LUCID_ALGO_VOLUME_MIS>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_VOLUME_MIS>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case LUCID_ALGO_GUIDED_MIS: {
                     guide::GuideDev Gd = G;
                     if (!guide_here) Gd.alpha = 0.0f;  // this bounce samples the BSDF only
-                    launch_shade<LUCID_ALGO_GUIDED_MIS>(gs, st, SD, W, Gd, q, max_depth);
+                    launch_shade<LUCID_ALGO_GUIDED_MIS>(gs, st, SD, Wsh, Gd, q, max_depth);
                     break;
                 }
                 case kAlgoDiagSimple:
-                    launch_shade<kAlgoDiagSimple>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagSimple>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case kAlgoDiagNee:
-                    launch_shade<kAlgoDiagNee>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagNee>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case kAlgoDiagMis:
-                    launch_shade<kAlgoDiagMis>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<kAlgoDiagMis>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 default:
-                    launch_shade<LUCID_ALGO_NEE>(gs, st, SD, W, G, q, max_depth);
+                    launch_shade<LUCID_ALGO_NEE>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
             }
             if (march) {
@@ -3229,11 +3353,13 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     ctx->cancel.store(0);
     const int32_t prof = ctx->profiling;
     ctx->profiling = 0;
+    int32_t so = 0;  // shading order, searched after the ray order
     auto measure = [&](int32_t ro, int32_t tg, int32_t rf, int32_t sg, double* ms) {
         ctx->ray_order = ro;
         ctx->traverse_groups = tg;
         ctx->refill = rf;
         ctx->shade_groups = sg;
+        ctx->shade_order = so;
         double a = 0.0, b = 0.0;
         int r = timed_render(ctx, scene, cam, &tp, &a);  // first run: allocation, caches
         if (!r) r = timed_render(ctx, scene, cam, &tp, &b);
@@ -3247,6 +3373,12 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     {
         double t = 0.0;
         if (!rc && !(rc = measure(1, tg, rf, sg, &t)) && t < t_best) { t_best = t; ro = 1; }
+    }
+    {
+        double t = 0.0;
+        so = 1;
+        if (!rc && !(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) t_best = t;
+        else so = 0;
     }
     const int32_t tgs[] = {8, 12, 16, 24}, rfs[] = {32, 48, 60}, sgs[] = {4, 8, 16};
     for (int32_t v : tgs) {
@@ -3265,6 +3397,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     ctx->traverse_groups = tg;
     ctx->refill = rf;
     ctx->shade_groups = sg;
+    ctx->shade_order = so;
     ctx->profiling = prof;
     if (rc) return rc;
     rc = finish_stats(ctx);
@@ -3272,6 +3405,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     best->refill = rf;
     best->shade_groups = sg;
     best->ray_order = ro;
+    best->shade_order = so;
     best->ms = t_best;
     return rc;
 }
@@ -3294,6 +3428,13 @@ int lucid_set_ray_order(lucid_ctx* ctx, int32_t order) {
     if (!ctx || order < 0 || order > 2) return LUCID_ERR_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->ray_order = order;
+    return LUCID_OK;
+}
+
+int lucid_set_shade_order(lucid_ctx* ctx, int32_t order) {
+    if (!ctx || order < 0 || order > 1) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->shade_order = order;
     return LUCID_OK;
 }
 
@@ -3610,7 +3751,7 @@ struct PassKey {
     lucid_camera cam;
     const float4* accum;
     int32_t w, h, max_depth, algorithm, spp, active_lanes;
-    int32_t traverse_groups, refill, shade_groups, ray_order, num_cu;
+    int32_t traverse_groups, refill, shade_groups, ray_order, shade_order, num_cu;
 };
 std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp, int active_lanes) {
     PassKey k;
@@ -3621,7 +3762,7 @@ std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp
     k.w = s->w; k.h = s->h; k.max_depth = s->max_depth; k.algorithm = s->algorithm; k.spp = spp;
     k.active_lanes = active_lanes;
     k.traverse_groups = ctx->traverse_groups; k.refill = ctx->refill; k.shade_groups = ctx->shade_groups;
-    k.ray_order = ctx->ray_order; k.num_cu = ctx->num_cu;
+    k.ray_order = ctx->ray_order; k.shade_order = ctx->shade_order; k.num_cu = ctx->num_cu;
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
     return std::vector<uint8_t>(b, b + sizeof(k));
 }

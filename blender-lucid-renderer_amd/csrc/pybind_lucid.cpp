@@ -614,6 +614,7 @@ public:
         d["refill"] = b.refill;
         d["shade_groups"] = b.shade_groups;
         d["ray_order"] = b.ray_order;
+        d["shade_order"] = b.shade_order;
         d["ms"] = b.ms;
         return d;
     }

@@ -371,6 +371,12 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
         put3(g.emission, get3(em00));
         if (luminance(em00) > 1e-6f) g.flags |= MF_EMISSIVE;
         if (g.flags & MF_NODES) ps.any_nodes = true;
+        g.shade_class = (g.flags & MF_NODES) ? kClassNodes
+                        : (g.flags & MF_EMISSIVE) ? kClassEmitter
+                        : g.transmission > 0.5f ? kClassGlass
+                        : g.transmission > 0.0f ? kClassMixed
+                        : g.metallic > 0.5f ? kClassMetal
+                                              : kClassDielectric;
         if (d.has_volume) {
             GVolume v;
             std::memset(&v, 0, sizeof(v));

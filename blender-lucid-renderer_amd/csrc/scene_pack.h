@@ -46,6 +46,14 @@ enum MatFlags : int32_t {
     MF_EMISSIVE = 1, MF_VOLUME_BOUNDARY = 2, MF_VOLUME = 4, MF_NODES = 8, MF_EM_UV = 16
 };
 
+// Shading classes: hits of one class take the same branches of a shading
+// bounce (class 0 is a miss).  The key of the regrouped shading order
+// (lucid_set_shade_order).
+enum ShadeClass : int32_t {
+    kClassMiss = 0, kClassEmitter = 1, kClassDielectric = 2, kClassMetal = 3, kClassMixed = 4, kClassGlass = 5,
+    kClassNodes = 6, kShadeClasses = 8
+};
+
 struct alignas(16) GMat {
     float albedo[3];
     float metallic;
@@ -57,7 +65,8 @@ struct alignas(16) GMat {
     int32_t volume;       // index into volumes, -1 if none
     int32_t prog_albedo;  // first GNodeOp of the albedo program, -1: albedo above
     int32_t prog_emission;// first GNodeOp of the emission program, -1: emission above
-    int32_t pad[2];
+    int32_t shade_class;  // ShadeClass of the material
+    int32_t pad;
 };
 static_assert(sizeof(GMat) == 64, "GMat is 64 bytes");
 

@@ -152,6 +152,29 @@ __device__ __forceinline__ float pdf_bsdf(const Mat& m, v3f wo, v3f wi, v3f n) {
     v3f h = make_dir_or(wo + wi, mk(0.0f, 0.0f, 1.0f));
     return (sp * pdf_vndf(wo, h, r, n) + (1.0f - sp) * pdf_cosine(nl)) * (1.0f - m.transmission);
 }
+// The density with which sample_bsdf's opaque lobes actually draw wi.  Its
+// VNDF branch (Heitz 2018) samples the exact GGX distribution of visible
+// normals, while ggx_d above carries the reference's 1e-6 guard in the
+// denominator (ggx.hpp:87-99), which flattens the peak of D below roughness
+// ~0.2 (at 0.1 by two orders of magnitude).  So pdf_bsdf is the sampler's
+// density only where the guard is negligible.  The plain tracers weigh a
+// BSDF sample by eval / pdf_bsdf exactly as the reference does; the guided
+// bounce needs the true density to estimate the same thing (shade_one).
+__device__ __forceinline__ float pdf_bsdf_sampled(const Mat& m, v3f wo, v3f wi, v3f n) {
+    float nl = dot(n, wi);
+    if (nl <= 0.0f) return 0.0f;
+    float sp = spec_prob(m);
+    float r = smax(m.roughness, MIN_ROUGH);
+    v3f h = make_dir_or(wo + wi, mk(0.0f, 0.0f, 1.0f));
+    float nh = dot(n, h), vh = dot(wo, h), nv = dot(n, wo);
+    float pv = 0.0f;
+    if (nh > 0.0f && vh > 0.0f && nv > 0.0f) {
+        float al = r * r, a2 = al * al;
+        float den = nh * nh * (a2 - 1.0f) + 1.0f;
+        pv = a2 * INV_PI_F / (den * den) * ggx_g1(nv, r) / (4.0f * nv);
+    }
+    return (sp * pv + (1.0f - sp) * pdf_cosine(nl)) * (1.0f - m.transmission);
+}
 // bsdf_sample_weight, render_units.hpp:1413-1420
 __device__ __forceinline__ v3f bsdf_weight(v3f f, float c, float pdf) {
     if (pdf < MIN_PDF) return mk(0.0f, 0.0f, 0.0f);

@@ -104,7 +104,7 @@ class GuidingStats(C.Structure):
 
 class LaunchShape(C.Structure):
     _fields_ = [("traverse_groups", C.c_int32), ("refill", C.c_int32), ("shade_groups", C.c_int32),
-                ("ray_order", C.c_int32), ("ms", C.c_double)]
+                ("ray_order", C.c_int32), ("shade_order", C.c_int32), ("ms", C.c_double)]
 
 
 SIGNATURES = {
@@ -138,6 +138,7 @@ SIGNATURES = {
     "lucid_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lucid_set_refill_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_ray_order": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_set_shade_order": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_session_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                  C.POINTER(LaunchShape)]),

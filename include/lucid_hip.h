@@ -311,10 +311,20 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
 typedef struct lucid_launch_shape {
     int32_t traverse_groups, refill, shade_groups;
     int32_t ray_order;  /* lucid_set_ray_order: 0 or 1 */
+    int32_t shade_order;  /* lucid_set_shade_order: 0 or 1 */
     double ms;          /* device time of the chosen shape's render */
 } lucid_launch_shape;
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);
+
+/* Shading order (no counterpart in the reference; the regrouping by
+   material the reference's megakernel cannot do): 1 regroups each bounce's
+   live queue by the shading class of its hits (miss, emitter, dielectric,
+   metal, partly transmissive, glass, node-tree material) with wavefront
+   ballots and prefix sums before the shading launch, so that a wavefront's
+   paths take the same branches; 0 (default) shades in compaction order.
+   Results do not depend on it.  The autotuner measures both. */
+int lucid_set_shade_order(lucid_ctx* ctx, int32_t order);
 
 /* Session passes as graphs (no counterpart in the reference): 1 (default)
    launches a plain (not guided) one-wave session pass as one captured HIP

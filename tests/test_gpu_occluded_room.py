@@ -39,44 +39,69 @@ pytestmark = pytest.mark.gpu
 FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "occluded_room_reference.npz")
 
 
-@pytest.mark.timeout(600)
-def test_plain_tracer_within_3sigma_of_cpu_reference(lucid):
+def _reference():
     d = np.load(FIXTURE)
-    W, H, depth = int(d["width"]), int(d["height"]), int(d["max_depth"])
     n_ref = int(d["spp"])
     m_ref = d["sum"] / n_ref
     v_ref = np.maximum(d["sumsq"] / n_ref - m_ref * m_ref, 0.0) * n_ref / (n_ref - 1)
+    return d, n_ref, m_ref, v_ref
+
+
+def _render_subset(r, d, n_gpu, seed0=0, chunk=64):
+    W, H, depth = int(d["width"]), int(d["height"]), int(d["max_depth"])
     xs, ys = d["xs"], d["ys"]
-    wl = workloads.WORKLOADS["occluded_room"]
-    r = make_renderer(lucid, wl["make"](), wl["camera"], "mis")
-    n_gpu, chunk = 1024, 64
     acc = np.zeros((len(xs), 3))
-    for s0 in range(0, n_gpu, chunk):
+    for s0 in range(seed0, seed0 + n_gpu, chunk):
         img = r.render_tile_numpy(0, 0, W, H, W, H, chunk, s0, depth)  # rows bottom first
         acc += img[H - 1 - ys, xs, :3].astype(np.float64)
-    m_gpu = acc / n_gpu
+    return acc / n_gpu
+
+
+def _check_against_reference(m_gpu, n_gpu, what):
+    d, n_ref, m_ref, v_ref = _reference()
     se = np.sqrt(v_ref * (1.0 / n_gpu + 1.0 / n_ref))
     live = se > 0
     assert live.all(axis=1).mean() > 0.95, "most subset pixels see light"
     z = np.where(live, (m_gpu - m_ref) / np.where(live, se, 1.0), 0.0)
-    m_pix = len(xs)
-    # image-mean z: channels of a pixel are correlated, so the pixel count is the trial count
-    zp = z.mean(axis=1)
-    assert abs(zp.mean()) < 3.0 / math.sqrt(m_pix), f"image-mean z {zp.mean():+.4f}"
-    # the whole subset's sum
+    m_pix = len(d["xs"])
+    zp = z.mean(axis=1)  # channels of a pixel are correlated: the pixel count is the trial count
     var_sum = (v_ref * (1.0 / n_gpu + 1.0 / n_ref)).sum()
     z_all = (m_gpu - m_ref).sum() / math.sqrt(var_sum)
-    assert abs(z_all) < 4.0, f"subset-sum z {z_all:+.3f}"
-    # 8 regions of 2,048 pixels
     reg = np.arange(m_pix) // 2048
     zr = np.array([(m_gpu - m_ref)[reg == b].sum() / math.sqrt((se[reg == b] ** 2).sum()) for b in range(8)])
-    assert np.abs(zr).max() < 4.5, f"region z {np.round(zr, 2)}"
-    # per pixel: the lower tail
     lo, hi = float(np.mean(z[live] < -3.0)), float(np.mean(z[live] > 3.0))
-    p1 = 0.00135
-    assert lo <= p1 + 4.0 * math.sqrt(p1 * (1.0 - p1) / m_pix), f"{lo:.4f} of channels below -3 sigma"
-    # relMSE of the 1,024-spp estimate against the FP64 oracle (stated, not a bar)
     rel = workloads.relmse(m_gpu, m_ref)
-    print(f"occluded room 1024 spp: image-mean z {zp.mean():+.4f}, subset-sum z {z_all:+.3f}, region z "
+    print(f"occluded room, {what}, {n_gpu} spp: image-mean z {zp.mean():+.4f}, subset-sum z {z_all:+.3f}, region z "
           f"{np.round(zr, 2)}, channels below -3 sigma {lo:.4f}, above +3 sigma {hi:.4f}, relMSE {rel:.4f} "
           f"(oracle's own {float(d['self_relmse']):.5f})")
+    assert abs(zp.mean()) < 3.0 / math.sqrt(m_pix), f"{what}: image-mean z {zp.mean():+.4f}"
+    assert abs(z_all) < 4.0, f"{what}: subset-sum z {z_all:+.3f}"
+    assert np.abs(zr).max() < 4.5, f"{what}: region z {np.round(zr, 2)}"
+    p1 = 0.00135
+    assert lo <= p1 + 4.0 * math.sqrt(p1 * (1.0 - p1) / m_pix), f"{what}: {lo:.4f} of channels below -3 sigma"
+
+
+@pytest.mark.timeout(600)
+def test_plain_tracer_within_3sigma_of_cpu_reference(lucid):
+    d = _reference()[0]
+    wl = workloads.WORKLOADS["occluded_room"]
+    r = make_renderer(lucid, wl["make"](), wl["camera"], "mis")
+    _check_against_reference(_render_subset(r, d, 1024), 1024, "plain mis")
+
+
+@pytest.mark.timeout(600)
+def test_guided_tracer_within_3sigma_of_cpu_reference(lucid):
+    """The guided estimator against the same reference: a network trained
+    online for 32 passes, then frozen (its weights copied into a guide that
+    no longer trains), renders 1,024 spp of new seeds."""
+    d = _reference()[0]
+    wl = workloads.WORKLOADS["occluded_room"]
+    W, H, depth = int(d["width"]), int(d["height"]), int(d["max_depth"])
+    r = make_renderer(lucid, wl["make"](), wl["camera"], "guided")
+    r.set_guiding(alpha=0.7, train_steps=16, batch=32768, learning_rate=1e-2, record_stride=32, seed=7)
+    for i in range(32):
+        r.render_tile_numpy(0, 0, W, H, W, H, 16, 5_000_000 + 16 * i, depth)
+    params = np.asarray(r.get_guiding_params(), np.float32)
+    r.set_guiding(alpha=0.7, train_steps=0, batch=32768, learning_rate=1e-2, record_stride=32, seed=7)
+    r.set_guiding_params(params)
+    _check_against_reference(_render_subset(r, d, 1024), 1024, "guided, frozen network")

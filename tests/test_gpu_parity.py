@@ -141,8 +141,9 @@ def test_numpy_and_stats(lucid):
 @pytest.mark.parametrize("case", ["box_mis", "box_nee", "spheres_mis", "smoke_volume_mis"])
 def test_traversal_launch_shapes_agree(case, tmp_path):
     """The persistent refill traversal (default) and one lane per ray give
-    the same per-sample radiance bit for bit, whatever the refill threshold
-    and with or without regrouping the ray queues by bin, including
+    the same per-sample radiance bit for bit, whatever the refill threshold,
+    with or without regrouping the ray queues by bin and with or without
+    regrouping the shading queue by material class, including
     volume-boundary re-queries (smoke box) and any-hit shadows."""
     import ctypes as C
 
@@ -162,10 +163,12 @@ def test_traversal_launch_shapes_agree(case, tmp_path):
     spp = 3
     prm = lucid_capi.params(0, 0, W, H, W, H, spp, 7, 8, algo)
     outs = []
-    for shape, refill, order in ((0, 48, 0), (16, 48, 0), (16, 1, 1), (3, 64, 1), (0, 48, 1), (16, 48, 2)):
+    for shape, refill, order, shade in ((0, 48, 0, 0), (16, 48, 0, 0), (16, 1, 1, 1), (3, 64, 1, 0), (0, 48, 1, 1),
+                                        (16, 48, 2, 1)):
         ctx.check(ctx.L.lucid_set_launch_shape(ctx.h, shape, 16))
         ctx.check(ctx.L.lucid_set_refill_threshold(ctx.h, refill))
         ctx.check(ctx.L.lucid_set_ray_order(ctx.h, order))
+        ctx.check(ctx.L.lucid_set_shade_order(ctx.h, shade))
         out = np.zeros(W * H * spp * 3, np.float32)
         ctx.check(ctx.L.lucid_render_samples(ctx.h, sh, C.byref(camd), C.byref(prm),
                                              out.ctypes.data_as(C.POINTER(C.c_float))))
@@ -186,5 +189,6 @@ def test_autotune_picks_a_shape_and_keeps_results(lucid):
     shape = r.autotune(64, 48, 4, 8)
     assert shape["traverse_groups"] in (8, 12, 16, 24) and shape["refill"] in (32, 48, 60)
     assert shape["shade_groups"] in (4, 8, 16) and shape["ray_order"] in (0, 1) and shape["ms"] > 0
+    assert shape["shade_order"] in (0, 1)
     after = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
     assert np.array_equal(before, after)
