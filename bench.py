@@ -269,7 +269,7 @@ def main():
                     help="shading order after tuning: auto (the autotuner's), 0 compaction, 1 by material class")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates profiler byte counters)")
-    ap.add_argument("--alpha", type=float, default=0.7)
+    ap.add_argument("--alpha", type=float, default=0.5)
     ap.add_argument("--train-steps", type=int, default=16)
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=1e-2)
@@ -279,6 +279,8 @@ def main():
                     help="time-to-relMSE only: render plain, then guided, until each crosses the target (measured)")
     ap.add_argument("--seed", type=int, default=0, help="--crossing: sample-index and network seed set")
     ap.add_argument("--max-seconds", type=float, default=540.0, help="--crossing: cap per tracer")
+    ap.add_argument("--guided-only", action="store_true",
+                    help="time-to-relMSE leg: the guided tracer only (parameter sweeps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -473,6 +475,19 @@ def main():
             ctx.close()
             return
 
+        if args.guided_only:
+            gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride,
+                                          7 + args.seed, args.guide_depth)
+            ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
+            guided_curve, guided_spp = progressive("guided", args.seed * 10_000_000)
+            print(json.dumps({"mode": "guided-only", "budget_s": args.budget, "seed": args.seed, "alpha": args.alpha,
+                              "train_steps": args.train_steps, "batch": args.batch, "lr": args.lr,
+                              "record_stride": args.record_stride, "guided_spp": guided_spp,
+                              "relmse_end": round(guided_curve[-1][1], 5),
+                              "relmse_x_t": round(guided_curve[-1][1] * guided_curve[-1][0], 4)}), flush=True)
+            L.lucid_scene_release(sc)
+            ctx.close()
+            return
         plain_curve, plain_spp = progressive("mis", 0)
         gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7,
                                       args.guide_depth)

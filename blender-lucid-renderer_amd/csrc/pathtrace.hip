@@ -912,6 +912,22 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
         float4 hv = W.hit[p];
         v3f o = mk(ro), d = mk(rdv), thr = mk(tv);
         float last_pdf = tv.w;
+        // guided runs: the previous bounce's plain BSDF pdf and 1 - rho (see
+        // the guided BSDF sampling below); a path from raygen or a plain
+        // bounce carries rho = 1
+        const float last_pb = GUIDED ? rdv.w : 0.0f;
+        const float rho = GUIDED ? 1.0f - __int_as_float(meta.w) : 1.0f;
+        // MIS weight of an emitter hit of the previous bounce's BSDF strategy
+        // against light pdf lp (power heuristic; 1 without a pdf); guided:
+        // rho w(q) + (1 - rho) w(pb)
+        const auto hit_mis = [&](float lp) {
+            float w = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
+            if (GUIDED && rho < 1.0f) {
+                const float wp = (last_pb > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pb, lp) : 1.0f;
+                w = rho * w + (1.0f - rho) * wp;
+            }
+            return w;
+        };
         int depth = meta.z;
         int32_t prim = __float_as_int(hv.y);
         ulonglong2 rs = W.rng[p];
@@ -1014,7 +1030,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                         float sp = area_ratio(S.lights[sli].area, S.total_area);
                         if (sp > 0.0f) lp = pdf_light(S.lights[sli].area, o, lh.point, lh.normal) * sp;
                     }
-                    float mw = (last_pdf > MIN_PDF && lp > MIN_PDF) ? power_heuristic(last_pdf, lp) : 1.0f;
+                    float mw = hit_mis(lp);
                     c = c * mw;
                 }
                 L = L + c;
@@ -1100,13 +1116,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 }
             } else if (has_em) {  // traceMIS / traceVolumeMIS
                 float mw = 1.0f;
-                if (S.n_lights > 0 && last_pdf > MIN_PDF) {
+                if (S.n_lights > 0 && (last_pdf > MIN_PDF || (GUIDED && rho < 1.0f))) {
                     int32_t li = __float_as_int(S.tri[3 * (size_t)prim + 2].w);
                     if (li >= 0) {
                         float sp = area_ratio(S.lights[li].area, S.total_area);
                         if (sp > 0.0f) {
                             float lp = pdf_light(S.lights[li].area, o, point, nrm) * sp;
-                            if (lp > MIN_PDF) mw = power_heuristic(last_pdf, lp);
+                            if (lp > MIN_PDF) mw = hit_mis(lp);
                         }
                     }
                 }
@@ -1138,8 +1154,12 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                             float pl = ls.pdf * sp;
                             v3f cc;
                             if (A == LUCID_ALGO_MIS) {
-                                // (a guided bounce keeps the plain MIS weights: see the BSDF sampling below)
                                 float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                                // guided bounce: weighted against the density of its
+                                // hemisphere technique (see the BSDF sampling below)
+                                if (GUIDED && guided)
+                                    pb = G.alpha * guide::mixture_pdf(gmx, ls.dir) +
+                                         (1.0f - G.alpha) * pdf_bsdf_sampled(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
@@ -1179,19 +1199,23 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             v3f smp_n = (mat.transmission > 0.5f) ? nrm : sn;
             const v3f thr_in = thr;
             BSample bs;
-            // the density the throughput divides by (bs.pdf, the plain
-            // tracers' value, stays the MIS pdf)
-            float pw = 0.0f;
+            // the density the throughput divides by, and the MIS pdf of the
+            // next emitter hit (bs.pdf: the plain tracers' value for both)
+            float pw = 0.0f, mis_pdf = 0.0f, next_1m_rho = 0.0f;
             if (GUIDED && guided) {
-                // One-sample MIS of the guide and the BSDF sampler.  The plain
-                // bounce draws wi with the sampler's true density pt and weighs
-                // it by f cos / pb (pb = pdf_bsdf, the reference's pdf, which
-                // differs from pt at low roughness: pdf_bsdf_sampled), so it
-                // estimates the integral of f (pt / pb) cos L.  The guided bounce
-                // draws wi with q = a p_guide + (1 - a) pt and weighs it by
-                // f (pt / pb) cos / q: the same integral, the same MIS weights
-                // (pb at the next emitter hit, pb in NEE), so guided and plain
-                // converge to the same image.
+                // One-sample MIS of the guide and the BSDF sampler, unbiased
+                // with respect to the plain tracer.  The plain bounce draws wi
+                // with the sampler's true density pt but weighs it by
+                // f cos / pb (pb = pdf_bsdf, the reference's pdf, below pt at
+                // low roughness: pdf_bsdf_sampled): its BSDF strategy
+                // integrates f_eff = f pt / pb, its NEE f.  Here wi is drawn
+                // with q = a p_guide + (1 - a) pt and the throughput takes
+                // f_eff cos / q.  Direct light then splits as
+                //   NEE:  w'_l f                    w'_l = pl^2 / (pl^2 + q^2)
+                //   hit:  w'_b f + w_b (f_eff - f)  w'_b = 1 - w'_l, w_b: plain's (pb)
+                // which sums to plain's w_l f + w_b f_eff: the same image, with
+                // the guide's own density in the weights.  At the hit, in units
+                // of the throughput's f_eff: rho w'_b + (1 - rho) w_b, rho = pb / pt.
                 if (randf(rng) < G.alpha) {
                     const float g0 = randf(rng), g1 = randf(rng), g2 = randf(rng);
                     bs.wi = guide::mixture_sample(gmx, g0, g1, g2);
@@ -1206,10 +1230,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 const float pt = pdf_bsdf_sampled(mat, wo, bs.wi, smp_n);
                 const float q = G.alpha * guide::mixture_pdf(gmx, bs.wi) + (1.0f - G.alpha) * pt;
                 pw = (pt > 0.0f && q > 0.0f) ? q * (bs.pdf / pt) : 0.0f;
+                mis_pdf = q;
+                next_1m_rho = pt > 0.0f ? 1.0f - bs.pdf / pt : 0.0f;
             } else {
                 const float u1 = randf(rng), u2 = randf(rng), u3 = randf(rng);
                 bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
                 pw = bs.pdf;
+                mis_pdf = bs.pdf;
             }
             bool ok = bs.kind == 1 || bs.pdf > MIN_PDF;
             v3f w = bs.fw;
@@ -1231,7 +1258,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             }
             if (ok) {
                 thr = cmul(thr, w);
-                if (MIS) last_pdf = bs.kind == 0 ? bs.pdf : 0.0f;
+                if (MIS) last_pdf = bs.kind == 0 ? mis_pdf : 0.0f;
                 if (depth >= 3) {
                     float mx = max3(thr.x, thr.y, thr.z);
                     float rr = smin(mx, 0.95f);
@@ -1242,10 +1269,10 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             if (ok && finite3(thr)) {
                 cont = true;
                 W.ray_o[p] = make_float4(point.x, point.y, point.z, RAY_T_MIN);
-                W.ray_d[p] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, 0.0f);
+                W.ray_d[p] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, GUIDED && bs.kind == 0 ? bs.pdf : 0.0f);
                 W.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 W.rng[p] = make_ulonglong2(rng.state, rng.inc);
-                W.meta[p] = make_int4(meta.x, meta.y, depth + 1, 0);
+                W.meta[p] = make_int4(meta.x, meta.y, depth + 1, GUIDED && bs.kind == 0 ? __float_as_int(next_1m_rho) : 0);
             }
             }
         }
