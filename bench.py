@@ -374,6 +374,36 @@ def main():
         f"tris/query {p['tris_closest'] / max(q, 1):.1f}; shade {p['ms_shade']:.1f} ms, shadow {p['ms_shadow']:.1f} ms, "
         f"ray binning {p['ms_ray_binning']:.1f} ms")
 
+    # the other kernels of the pass, against the same 8 TB/s: shading reads and
+    # writes the 172-byte path state, the hit's material (64 B) and one light
+    # record (144 B) per shaded path; the shadow query reads its 32-byte ray,
+    # writes a 4-byte result and fetches N'_node nodes and N'_tri triangles
+    def _rate(nbytes, ms):
+        return nbytes / max(ms, 1e-9) / 1e6  # GB/s
+
+    bytes_shade = q * (2 * 172 + 64 + 144)
+    qs = p["rays_shadow"]
+    bytes_shadow = qs * (32 + 4) + p["nodes_shadow"] * 64 + p["tris_shadow"] * 48
+    bytes_pass = bytes_closest + bytes_shade + bytes_shadow
+    kernels = {
+        "k_shade": {"bound": "hbm", "limiter": "ALU / divergence", "bytes_per_path_bounce": 2 * 172 + 64 + 144,
+                    "path_bounces": int(q), "launches": int(p["launches_shade"]), "ms": round(p["ms_shade"], 2),
+                    "achieved": round(_rate(bytes_shade, p["ms_shade"]), 1),
+                    "frac": round(_rate(bytes_shade, p["ms_shade"]) / HBM_PEAK_GBS, 4)},
+        "k_trace_refill<1>": {"bound": "hbm", "limiter": "latency (dependent node fetches)",
+                              "queries": int(qs), "launches": int(p["launches_shadow"]),
+                              "per_query": {"nodes": round(p["nodes_shadow"] / max(qs, 1), 2),
+                                            "tris": round(p["tris_shadow"] / max(qs, 1), 2),
+                                            "ray_bytes": 32, "result_bytes": 4},
+                              "ms": round(p["ms_shadow"], 2),
+                              "achieved": round(_rate(bytes_shadow, p["ms_shadow"]), 1),
+                              "frac": round(_rate(bytes_shadow, p["ms_shadow"]) / HBM_PEAK_GBS, 4)},
+        "end_to_end": {"bytes": int(bytes_pass), "ms": round(p["ms_render"], 2),
+                       "achieved": round(_rate(bytes_pass, p["ms_render"]), 1),
+                       "frac": round(_rate(bytes_pass, p["ms_render"]) / HBM_PEAK_GBS, 4),
+                       "note": "closest hit + shade + shadow algorithmic bytes over the whole pass"},
+    }
+
     traffic = None
     if args.workload == "occluded_room" and spp == 16 and os.path.exists(TRAFFIC_PROFILE):
         for line in open(TRAFFIC_PROFILE):
@@ -402,6 +432,8 @@ def main():
                                "shadow": round(p["ms_shadow"], 1), "ray_binning": round(p["ms_ray_binning"], 1)},
                    "msamples_per_s": round(W * H * spp * args.steps / dt / 1e6, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_trace_refill<0> (closest-hit BVH traversal, persistent with ray refill)",
+                     "limiter": "latency: dependent node fetches, mostly L2 hits (traffic beyond L2 is a "
+                                "fraction of the algorithmic bytes)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "profiles/k_trace_refill_occluded_room_1080p_16spp.md (rocprofv3 PMC)"
@@ -410,7 +442,8 @@ def main():
                      "launches": n_launch,
                      "per_query": {"nodes": round(p["nodes_closest"] / max(q, 1), 2),
                                    "tris": round(p["tris_closest"] / max(q, 1), 2),
-                                   "node_bytes": 64, "tri_bytes": 48, "ray_bytes": 32, "hit_bytes": 16}},
+                                   "node_bytes": 64, "tri_bytes": 48, "ray_bytes": 32, "hit_bytes": 16},
+                     "other_kernels": kernels},
     }
 
     # ---- time-to-relMSE 0.01, guided vs plain ----

@@ -833,6 +833,36 @@ public:
         if (!film(f)) return {};
         return lucid::diag::convert_refs(lucid::diag::top_groups(f, count, false));
     }
+    // Extensions: the DiagnosticExporter queries (diagnostic_export.hpp:78-249)
+    // over the device film, which the reference reaches only through a host
+    // DiagnosticFilm.
+    std::vector<lucid::diag::GroupInfo> get_top_mean_groups(size_t count) {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return {};
+        return lucid::diag::convert_refs(lucid::diag::top_groups(f, count, true));
+    }
+    std::vector<lucid::diag::CoarseInfo> get_diagnostic_coarse_stats() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return {};
+        return lucid::diag::coarse_stats(f);
+    }
+    std::vector<lucid::diag::Suggestion> generate_diagnostic_suggestions() {
+        std::lock_guard<std::mutex> l(render_mu_);
+        lucid::diag::FilmSource f;
+        if (!film(f)) return {};
+        return lucid::diag::suggestions(f);
+    }
+    py::bytes export_diagnostic_binary() {
+        std::string b;
+        {
+            std::lock_guard<std::mutex> l(render_mu_);
+            lucid::diag::FilmSource f;
+            if (film(f)) b = lucid::diag::export_binary(f);
+        }
+        return py::bytes(b);
+    }
     PixelInfo get_pixel_diagnostic(size_t x, size_t y, const std::string& sort_by) {
         using namespace lucid::diag;
         PixelInfo info;
@@ -1135,6 +1165,14 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("export_diagnostic_json", &Renderer::export_diagnostic_json, "Export diagnostic data as JSON string")
         .def("get_top_variance_groups", &Renderer::get_top_variance_groups, py::arg("count") = 10,
              "Get top N path groups by variance")
+        .def("get_top_mean_groups", &Renderer::get_top_mean_groups, py::arg("count") = 10,
+             "Extension: top N path groups of the device film by mean contribution")
+        .def("get_diagnostic_coarse_stats", &Renderer::get_diagnostic_coarse_stats,
+             "Extension: the 8 coarse path-type statistics of the device film")
+        .def("generate_diagnostic_suggestions", &Renderer::generate_diagnostic_suggestions,
+             "Extension: improvement suggestions from the device film")
+        .def("export_diagnostic_binary", &Renderer::export_diagnostic_binary,
+             "Extension: the device film in the PVAR v1 binary format")
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
@@ -1234,7 +1272,9 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def_readonly("name", &CoarseInfo::name)
         .def_readonly("sample_count", &CoarseInfo::sample_count)
         .def_readonly("mean_luminance", &CoarseInfo::mean_luminance)
-        .def_readonly("variance_luminance", &CoarseInfo::variance_luminance);
+        .def_readonly("variance_luminance", &CoarseInfo::variance_luminance)
+        .def_readonly("mean_rgb", &CoarseInfo::mean_rgb, "Extension: the merged mean")
+        .def_readonly("variance_rgb", &CoarseInfo::variance_rgb, "Extension: the merged (max) variance");
     py::class_<HostFilm>(m, "DiagnosticFilm")
         .def(py::init<size_t, size_t, const RecordingConfig&>(), py::arg("width"), py::arg("height"),
              py::arg("config"))
@@ -1258,7 +1298,18 @@ PYBIND11_MODULE(lucidrenderer, m) {
                  });
                  return py::bytes(b);
              },
-             py::arg("x"), py::arg("y"), "Extension: PixelPathData<16,4> bytes of the film pixel holding (x, y)");
+             py::arg("x"), py::arg("y"), "Extension: PixelPathData<16,4> bytes of the film pixel holding (x, y)")
+        .def("film_bytes",
+             [](const HostFilm& f) {
+                 FilmSource src = f.source();
+                 std::string b;
+                 b.reserve(src.sampled_w * src.sampled_h * sizeof(PixelData));
+                 src.for_each([&](size_t, const PixelData& p) {
+                     b.append(reinterpret_cast<const char*>(&p), sizeof(p));
+                 });
+                 return py::bytes(b);
+             },
+             "Extension: every film pixel as PixelPathData<16,4> bytes, row-major over the sampled grid");
     struct HostTracer {
         HostFilm film;
         bool enabled = true;
