@@ -275,6 +275,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--record-stride", type=int, default=32)
     ap.add_argument("--guide-depth", type=int, default=0, help="guided bounces: depth < N (0: all)")
+    ap.add_argument("--learn-selection", type=int, default=1, choices=(0, 1),
+                    help="1: guide-sampling probability learned per surface point (starting at --alpha)")
     ap.add_argument("--crossing", action="store_true",
                     help="time-to-relMSE only: render plain, then guided, until each crosses the target (measured)")
     ap.add_argument("--seed", type=int, default=0, help="--crossing: sample-index and network seed set")
@@ -487,7 +489,7 @@ def main():
             base = args.seed * 10_000_000
             plain_curve, plain_spp = progressive("mis", base, True, args.max_seconds)
             gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride,
-                                          7 + args.seed, args.guide_depth)
+                                          7 + args.seed, args.guide_depth, args.learn_selection)
             ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
             ctx.check(L.lucid_guiding_reset(ctx.h))
             guided_curve, guided_spp = progressive("guided", base, True, args.max_seconds)
@@ -500,7 +502,8 @@ def main():
                 "guided_s": round(tg, 2), "guided_method": mg, "guided_spp": guided_spp,
                 "speedup": round(tp / tg, 3), "mrays_s_plain_passes": result["value"],
                 "reference": {"spp": rspp, "self_relmse": round(ref_err, 5)},
-                "guiding": {"alpha": args.alpha, "train_steps_per_pass": args.train_steps, "batch": args.batch,
+                "guiding": {"alpha": args.alpha, "learn_selection": args.learn_selection,
+                            "train_steps_per_pass": args.train_steps, "batch": args.batch,
                             "lr": args.lr, "record_stride": args.record_stride},
                 "plain_curve": [(round(t, 2), round(e, 5)) for t, e in plain_curve[::8]],
                 "guided_curve": [(round(t, 2), round(e, 5)) for t, e in guided_curve[::8]]}), flush=True)
@@ -510,10 +513,11 @@ def main():
 
         if args.guided_only:
             gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride,
-                                          7 + args.seed, args.guide_depth)
+                                          7 + args.seed, args.guide_depth, args.learn_selection)
             ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
             guided_curve, guided_spp = progressive("guided", args.seed * 10_000_000)
             print(json.dumps({"mode": "guided-only", "budget_s": args.budget, "seed": args.seed, "alpha": args.alpha,
+                              "learn_selection": args.learn_selection,
                               "train_steps": args.train_steps, "batch": args.batch, "lr": args.lr,
                               "record_stride": args.record_stride, "guided_spp": guided_spp,
                               "relmse_end": round(guided_curve[-1][1], 5),
@@ -523,7 +527,7 @@ def main():
             return
         plain_curve, plain_spp = progressive("mis", 0)
         gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride, 7,
-                                      args.guide_depth)
+                                      args.guide_depth, args.learn_selection)
         ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
         guided_curve, guided_spp = progressive("guided", 0)
         gst = lucid_capi.GuidingStats()
@@ -559,7 +563,8 @@ def main():
                      "infer_achieved": round(inf_tflops, 1), "infer_frac": round(inf_tflops / 2500.0, 4),
                      "train_achieved": round(trn_tflops, 1), "train_frac": round(trn_tflops / 2500.0, 4),
                      "flop_per_inference": flop_inf},
-            "guiding": {"alpha": args.alpha, "train_steps_per_pass": args.train_steps, "batch": args.batch,
+            "guiding": {"alpha": args.alpha, "learn_selection": args.learn_selection,
+                            "train_steps_per_pass": args.train_steps, "batch": args.batch,
                         "lr": args.lr, "record_stride": args.record_stride, "guided_depth": args.guide_depth,
                         "samples": int(gst.samples_total),
                         "adam_steps": int(gst.train_steps), "last_loss": round(float(gst.last_loss), 4)},

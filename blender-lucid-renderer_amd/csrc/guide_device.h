@@ -127,6 +127,15 @@ __device__ __forceinline__ bool decode(const T* o, Mixture& m) {
     return ok;
 }
 
+// Guide-selection probability at a surface point: the fixed fraction, or
+// the network's sigmoid(o[kSel]) kept inside [kSelMin, kSelMax].
+template <class T>
+__device__ __forceinline__ float selection(const T* o, float alpha, int learn_sel) {
+    if (!learn_sel) return alpha;
+    const float a = 1.0f / (1.0f + expf(-out_f32(o[kSel])));
+    return isfinite(a) ? fminf(fmaxf(a, kSelMin), kSelMax) : alpha;
+}
+
 __device__ __forceinline__ float mixture_pdf(const Mixture& m, v3f w) {
     float p = 0.0f;
     for (int k = 0; k < kLobes; ++k) p += m.w[k] * expf(m.logc[k] + m.kappa[k] * (dot(m.mu[k], w) - 1.0f));

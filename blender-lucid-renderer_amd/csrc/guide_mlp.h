@@ -20,6 +20,7 @@ struct GuideDev {
     const float* params;  // FP32 master parameters (biases are read from here)
     float bmin[3], binv[3];  // scene box: pn = (p - bmin) * binv
     float alpha;             // guide-sampling probability of the guided bounce
+    int learn_sel;           // 1: alpha per surface point from output kSel
 };
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) {

@@ -5,10 +5,13 @@
 //   input  64 features: frequency-encoded position (scene box -> [0,1]^3,
 //          8 octaves), shading normal and outgoing direction
 //   hidden 3 x 64, ReLU
-//   output 40 = 8 lobes x (mean direction xyz, log concentration, weight logit)
+//   output 40 = 8 lobes x (mean direction xyz, log concentration, weight logit),
+//          plus output 40, the logit of the guide-selection probability
+//          (learned when selection learning is on)
 // Weights are BF16, arithmetic FP32 (MFMA accumulate).  The guided bounce
 // mixes the network's density with BSDF sampling by one-sample MIS:
-// pdf = alpha * p_guide + (1 - alpha) * p_bsdf.
+// pdf = alpha * p_guide + (1 - alpha) * p_bsdf, alpha fixed or learned per
+// surface point.
 //
 // Device layout of the MFMA operands (v_mfma_f32_16x16x32_bf16, lane l,
 // group g = l / 16): the network runs transposed, out^T = W^T in^T, with
@@ -29,6 +32,12 @@ namespace lucid {
 namespace guide {
 
 constexpr int kIn = 64, kHid = 64, kOut = 40, kOutPad = 48, kLobes = 8, kLayers = 4;
+// output 40: selection logit; the render-side inference stores outputs
+// 0..43 per path (FP16, 88 bytes: whole 8-byte runs of 4)
+constexpr int kSel = 40, kOutTrained = 41, kGoutStride = 44;
+// the learned selection probability is kept inside [kSelMin, kSelMax], so
+// each technique always keeps some of the samples
+constexpr float kSelMin = 0.02f, kSelMax = 0.98f;
 constexpr int kOctaves = 8, kFeatPos = 6 * kOctaves;  // frequency features of the position
 
 // per layer: input width, output width (padded to whole 16-column tiles)
@@ -55,10 +64,11 @@ __host__ __device__
 #endif
 inline int perm(int kb, int g, int i) { return 32 * kb + (i < 4 ? 4 * g + i : 16 + 4 * g + i - 4); }
 
-// Training sample: where, how it was reached, which way it went, and the
-// weight lum((L_end - L_vertex) / T_in) = lum(W_bsdf * L_incident).
+// Training sample: where, how it was reached, which way it went, the
+// weight lum((L_end - L_vertex) / T_in) = lum(W_bsdf * L_incident), and the
+// BSDF sampler's density of wi there (pbsdf: selection learning).
 struct alignas(16) Sample {
-    float p[3], n[3], wo[3], wi[3], weight, pad[3];
+    float p[3], n[3], wo[3], wi[3], weight, pbsdf, pad[2];
 };
 static_assert(sizeof(Sample) == 64, "64-byte training samples");
 

@@ -4,7 +4,9 @@
 //   k_select      batch indices (uniform over the valid ring entries)
 //   k_wsum        sum of the batch's sample weights (loss normalisation)
 //   k_fwd         MFMA forward pass, stores BF16 activations and FP32 outputs
-//   k_loss        d loss / d outputs of  L = -sum_s w_s log p(wi_s) / sum_s w_s
+//   k_loss        d loss / d outputs of  L = -sum_s w_s log p(wi_s) / sum_s w_s,
+//                 plus, with selection learning, the selection logit's part of
+//                 L_sel = -sum_s c_s log(a p(wi_s) + (1 - a) pbsdf_s)
 //   k_bwd         MFMA backward pass through the hidden layers (BF16 deltas)
 //   k_wgrad       weight and bias gradients dW_l = X_l^T dY_l, db_l = 1^T dY_l,
 //                 split-K MFMA, FP32 atomics
@@ -34,8 +36,10 @@ __device__ __forceinline__ uint32_t hash32(uint32_t a) {
 }
 __device__ __forceinline__ float hash_unit(uint32_t a) { return (float)(hash32(a) >> 8) * 0x1p-24f; }
 
-// He-uniform weights, zero biases, zero output padding (columns 40..47).
-__global__ void k_init(float* params, float* m, float* v, uint32_t seed) {
+// He-uniform weights, zero biases, zero output padding (columns 40..47);
+// the selection column (40) starts as the bias sel_bias alone, i.e. the
+// configured guide fraction at every surface point.
+__global__ void k_init(float* params, float* m, float* v, uint32_t seed, float sel_bias) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= kParams) return;
     m[i] = 0.0f;
@@ -49,6 +53,7 @@ __global__ void k_init(float* params, float* m, float* v, uint32_t seed) {
             val = (l == kLayers - 1 && out >= kOut) ? 0.0f : (2.0f * hash_unit(seed * 0x9e3779b9U + (uint32_t)i) - 1.0f) * lim;
         }
     }
+    if (i == b_offset(kLayers - 1) + kSel) val = sel_bias;
     params[i] = val;
 }
 
@@ -143,8 +148,13 @@ __global__ __launch_bounds__(64) void k_fwd(GuideDev G, const Sample* samples, c
 //   d/d mean_k   = -c r_k kappa_k (wi - (mu_k . wi) mu_k) / |raw mean_k|
 //   d/d logk_k   = -c r_k (1 - 2 kappa_k / (exp(2 kappa_k) - 1) + kappa_k (mu_k . wi - 1))  (0 outside the clamp)
 //   d/d logit_k  = -c (r_k - w_k)
+// Selection learning: the one-sample mixture q = a p + (1 - a) pbsdf with
+// a = sigmoid(o_sel) is fitted to the same target through a alone (the
+// guide's own loss stays the one above):
+//   d/d o_sel    = -c a (1 - a) (p - pbsdf) / q
+// and, like the loss value, only the guide's part enters `loss`.
 __global__ void k_loss(const Sample* samples, const int32_t* sel, int batch, const float* out, const float* wsum,
-                       float* dout, float* loss) {
+                       float* dout, float* loss, int learn_sel) {
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= batch) return;
     const Sample s = samples[sel[b]];
@@ -173,6 +183,11 @@ __global__ void k_loss(const Sample* samples, const int32_t* sel, int batch, con
         d[5 * k + 4] = -c * (r - mx.w[k]);
     }
     for (int f = kOut; f < kOutPad; ++f) d[f] = 0.0f;
+    if (learn_sel && s.pbsdf > 0.0f) {
+        const float a = 1.0f / (1.0f + expf(-o[kSel]));
+        const float q = a * p + (1.0f - a) * s.pbsdf;
+        if (q > 0.0f && isfinite(q)) d[kSel] = -c * a * (1.0f - a) * (p - s.pbsdf) / q;
+    }
     atomicAdd(loss, -c * logf(p));
 }
 
@@ -257,7 +272,7 @@ __global__ __launch_bounds__(64) void k_wgrad(int batch, int chunks_per_wave, co
     }
     // acc[j] = product row 4g + j, column 16n + s
     const int out = 16 * n + s;
-    if (out >= (l == kLayers - 1 ? kOut : layer_out(l))) return;
+    if (out >= (l == kLayers - 1 ? kOutTrained : layer_out(l))) return;
     if (bias) {
         if (g == 0) atomicAdd(&grad[b_offset(l) + out], acc[0]);
         return;
@@ -363,14 +378,15 @@ void destroy(State* s) {
     delete s;
 }
 
-void reset(State* s, uint32_t seed, hipStream_t st) {
-    k_init<<<nblk(kParams, 256), 256, 0, st>>>(s->params, s->m, s->v, seed);
+void reset(State* s, uint32_t seed, float alpha0, hipStream_t st) {
+    const float a = fminf(fmaxf(alpha0, kSelMin), kSelMax);
+    k_init<<<nblk(kParams, 256), 256, 0, st>>>(s->params, s->m, s->v, seed, logf(a / (1.0f - a)));
     k_pack<<<nblk(kPackedFwd > kPackedBwd ? kPackedFwd : kPackedBwd, 256), 256, 0, st>>>(s->params, s->fwd, s->bwd);
     k_zero_u64<<<1, 64, 0, st>>>(s->total);
     s->adam_t = 0;
 }
 
-GuideDev dev(const State* s, const float bmin[3], const float bmax[3], float alpha) {
+GuideDev dev(const State* s, const float bmin[3], const float bmax[3], float alpha, bool learn_sel) {
     GuideDev g;
     g.fwd = reinterpret_cast<const uint4*>(s->fwd);
     g.bwd = reinterpret_cast<const uint4*>(s->bwd);
@@ -381,6 +397,7 @@ GuideDev dev(const State* s, const float bmin[3], const float bmax[3], float alp
         g.binv[c] = e > 0.0f ? 1.0f / e : 1.0f;
     }
     g.alpha = alpha;
+    g.learn_sel = learn_sel ? 1 : 0;
     return g;
 }
 
@@ -396,7 +413,8 @@ static void grad_step(State* s, const GuideDev& g, int batch, hipStream_t st) {
     k_wsum<<<nblk(batch, 256), 256, 0, st>>>(s->ring, s->sel, batch, s->scal);
     const unsigned tiles = nblk(batch, 16);
     k_fwd<<<tiles < 4096 ? tiles : 4096, 64, 0, st>>>(g, s->ring, s->sel, batch, s->act, s->out);
-    k_loss<<<nblk(batch, 256), 256, 0, st>>>(s->ring, s->sel, batch, s->out, s->scal, s->dout, s->scal + 1);
+    k_loss<<<nblk(batch, 256), 256, 0, st>>>(s->ring, s->sel, batch, s->out, s->scal, s->dout, s->scal + 1,
+                                             g.learn_sel);
     k_bwd<<<tiles < 4096 ? tiles : 4096, 64, 0, st>>>(g, batch, s->act, s->dout, s->dy);
     int wtiles = 0;
     for (int l = 0; l < kLayers; ++l) wtiles += (layer_in(l) / 16 + 1) * layer_mtiles(l);
@@ -428,7 +446,7 @@ int debug_step(State* s, const float* params, const Sample* samples, int n, cons
     std::vector<int32_t> id(n);
     for (int i = 0; i < n; ++i) id[i] = i;
     if (hipMemcpyAsync(s->sel, id.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess) return 2;
-    GuideDev g = dev(s, bmin, bmax, 0.5f);
+    GuideDev g = dev(s, bmin, bmax, 0.5f, true);
     grad_step(s, g, n, st);
     std::vector<float> o((size_t)n * kOutPad);
     if (hipMemcpyAsync(o.data(), s->out, o.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
@@ -450,7 +468,7 @@ int debug_infer(State* s, const float* params, const Sample* samples, int n, con
     if (hipMemcpyAsync(s->params, params, kParams * 4, hipMemcpyHostToDevice, st) != hipSuccess) return 2;
     k_pack<<<nblk(kPackedFwd > kPackedBwd ? kPackedFwd : kPackedBwd, 256), 256, 0, st>>>(s->params, s->fwd, s->bwd);
     if (hipMemcpyAsync(s->ring, samples, (size_t)n * sizeof(Sample), hipMemcpyHostToDevice, st) != hipSuccess) return 2;
-    GuideDev g = dev(s, bmin, bmax, 0.5f);
+    GuideDev g = dev(s, bmin, bmax, 0.5f, false);
     k_infer<<<nblk(n, 64), 64, 0, st>>>(g, s->ring, n, s->out);
     if (hipMemcpyAsync(out, s->out, (size_t)n * kOut * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return 2;
     k_zero_u64<<<1, 64, 0, st>>>(s->total);

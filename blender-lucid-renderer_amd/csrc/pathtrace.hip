@@ -1093,9 +1093,14 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             const bool guidable = GUIDED && mat.transmission == 0.0f && !(mat.roughness < 0.05f && mat.metallic > 0.5f);
             bool guided = guidable && G.alpha > 0.0f;
             guide::Mixture gmx;
+            float ga = G.alpha;  // guide-selection probability of this vertex
             // a degenerate learned density (non-finite outputs) leaves this
             // vertex to plain BSDF sampling and NEE
-            if (GUIDED && guided) guided = guide::decode(W.gout + (size_t)p * guide::kOut, gmx);
+            if (GUIDED && guided) {
+                const __half* go = W.gout + (size_t)p * guide::kGoutStride;
+                guided = guide::decode(go, gmx);
+                ga = guide::selection(go, G.alpha, G.learn_sel);
+            }
             if (SIMPLE) {
                 if (DIAG) {
                     if (has_em) {
@@ -1158,8 +1163,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                                 // guided bounce: weighted against the density of its
                                 // hemisphere technique (see the BSDF sampling below)
                                 if (GUIDED && guided)
-                                    pb = G.alpha * guide::mixture_pdf(gmx, ls.dir) +
-                                         (1.0f - G.alpha) * pdf_bsdf_sampled(mat, wo, ls.dir, sn);
+                                    pb = ga * guide::mixture_pdf(gmx, ls.dir) +
+                                         (1.0f - ga) * pdf_bsdf_sampled(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
@@ -1201,7 +1206,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             BSample bs;
             // the density the throughput divides by, and the MIS pdf of the
             // next emitter hit (bs.pdf: the plain tracers' value for both)
-            float pw = 0.0f, mis_pdf = 0.0f, next_1m_rho = 0.0f;
+            float pw = 0.0f, mis_pdf = 0.0f, next_1m_rho = 0.0f, pt_wi = -1.0f;
             if (GUIDED && guided) {
                 // One-sample MIS of the guide and the BSDF sampler, unbiased
                 // with respect to the plain tracer.  The plain bounce draws wi
@@ -1216,7 +1221,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 // which sums to plain's w_l f + w_b f_eff: the same image, with
                 // the guide's own density in the weights.  At the hit, in units
                 // of the throughput's f_eff: rho w'_b + (1 - rho) w_b, rho = pb / pt.
-                if (randf(rng) < G.alpha) {
+                if (randf(rng) < ga) {
                     const float g0 = randf(rng), g1 = randf(rng), g2 = randf(rng);
                     bs.wi = guide::mixture_sample(gmx, g0, g1, g2);
                     bs.kind = 0;
@@ -1228,7 +1233,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                     bs = sample_bsdf(mat, wo, smp_n, u1, u2, u3, rng);
                 }
                 const float pt = pdf_bsdf_sampled(mat, wo, bs.wi, smp_n);
-                const float q = G.alpha * guide::mixture_pdf(gmx, bs.wi) + (1.0f - G.alpha) * pt;
+                pt_wi = pt;
+                const float q = ga * guide::mixture_pdf(gmx, bs.wi) + (1.0f - ga) * pt;
                 pw = (pt > 0.0f && q > 0.0f) ? q * (bs.pdf / pt) : 0.0f;
                 mis_pdf = q;
                 next_1m_rho = pt > 0.0f ? 1.0f - bs.pdf / pt : 0.0f;
@@ -1248,12 +1254,15 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             // training slot of this bounce: where, how much arrived, which way it left
             if (GUIDED && trec >= 0 && depth < kTrainSlots) {
                 const bool rec = ok && guidable && (w.x > 0.0f || w.y > 0.0f || w.z > 0.0f);
+                // the sampler's density of wi (> 0 wherever a sample is kept)
+                float pts = 0.0f;
+                if (rec) pts = pt_wi >= 0.0f ? pt_wi : pdf_bsdf_sampled(mat, wo, bs.wi, smp_n);
                 if (rec) {
                     *tslot_at(W, trec, depth, 0) = make_float4(point.x, point.y, point.z, thr_in.x);
                     *tslot_at(W, trec, depth, 1) = make_float4(nrm.x, nrm.y, nrm.z, thr_in.y);
                     *tslot_at(W, trec, depth, 2) = make_float4(wo.x, wo.y, wo.z, thr_in.z);
                 }
-                *tslot_at(W, trec, depth, 3) = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, rec ? 1.0f : 0.0f);
+                *tslot_at(W, trec, depth, 3) = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, rec ? fmaxf(pts, 1e-30f) : 0.0f);
                 *tslot_at(W, trec, depth, 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
             if (ok) {
@@ -1444,10 +1453,10 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
 #pragma unroll
             for (int t = 0; t < kInferTiles; ++t) {
                 if (pt[t] < 0) continue;
-                __half* out = W.gout + (size_t)pt[t] * guide::kOut;
+                __half* out = W.gout + (size_t)pt[t] * guide::kGoutStride;
                 for (int m = 0; m < 3; ++m) {
                     const int f = 16 * m + 4 * g;  // 4 outputs per lane and tile, 8-byte aligned
-                    if (f >= guide::kOut) continue;
+                    if (f >= guide::kGoutStride) continue;
                     __half2 lo = __floats2half2_rn(o[t][m][0], o[t][m][1]);
                     __half2 hi = __floats2half2_rn(o[t][m][2], o[t][m][3]);
                     uint2 v;
@@ -1478,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_path
             d = *tslot_at(W, r, k, 3); e = *tslot_at(W, r, k, 4);
         }
         float wgt = 0.0f;
-        if (d.w == 1.0f && e.w == 1.0f) {
+        if (d.w > 0.0f && e.w == 1.0f) {
             const float tr = a.w, tg = b.w, tb = c.w;
             const float cr = tr > 0.0f ? (le.x - e.x) / tr : 0.0f;
             const float cg = tg > 0.0f ? (le.y - e.y) / tg : 0.0f;
@@ -1495,7 +1504,8 @@ __global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_path
         sm.wo[0] = c.x; sm.wo[1] = c.y; sm.wo[2] = c.z;
         sm.wi[0] = d.x; sm.wi[1] = d.y; sm.wi[2] = d.z;
         sm.weight = wgt;
-        sm.pad[0] = sm.pad[1] = sm.pad[2] = 0.0f;
+        sm.pbsdf = d.w;
+        sm.pad[0] = sm.pad[1] = 0.0f;
         ring[slot] = sm;
     }
 }
@@ -2133,7 +2143,7 @@ struct lucid_ctx {
     int32_t profiling = 0;
     // learned path guiding
     guide::State* gstate = nullptr;
-    lucid_guiding_config gcfg{0.7f, 16, 32768, 1e-2f, 32, 1u, 0};
+    lucid_guiding_config gcfg{0.5f, 16, 32768, 1e-2f, 32, 1u, 0, 0};
     int64_t gsteps = 0;
     __half* gout = nullptr;  // FP16 network outputs, 40 per path
     int64_t gout_cap = 0;
@@ -2349,7 +2359,7 @@ int ensure_guide(lucid_ctx* ctx) {
     if (ctx->gstate) return LUCID_OK;
     ctx->gstate = guide::create(kGuideRing, kGuideBatchCap);
     if (!ctx->gstate) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (guiding network)");
-    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->gcfg.alpha, ctx->stream);
     ctx->gsteps = 0;
     return LUCID_OK;
 }
@@ -2536,7 +2546,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     if (guided) {
         rc = ensure_guide(ctx);
         if (rc) return rc;
-        rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, P * B * guide::kOut);
+        rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, P * B * guide::kGoutStride);
         if (rc) return rc;
         W.gout = ctx->gout;
         if (train) {
@@ -2548,7 +2558,8 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             W.tslot = ctx->tslot;
         }
         // the guide samples only once it has been trained
-        G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gsteps > 0 ? ctx->gcfg.alpha : 0.0f);
+        G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gsteps > 0 ? ctx->gcfg.alpha : 0.0f,
+                       ctx->gcfg.learn_selection != 0);
     }
     // LDS stack sized to the scene's BVH depth; persistent grids of a few
     // workgroups per CU walk the live queues (grid-stride).
@@ -2663,7 +2674,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                 kev_end(ctx, kb, st);
                 Wsh.q[q] = W.sq;
             }
-            const bool guide_here = guided && G.alpha > 0.0f && (ctx->gcfg.max_depth <= 0 || depth < ctx->gcfg.max_depth);
+            // the last bounce samples no next direction: its NEE keeps the
+            // plain weights, as the plain tracer's last bounce does
+            const bool guide_here = guided && G.alpha > 0.0f && depth + 1 < max_depth &&
+                                    (ctx->gcfg.max_depth <= 0 || depth < ctx->gcfg.max_depth);
             if (guide_here) {
                 ks = kev_begin(ctx, 3, st);
                 const unsigned gg =
@@ -3584,7 +3598,7 @@ int lucid_set_guiding(lucid_ctx* ctx, const lucid_guiding_config* cfg) {
     ctx->gcfg = *cfg;
     int rc = ensure_guide(ctx);
     if (rc) return rc;
-    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->gcfg.alpha, ctx->stream);
     ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
@@ -3644,7 +3658,7 @@ int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* 
     rc = guide::debug_infer(ctx->gstate, params, reinterpret_cast<const guide::Sample*>(samples), n, bmin, bmax, out,
                             ctx->stream);
     if (rc) return fail(ctx, rc == 1 ? LUCID_ERR_INVALID : LUCID_ERR_DEVICE, "guiding debug inference failed");
-    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->gcfg.alpha, ctx->stream);
     ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;
@@ -3666,7 +3680,7 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
     int rc = ensure_guide(ctx);
     if (!rc) rc = ensure_wave(ctx, ctx->lanes[0], n);
-    if (!rc) rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, (int64_t)n * guide::kOut);
+    if (!rc) rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, (int64_t)n * guide::kGoutStride);
     if (rc) return rc;
     if (guide::set_params(ctx->gstate, params, ctx->stream)) return fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed");
     ctx->gsteps = std::max<int64_t>(ctx->gsteps, 1);
@@ -3694,21 +3708,24 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     if (m > 0) HIP_OK(hipMemcpyAsync(W.q[0], order, (size_t)m * 4, hipMemcpyHostToDevice, st));
     k_set_i32<<<1, 64, 0, st>>>(W.cnt, m);
     // outputs start as NaN: entries the kernel must not write stay NaN
-    HIP_OK(hipMemsetAsync(W.gout, 0xff, (size_t)n * guide::kOut * sizeof(__half), st));
-    const guide::GuideDev G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gcfg.alpha);
+    HIP_OK(hipMemsetAsync(W.gout, 0xff, (size_t)n * guide::kGoutStride * sizeof(__half), st));
+    const guide::GuideDev G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gcfg.alpha,
+                                         ctx->gcfg.learn_selection != 0);
     const unsigned gg = (unsigned)std::min<int64_t>(blocks(std::max<int32_t>(m, 1), kInferWaves * 64),
                                                     (int64_t)ctx->num_cu * 2);
     k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(scene->dev, W, G, 0);
     k_debug_frame<<<blocks(n, 256), 256, 0, st>>>(scene->dev, W, n);
     HIP_OK(hipGetLastError());
-    std::vector<__half> h((size_t)n * guide::kOut);
+    std::vector<__half> h((size_t)n * guide::kGoutStride);
     std::vector<float4> hit((size_t)n);
     HIP_OK(hipMemcpyAsync(h.data(), W.gout, h.size() * sizeof(__half), hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(hit.data(), W.hit, (size_t)n * 16, hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(o4.data(), W.sh_o, (size_t)n * 16, hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(d4.data(), W.sh_d, (size_t)n * 16, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    for (size_t i = 0; i < h.size(); ++i) out[i] = __half2float(h[i]);
+    for (int32_t i = 0; i < n; ++i)
+        for (int f = 0; f < guide::kOutTrained; ++f)
+            out[(size_t)i * guide::kOutTrained + f] = __half2float(h[(size_t)i * guide::kGoutStride + f]);
     for (int32_t i = 0; i < n; ++i) {
         int32_t prim;
         std::memcpy(&prim, &hit[i].y, 4);
@@ -3740,7 +3757,7 @@ int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* s
                            grad, loss, ctx->stream);
     if (rc) return fail(ctx, rc == 1 ? LUCID_ERR_INVALID : LUCID_ERR_DEVICE, "guiding debug step failed");
     // the network under training was overwritten: start afresh
-    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->stream);
+    guide::reset(ctx->gstate, ctx->gcfg.seed, ctx->gcfg.alpha, ctx->stream);
     ctx->gsteps = 0;
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return LUCID_OK;

@@ -724,10 +724,11 @@ public:
     }
 
     // ---- learned path guiding (extension) ----
-    void set_guiding(float alpha, int train_steps, int batch, float lr, int stride, uint32_t seed, int max_depth) {
+    void set_guiding(float alpha, int train_steps, int batch, float lr, int stride, uint32_t seed, int max_depth,
+                     bool learn_selection) {
         std::lock_guard<std::mutex> l(render_mu_);
         ensure_ctx();
-        lucid_guiding_config c{alpha, train_steps, batch, lr, stride, seed, max_depth};
+        lucid_guiding_config c{alpha, train_steps, batch, lr, stride, seed, max_depth, learn_selection ? 1 : 0};
         check(lucid_set_guiding(ctx_, &c));
     }
     void reset_guiding() {
@@ -1176,9 +1177,9 @@ PYBIND11_MODULE(lucidrenderer, m) {
         .def("get_pixel_diagnostic", &Renderer::get_pixel_diagnostic, py::arg("x"), py::arg("y"),
              py::arg("sort_by") = "variance",
              "Get diagnostic data for a specific pixel, sorted by 'variance' (default) or 'mean'")
-        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.7f, py::arg("train_steps") = 16,
+        .def("set_guiding", &Renderer::set_guiding, py::arg("alpha") = 0.5f, py::arg("train_steps") = 16,
              py::arg("batch") = 32768, py::arg("learning_rate") = 1e-2f, py::arg("record_stride") = 32,
-             py::arg("seed") = 1u, py::arg("max_depth") = 0,
+             py::arg("seed") = 1u, py::arg("max_depth") = 0, py::arg("learn_selection") = false,
              "Extension: configure learned path guiding (algorithm 'guided') and start from a fresh network")
         .def("reset_guiding", &Renderer::reset_guiding, "Extension: fresh guiding network")
         .def("guiding_stats", &Renderer::guiding_stats, "Extension: training state of the guiding network")

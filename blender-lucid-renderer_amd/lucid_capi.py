@@ -94,7 +94,7 @@ class DiagInfo(C.Structure):
 class GuidingConfig(C.Structure):
     _fields_ = [("alpha", C.c_float), ("train_steps", C.c_int32), ("batch", C.c_int32),
                 ("learning_rate", C.c_float), ("record_stride", C.c_int32), ("seed", C.c_uint32),
-                ("max_depth", C.c_int32)]
+                ("max_depth", C.c_int32), ("learn_selection", C.c_int32)]
 
 
 class GuidingStats(C.Structure):

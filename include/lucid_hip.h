@@ -414,6 +414,10 @@ typedef struct lucid_guiding_config {
                               records training samples, >= 1; default 32 */
     uint32_t seed;         /* network initialisation */
     int32_t max_depth;     /* bounces guided: depth < max_depth (0: all) */
+    int32_t learn_selection; /* 1: the guide-sampling probability is a network
+                                output per surface point, trained with the
+                                one-sample mixture's loss and starting at alpha
+                                everywhere; 0: alpha everywhere */
 } lucid_guiding_config;
 
 typedef struct lucid_guiding_stats {
@@ -430,13 +434,15 @@ int lucid_set_guiding(lucid_ctx* ctx, const lucid_guiding_config* cfg);
 int lucid_guiding_reset(lucid_ctx* ctx);
 int lucid_guiding_get_stats(lucid_ctx* ctx, lucid_guiding_stats* out);
 /* FP32 master parameters (num_params floats): per layer l, W_l [in][out]
-   row-major then b_l; layers 64x64, 64x64, 64x64, 64x48 (columns 40..47 0). */
+   row-major then b_l; layers 64x64, 64x64, 64x64, 64x48 (column 40: the
+   selection logit; columns 41..47 0). */
 int lucid_guiding_get_params(lucid_ctx* ctx, float* params, int32_t n);
 int lucid_guiding_set_params(lucid_ctx* ctx, const float* params, int32_t n);
 /* One forward / backward pass of the training kernels over exactly n
    samples (16 floats each: position, normal, outgoing, incoming direction,
-   weight, 3 unused) with `params`, positions mapped through the box
-   [bmin, bmax]: raw outputs (n x 40), gradient (num_params) and loss.  For
+   weight, the BSDF sampler's density of the incoming direction, 2 unused)
+   with `params`, positions mapped through the box [bmin, bmax], selection
+   learning on: raw outputs (n x 40), gradient (num_params) and loss.  For
    the gradient check against the CPU reference. */
 int lucid_guiding_debug_step(lucid_ctx* ctx, const float* params, const float* samples, int32_t n,
                              const float bmin[3], const float bmax[3], float* out, float* grad, float* loss);
@@ -447,8 +453,9 @@ int lucid_guiding_debug_infer(lucid_ctx* ctx, const float* params, const float* 
 /* The inference kernel of guided renders (k_guide_infer), run as a guided
    bounce runs it: n rays (6 floats each: origin, direction) are traced by
    the closest-hit kernel, then the network with `params` runs over the queue
-   `order` (m path indices, any order, each < n).  out: n x 40 raw outputs
-   (FP16 widened; NaN for paths the kernel must not write: not queued, or
+   `order` (m path indices, any order, each < n).  out: n x 41 raw outputs
+   (the 40 mixture outputs and the selection logit; FP16 widened;
+   NaN for paths the kernel must not write: not queued, or
    missed), frame: n x 6 (hit point, shading normal; NaN on a miss), box: the
    scene box the features are normalised by (min[3], max[3]).  For tests. */
 int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const float* params, int32_t n,

@@ -14,7 +14,11 @@ code:
   log concentration clamped to [-2, 7], weight logit), softmax weights;
 * density p(w) = sum_k w_k kappa_k / (2 pi (1 - e^{-2 kappa_k}))
   exp(kappa_k (mu_k . w - 1));
-* loss L = -sum_s c_s log p(wi_s), c_s = weight_s / sum weight.
+* loss L = -sum_s c_s log p(wi_s), c_s = weight_s / sum weight;
+* selection learning: output 40 is the logit of the guide-selection
+  probability a = sigmoid(o_40), fitted through a alone to the one-sample
+  mixture's loss -sum_s c_s log(a p(wi_s) + (1 - a) pbsdf_s) (p held fixed);
+  the reported loss stays L.
 
 `bf16=True` mirrors the device's precision split: inputs, hidden activations,
 weights and back-propagated deltas rounded to BF16, sums in float64 (the
@@ -27,6 +31,7 @@ from __future__ import annotations
 import numpy as np
 
 N_IN, N_HID, N_OUT, N_PAD, N_LOBES = 64, 64, 40, 48, 8
+SEL = 40  # selection logit
 LAYERS = [(64, 64), (64, 64), (64, 64), (64, 48)]
 LOGK_MIN, LOGK_MAX = -2.0, 7.0
 
@@ -134,6 +139,22 @@ def loss_grad(o, wi, weight):
     return d.reshape(-1, N_OUT)
 
 
+def sel_grad(o_sel, p, pbsdf, weight):
+    """d L_sel / d o_40 per sample; 0 where pbsdf <= 0."""
+    c = np.asarray(weight, np.float64) / np.sum(weight)
+    a = 1.0 / (1.0 + np.exp(-np.asarray(o_sel, np.float64)))
+    pb = np.asarray(pbsdf, np.float64)
+    q = a * p + (1.0 - a) * pb
+    return np.where(pb > 0, -c * a * (1.0 - a) * (p - pb) / np.where(q > 0, q, 1.0), 0.0)
+
+
+def sel_loss(o_sel, p, pbsdf, weight):
+    c = np.asarray(weight, np.float64) / np.sum(weight)
+    a = 1.0 / (1.0 + np.exp(-np.asarray(o_sel, np.float64)))
+    pb = np.asarray(pbsdf, np.float64)
+    return float(-np.sum(np.where(pb > 0, c * np.log(a * p + (1.0 - a) * pb), 0.0)))
+
+
 def forward(params, x, bf=True):
     """Activations X_0..X_3 (inputs of each layer) and the raw outputs."""
     ws, bs = unpack(params)
@@ -148,9 +169,9 @@ def forward(params, x, bf=True):
     return xs, out
 
 
-def step(params, samples, bmin, bmax, bf=True):
+def step(params, samples, bmin, bmax, bf=True, learn_sel=True):
     """Raw outputs (S, 40), gradient (N_PARAMS,) and loss of one training
-    step over `samples` (S, 16): p, n, wo, wi, weight."""
+    step over `samples` (S, 16): p, n, wo, wi, weight, pbsdf."""
     s = np.asarray(samples, np.float64)
     x = features(s[:, 0:3], s[:, 3:6], s[:, 6:9], bmin, bmax)
     xs, out = forward(params, x, bf)
@@ -158,6 +179,8 @@ def step(params, samples, bmin, bmax, bf=True):
     L = loss(o, s[:, 9:12], s[:, 12])
     d = np.zeros((s.shape[0], N_PAD))
     d[:, :N_OUT] = loss_grad(o, s[:, 9:12], s[:, 12])
+    if learn_sel:
+        d[:, SEL] = sel_grad(out[:, SEL], np.maximum(pdf(o, s[:, 9:12]), 1e-300), s[:, 13], s[:, 12])
     ws, _ = unpack(params)
     rnd = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     dy = [None] * 4
@@ -170,8 +193,8 @@ def step(params, samples, bmin, bmax, bf=True):
         gw = xs[l].T @ dy[l]
         gb = dy[l].sum(axis=0)
         if l == 3:
-            gw[:, N_OUT:] = 0.0
-            gb[N_OUT:] = 0.0
+            gw[:, SEL + 1:] = 0.0
+            gb[SEL + 1:] = 0.0
         gws.append(gw)
         gbs.append(gb)
     return o, pack(gws, gbs), L

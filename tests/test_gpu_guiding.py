@@ -6,7 +6,8 @@
   render-path inference kernel on a traced, shuffled queue.
 * Online training lowers the loss of the guide on the closed box.
 * The guided estimator is unbiased: per-pixel means agree with plain MIS
-  within 3 sigma (independent seeds).
+  within 3 sigma (independent seeds), with a fixed guide fraction and with
+  the learned per-point selection probability.
 * What the network gains: on the occluded room (light behind a 5 cm slit)
   the trained guided tracer has lower relMSE than plain MIS at equal spp.
 """
@@ -40,6 +41,8 @@ def test_training_step_matches_reference():
     smp = np.zeros((S, 16), np.float32)
     smp[:, 0:3], smp[:, 3:6], smp[:, 6:9], smp[:, 9:12] = p, dirs[0], dirs[1], dirs[2]
     smp[:, 12] = rng.uniform(0.05, 3.0, S)
+    smp[:, 13] = rng.uniform(0.02, 2.0, S)  # the sampler's density: the selection term is on
+    smp[::7, 13] = 0.0
     ctx = lucid_capi.Context(0)
     out = np.zeros((S, 40), np.float32)
     grad = np.zeros(G.N_PARAMS, np.float32)
@@ -54,6 +57,12 @@ def test_training_step_matches_reference():
         for a, b in ((wo, bo), (bo, bo + j)):
             e = np.linalg.norm(grad[a:b] - g_ref[a:b]) / max(np.linalg.norm(g_ref[a:b]), 1e-12)
             assert e < 3e-2, (a, b, e)
+    # the selection column on its own (weights into output 40 and its bias)
+    wo3, bo3 = G.OFFS[3]
+    col = np.r_[wo3 + np.arange(64) * 48 + G.SEL, bo3 + G.SEL]
+    assert np.linalg.norm(g_ref[col]) > 0
+    e = np.linalg.norm(grad[col] - g_ref[col]) / np.linalg.norm(g_ref[col])
+    assert e < 3e-2, e
 
 
 def test_render_inference_matches_reference():
@@ -83,7 +92,10 @@ def test_render_path_inference_kernel_matches_reference():
     Queued hits match the float64 BF16-mode reference on the device's own
     hit frames; every other output stays untouched (NaN)."""
     rng = np.random.default_rng(31)
-    params = G.init_params(seed=32, scale_out=1.0).astype(np.float32)
+    params = G.init_params(seed=32, scale_out=1.0)
+    ws, bs = G.unpack(params)
+    ws[3][:, G.SEL] = rng.uniform(-0.5, 0.5, 64)  # a selection logit that varies
+    params = G.pack(ws, bs).astype(np.float32)
     n, m = 300007, 287113
     o = rng.uniform([-0.9, 0.1, -0.9], [0.9, 1.9, 0.9], (n, 3))
     d = rng.normal(size=(n, 3))
@@ -96,7 +108,7 @@ def test_render_path_inference_kernel_matches_reference():
     order[0] = 0
     ctx = lucid_capi.Context(0)
     sh = ctx.scene(workloads.closed_box())
-    out = np.zeros((n, 40), np.float32)
+    out = np.zeros((n, 41), np.float32)
     frame = np.zeros((n, 6), np.float32)
     box = np.zeros(6, np.float32)
     ctx.check(ctx.L.lucid_guiding_debug_render_infer(ctx.h, sh, F(params), n, F(rays),
@@ -113,8 +125,9 @@ def test_render_path_inference_kernel_matches_reference():
     assert np.isfinite(out[live]).all()
     x = G.features(frame[live, :3], frame[live, 3:6], -rays[live, 3:6], box[:3], box[3:])
     _, o_ref = G.forward(params.astype(np.float64), x, bf=True)
-    err = np.abs(out[live] - o_ref[:, :40])
+    err = np.abs(out[live] - o_ref[:, :41])
     assert err.max() < 2e-2 * np.max(np.abs(o_ref)), err.max()
+    assert np.std(out[live, G.SEL]) > 0.05  # the selection logit is stored per path
 
 
 def _box_renderer(lucid, algo):
@@ -134,11 +147,13 @@ def test_online_training_lowers_loss(lucid):
     assert np.mean(losses[-3:]) < losses[0] - 0.1, losses
 
 
-def test_guided_is_unbiased(lucid):
+@pytest.mark.parametrize("learn_selection", [False, True])
+def test_guided_is_unbiased(lucid, learn_selection):
     W = H = 24
     n = 256
     g = _box_renderer(lucid, "guided")
-    g.set_guiding(alpha=0.5, train_steps=16, batch=8192, learning_rate=5e-3, record_stride=1, seed=5)
+    g.set_guiding(alpha=0.5, train_steps=16, batch=8192, learning_rate=5e-3, record_stride=1, seed=5,
+                  learn_selection=learn_selection)
     for i in range(6):
         g.render_tile(0, 0, W, H, W, H, 16, 100000 + 16 * i, 8)
     sg = g.debug_render_samples(0, 0, W, H, W, H, n, 0, 8).astype(np.float64)

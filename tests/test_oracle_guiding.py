@@ -7,7 +7,12 @@ mathematics rather than by restating its formulas:
 * the decoded mixture is a density: it integrates to 1 over the sphere
   (Fibonacci-lattice quadrature), for concentrations from 0.14 to 50;
 * the BF16 mode only rounds: with exactly representable inputs it agrees
-  with the float64 mode to BF16 precision.
+  with the float64 mode to BF16 precision;
+* selection learning: the gradient with the selection term equals central
+  finite differences of L + L_sel, the mixture loss taken with the guide's
+  density held at its value (the selection logit is fitted through a alone),
+  on every selection parameter and on hidden-layer parameters; its sign
+  moves a towards the technique that has the larger density at the samples.
 """
 
 import numpy as np
@@ -75,3 +80,44 @@ def test_bf16_mode_is_rounding_only():
     assert np.max(np.abs(o1 - o2)) < 0.05 * (np.max(np.abs(o2)) + 1e-3)
     assert abs(l1 - l2) < 0.02 * abs(l2)
     assert np.linalg.norm(g1 - g2) < 0.05 * np.linalg.norm(g2)
+
+
+def test_selection_gradient_matches_finite_differences():
+    rng = np.random.default_rng(13)
+    params = G.init_params(seed=14, scale_out=1.0)
+    ws, bs = G.unpack(params)
+    ws[3][:, G.SEL] = rng.uniform(-0.3, 0.3, 64)  # a non-trivial selection column
+    bs[3][G.SEL] = 0.4
+    params = G.pack(ws, bs)
+    smp = _samples(rng, 24)
+    smp[:, 13] = rng.uniform(0.02, 1.5, 24)
+    smp[::5, 13] = 0.0  # no sampler density recorded: no selection term
+    box = ((-1, -1, -1), (1, 1, 1))
+    _, grad, _ = G.step(params, smp, *box, bf=False)
+    x = G.features(smp[:, 0:3], smp[:, 3:6], smp[:, 6:9], *box)
+    p0 = G.pdf(G.forward(params, x, bf=False)[1][:, :G.N_OUT], smp[:, 9:12])
+
+    def total(pr):
+        _, out = G.forward(pr, x, bf=False)
+        return (G.loss(out[:, :G.N_OUT], smp[:, 9:12], smp[:, 12])
+                + G.sel_loss(out[:, G.SEL], p0, smp[:, 13], smp[:, 12]))
+
+    wo3, bo3 = G.OFFS[3]
+    idx = [wo3 + r * 48 + G.SEL for r in range(0, 64, 7)] + [bo3 + G.SEL]
+    for (wo, bo), (i, j) in zip(G.OFFS[:3], G.LAYERS[:3]):
+        idx += [wo + r * j + c for r, c in zip(rng.integers(0, i, 4), rng.integers(0, j, 4))]
+    h = 1e-6
+    for k in idx:
+        pp, pm = params.copy(), params.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (total(pp) - total(pm)) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-5 + 1e-4 * abs(fd), (k, fd, grad[k])
+    # the padding columns 41..47 get no gradient
+    assert not grad[wo3:bo3].reshape(64, 48)[:, G.SEL + 1:].any() and not grad[bo3 + G.SEL + 1:bo3 + 48].any()
+    # a guide density above the sampler's at every sample pushes a up (gradient < 0 on the logit)
+    one = smp[:1].copy()
+    one[0, 13] = 1e-3
+    o = G.forward(params, G.features(one[:, 0:3], one[:, 3:6], one[:, 6:9], *box), bf=False)[1]
+    assert G.pdf(o[:, :G.N_OUT], one[:, 9:12])[0] > 1e-3
+    assert G.sel_grad(o[:, G.SEL], G.pdf(o[:, :G.N_OUT], one[:, 9:12]), one[:, 13], one[:, 12])[0] < 0
