@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "lucid_hip.h"
+#include "scene_build.h"
 #include "scene_pack.h"
 #include "shading.h"
 #include "diag_film.h"
@@ -2080,6 +2081,8 @@ struct lucid_scene {
     SceneDev dev;
     size_t grid_bytes = 0;  // device bytes of the density grids
     float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {1.0f, 1.0f, 1.0f};  // scene box
+    int64_t n_nodes = 0;                  // traversal nodes
+    int32_t built_on_device = 0;
 };
 
 // A stream with its own wavefront buffers.  Renders on a context use lane 0,
@@ -2135,6 +2138,7 @@ struct lucid_ctx {
     int32_t shade_order = 0;
     int32_t session_graphs = 1;  // replay plain one-wave session passes from captured graphs
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
+    int32_t scene_build = 1;  // 1: BVHs and triangle records built on the device, 0: on the host
     float4* film = nullptr;
     int64_t film_cap = 0;
     float* out_dev = nullptr;
@@ -2924,36 +2928,63 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     if (!s) return fail(ctx, LUCID_ERR_NOMEM, "out of host memory");
     s->device = ctx->device;
     std::string err;
-    if (!pack_scene(meshes, num_meshes, lights, num_lights, env_color, env_strength, s->host, err)) {
+    const bool device_build = ctx->scene_build == 1;
+    if (!pack_scene(meshes, num_meshes, lights, num_lights, env_color, env_strength, s->host, err, !device_build)) {
         delete s;
         return fail(ctx, LUCID_ERR_INVALID, err);
     }
-    const PackedScene& ps = s->host;
-    if (ps.max_depth + 2 > kStackMax) {
-        delete s;
-        return fail(ctx, LUCID_ERR_INVALID, "BVH deeper than the traversal stack");
-    }
     SceneDev& D = s->dev;
     std::memset(&D, 0, sizeof(D));
+    GNode root_node;
+    std::memset(&root_node, 0, sizeof(root_node));
+    int rc = LUCID_OK;
+    if (device_build) {
+        build::DeviceGeometry g;
+        const bool ok = build::build_geometry(meshes, num_meshes, s->host, ctx->stream, g, err);
+        s->allocs.insert(s->allocs.end(), g.allocs.begin(), g.allocs.end());
+        if (!ok) {
+            lucid_scene_release(s);
+            return fail(ctx, LUCID_ERR_DEVICE, err);
+        }
+        D.nodes = g.nodes;
+        D.tri = g.tri;
+        D.fn = g.fn;
+        D.vn = g.vn;
+        D.tuv = g.tuv;
+        s->host.root = g.root;
+        s->host.max_depth = g.max_depth;
+        s->n_nodes = g.n_nodes;
+        s->built_on_device = 1;
+        root_node = g.root_node;
+    }
+    const PackedScene& ps = s->host;
+    if (ps.max_depth + 2 > kStackMax) {
+        lucid_scene_release(s);
+        return fail(ctx, LUCID_ERR_INVALID, "BVH deeper than the traversal stack");
+    }
     auto up4 = [&](const std::vector<float>& v, const float4** dst) {
         const float* p = nullptr;
         int rc = upload(ctx, s, v, &p);
         *dst = reinterpret_cast<const float4*>(p);
         return rc;
     };
-    int rc = upload(ctx, s, ps.nodes, &D.nodes);
-    // triangles interleaved: a leaf's triangles are one contiguous block
-    std::vector<float> tri3 = interleave_triangles(ps);
-    if (!rc) rc = up4(tri3, &D.tri);
-    if (!rc) rc = up4(ps.tri_fn, &D.fn);
-    if (!rc) rc = up4(ps.tri_vn, &D.vn);
+    if (!device_build) {
+        rc = upload(ctx, s, ps.nodes, &D.nodes);
+        // triangles interleaved: a leaf's triangles are one contiguous block
+        std::vector<float> tri3 = interleave_triangles(ps);
+        if (!rc) rc = up4(tri3, &D.tri);
+        if (!rc) rc = up4(ps.tri_fn, &D.fn);
+        if (!rc) rc = up4(ps.tri_vn, &D.vn);
+        if (!rc) {
+            const float* uv = nullptr;
+            rc = upload(ctx, s, ps.tri_uv, &uv);
+            D.tuv = reinterpret_cast<const float2*>(uv);
+        }
+        s->n_nodes = (int64_t)ps.nodes.size();
+        if (!ps.nodes.empty()) root_node = ps.nodes[ps.root];
+    }
     if (!rc) rc = upload(ctx, s, ps.mats, &D.mats);
     if (!rc) rc = upload(ctx, s, ps.node_ops, &D.prog);
-    if (!rc) {
-        const float* uv = nullptr;
-        rc = upload(ctx, s, ps.tri_uv, &uv);
-        D.tuv = reinterpret_cast<const float2*>(uv);
-    }
     if (!rc) rc = upload(ctx, s, ps.lights, &D.lights);
     if (!rc) rc = upload(ctx, s, ps.cdf, &D.cdf);
     if (!rc) rc = upload(ctx, s, ps.native_light_ids, &D.native_ids);
@@ -3001,8 +3032,8 @@ int lucid_scene_create(lucid_ctx* ctx, const lucid_mesh_desc* meshes, int32_t nu
     std::memcpy(D.env, ps.env, sizeof(D.env));
     D.native_start = ps.native_start;
     // scene box = union of the root's child boxes
-    if (!ps.nodes.empty()) {
-        const GNode& r = ps.nodes[ps.root];
+    if (s->n_nodes > 0) {
+        const GNode& r = root_node;
         for (int c = 0; c < 3; ++c) {
             s->bmin[c] = r.b[c];
             s->bmax[c] = r.b[3 + c];
@@ -3357,6 +3388,39 @@ int lucid_set_grid_format(lucid_ctx* ctx, int32_t bits) {
 }
 
 int64_t lucid_scene_grid_bytes(const lucid_scene* s) { return s ? (int64_t)s->grid_bytes : 0; }
+
+int lucid_set_scene_build(lucid_ctx* ctx, int32_t mode) {
+    if (!ctx || (mode != 0 && mode != 1)) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->scene_build = mode;
+    return LUCID_OK;
+}
+
+int lucid_scene_read_geometry(lucid_ctx* ctx, lucid_scene* s, void* nodes, float* tri, float* fn, float* vn,
+                              float* uv, int64_t* n_nodes, int32_t* stack_depth, int32_t* built_on_device) {
+    if (!ctx || !s) return fail(ctx, LUCID_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n_nodes) *n_nodes = s->n_nodes;
+    if (stack_depth) *stack_depth = s->host.max_depth;
+    if (built_on_device) *built_on_device = s->built_on_device;
+    if (!nodes) return LUCID_OK;
+    HIP_OK(hipSetDevice(s->device));
+    const size_t nt = (size_t)s->host.num_triangles;
+    const SceneDev& D = s->dev;
+    HIP_OK(hipDeviceSynchronize());
+    if (s->n_nodes) HIP_OK(hipMemcpy(nodes, D.nodes, (size_t)s->n_nodes * sizeof(GNode), hipMemcpyDeviceToHost));
+    if (nt && tri) HIP_OK(hipMemcpy(tri, D.tri, nt * 48, hipMemcpyDeviceToHost));
+    if (nt && fn) HIP_OK(hipMemcpy(fn, D.fn, nt * 16, hipMemcpyDeviceToHost));
+    if (vn) {
+        if (nt && D.vn) HIP_OK(hipMemcpy(vn, D.vn, nt * 48, hipMemcpyDeviceToHost));
+        else std::memset(vn, 0, nt * 48);
+    }
+    if (uv) {
+        if (nt && D.tuv) HIP_OK(hipMemcpy(uv, D.tuv, nt * 24, hipMemcpyDeviceToHost));
+        else std::memset(uv, 0, nt * 24);
+    }
+    return LUCID_OK;
+}
 
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths) {
     if (!ctx || paths < 0) return LUCID_ERR_INVALID;

@@ -279,7 +279,7 @@ bool convert_light(const lucid_light_desc& d, GLight& l, std::string& err) {
 
 bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_light_desc* lights,
                 int32_t num_lights, const float env_color[3], float env_strength, PackedScene& ps,
-                std::string& err) {
+                std::string& err, bool geometry) {
     ps = PackedScene();
     if (num_meshes < 0 || num_lights < 0 || (num_meshes > 0 && !meshes) || (num_lights > 0 && !lights)) {
         err = "invalid mesh or light array";
@@ -308,17 +308,19 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
     }
     ps.num_triangles = (int32_t)total;
     ps.num_meshes = num_meshes;
-    ps.tri_v0.resize(4 * total);
-    ps.tri_e1.resize(4 * total);
-    ps.tri_e2.resize(4 * total);
-    ps.tri_fn.resize(4 * total);
+    if (geometry) {
+        ps.tri_v0.resize(4 * total);
+        ps.tri_e1.resize(4 * total);
+        ps.tri_e2.resize(4 * total);
+        ps.tri_fn.resize(4 * total);
+    }
     for (int m = 0; m < num_meshes; ++m)
         if (meshes[m].smooth && meshes[m].tri_normals && meshes[m].num_tri_normals > 0) ps.any_smooth = true;
-    if (ps.any_smooth) ps.tri_vn.assign(12 * total, 0.0f);
+    if (ps.any_smooth && geometry) ps.tri_vn.assign(12 * total, 0.0f);
 
     // ---- per-mesh BVHs (independent, built in parallel) ----
     std::vector<MeshBvh> bvhs(num_meshes);
-    {
+    if (geometry) {
         std::atomic<int> next{0};
         auto worker = [&]() {
             for (int m = next.fetch_add(1); m < num_meshes; m = next.fetch_add(1))
@@ -462,6 +464,15 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
         }
     }
 
+    if (!geometry) {  // the device builds the BVHs and triangle records
+        ps.tri_light.reserve((size_t)total);
+        for (int m = 0; m < num_meshes; ++m) ps.tri_light.insert(ps.tri_light.end(), tri_light[m].begin(), tri_light[m].end());
+        ps.env[0] = env_color[0] * env_strength;
+        ps.env[1] = env_color[1] * env_strength;
+        ps.env[2] = env_color[2] * env_strength;
+        return true;
+    }
+
     // ---- triangles in BVH order ----
     if (ps.any_nodes) ps.tri_uv.assign(6 * total, 0.0f);
     for (int m = 0; m < num_meshes; ++m) {
@@ -503,19 +514,37 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
     // ---- node array: TLAS on top, mesh BLASes below ----
     ps.nodes.clear();
     ps.nodes.push_back(GNode());  // root placeholder at index 0
-    std::vector<MeshRoot> roots;
-    int blas_depth_max = 0;
+    std::vector<TopRoot> tops;
     for (int m = 0; m < num_meshes; ++m) {
         if (meshes[m].num_triangles == 0) continue;
         const MeshBvh& bvh = bvhs[m];
-        MeshRoot r;
-        r.bmin = bvh.nodes[0].bmin;
-        r.bmax = bvh.nodes[0].bmax;
-        r.centroid = f3(0.5f * (r.bmin.x + r.bmax.x), 0.5f * (r.bmin.y + r.bmax.y), 0.5f * (r.bmin.z + r.bmax.z));
+        TopRoot t;
+        put3(t.bmin, bvh.nodes[0].bmin);
+        put3(t.bmax, bvh.nodes[0].bmax);
         int d = 0;
-        r.enc = emit_blas(bvh, 0, tri_off[m], ps.nodes, 0, d);
-        r.depth = d;
-        blas_depth_max = std::max(blas_depth_max, d);
+        t.enc = emit_blas(bvh, 0, tri_off[m], ps.nodes, 0, d);
+        t.depth = d;
+        tops.push_back(t);
+    }
+    ps.root = 0;
+    ps.max_depth = build_top_level(tops, ps.nodes);
+
+    // ---- environment: to_rgb3f(color) * strength (bvh.hpp:187-189) ----
+    ps.env[0] = env_color[0] * env_strength;
+    ps.env[1] = env_color[1] * env_strength;
+    ps.env[2] = env_color[2] * env_strength;
+    return true;
+}
+
+int build_top_level(const std::vector<TopRoot>& tops, std::vector<GNode>& nodes) {
+    std::vector<MeshRoot> roots;
+    for (const TopRoot& t : tops) {
+        MeshRoot r;
+        r.bmin = get3(t.bmin);
+        r.bmax = get3(t.bmax);
+        r.centroid = f3(0.5f * (r.bmin.x + r.bmax.x), 0.5f * (r.bmin.y + r.bmax.y), 0.5f * (r.bmin.z + r.bmax.z));
+        r.enc = t.enc;
+        r.depth = t.depth;
         roots.push_back(r);
     }
     GNode root;
@@ -545,22 +574,15 @@ bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_l
         });
         F3 lmin, lmax, rmin, rmax;
         int dl = 0;
-        int32_t cl = emit_tlas(roots, 0, mid, ps.nodes, 1, dl, lmin, lmax);
-        int32_t cr = emit_tlas(roots, mid, (int)roots.size(), ps.nodes, 1, dl, rmin, rmax);
+        int32_t cl = emit_tlas(roots, 0, mid, nodes, 1, dl, lmin, lmax);
+        int32_t cr = emit_tlas(roots, mid, (int)roots.size(), nodes, 1, dl, rmin, rmax);
         put3(root.b + 0, lmin); put3(root.b + 3, lmax); put3(root.b + 6, rmin); put3(root.b + 9, rmax);
         root.child[0] = cl;
         root.child[1] = cr;
         depth = std::max(1, dl);
     }
-    ps.nodes[0] = root;
-    ps.root = 0;
-    ps.max_depth = depth + 1;
-
-    // ---- environment: to_rgb3f(color) * strength (bvh.hpp:187-189) ----
-    ps.env[0] = env_color[0] * env_strength;
-    ps.env[1] = env_color[1] * env_strength;
-    ps.env[2] = env_color[2] * env_strength;
-    return true;
+    nodes[0] = root;
+    return depth + 1;
 }
 
 CameraBasis camera_basis(const lucid_camera& cam, int32_t full_w, int32_t full_h) {

@@ -148,12 +148,31 @@ struct PackedScene {
     bool any_smooth = false;
     bool any_volume = false;
     bool any_nodes = false;                  // a material with MF_NODES (tri_uv filled)
+    // without geometry: the light index of every triangle (mesh, triangle
+    // order; -1: not a light), for the device build
+    std::vector<int32_t> tri_light;
 };
 
-// Packs the scene; returns false and sets err on invalid input.
+// Packs the scene; returns false and sets err on invalid input.  With
+// geometry = false the BVHs, the node array and the triangle records are
+// left to the device build (scene_build.hip); materials, lights, volumes and
+// the environment are packed as always.
 bool pack_scene(const lucid_mesh_desc* meshes, int32_t num_meshes, const lucid_light_desc* lights,
                 int32_t num_lights, const float env_color[3], float env_strength, PackedScene& out,
-                std::string& err);
+                std::string& err, bool geometry = true);
+
+// A mesh's BVH root as the top-level tree sees it: its box, its child
+// encoding (node index or leaf) and the depth of its subtree.
+struct TopRoot {
+    float bmin[3], bmax[3];
+    int32_t enc;
+    int depth;
+};
+// The top-level tree over the mesh roots (median split on the longest axis
+// of the root-box centroids, std::nth_element): its nodes are appended to
+// `nodes`, whose index 0 (reserved by the caller) becomes the root.  Returns
+// the deepest node path, for the traversal stack.
+int build_top_level(const std::vector<TopRoot>& roots, std::vector<GNode>& nodes);
 
 // A material as getMaterialParams / getEmission see it at one uv
 // (include/integrator/path_tracer.hpp:45-72), compiled from a mesh

@@ -165,6 +165,12 @@ SIGNATURES = {
     "lucid_set_profiling": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_grid_format": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_scene_grid_bytes": (C.c_int64, [C.c_void_p]),
+    "lucid_set_scene_build": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_scene_read_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                            C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lucid_debug_select_nth": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                         C.c_int32]),
     "lucid_session_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.POINTER(C.c_void_p)]),
     "lucid_session_destroy": (None, [C.c_void_p]),

@@ -472,6 +472,14 @@ int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, in
                            const float* d, const float* tmin, const float* tmax, int32_t any_hit, int32_t* mesh,
                            int32_t* tri, float* u, float* v, float* t);
 
+/* The device BVH build's selection (scene_build.hip select_nth) run on the
+   host: permutes keys and idx together exactly as std::nth_element (GNU
+   libstdc++ introselect) orders n elements around position nth by key, as
+   the reference's BVH build calls it (include/core/scene.hpp:138-142).
+   depth_limit < 0: the library's own partition budget, 2 floor(log2 n);
+   >= 0: that budget (the heap-select fallback at 0).  For tests. */
+int lucid_debug_select_nth(float* keys, int32_t* idx, int32_t n, int32_t nth, int32_t depth_limit);
+
 /* Storage of the density grids of scenes created after this call: 32 (FP32,
    the reference's values, default) or 16 (FP16, round to nearest even; half
    the device bytes, arithmetic still FP32; each medium's delta-tracking
@@ -482,6 +490,22 @@ int lucid_debug_host_trace(const lucid_mesh_desc* meshes, int32_t num_meshes, in
 int lucid_set_grid_format(lucid_ctx* ctx, int32_t bits);
 /* Device bytes of a scene's density grids. */
 int64_t lucid_scene_grid_bytes(const lucid_scene* scene);
+
+/* Where scenes created from now on build their per-mesh BVHs, node array and
+   triangle records: 1 on the device (the default), 0 on the host.  Both give
+   the same bytes: the reference's median-split trees (core/scene.hpp:65-152,
+   std::nth_element order) under a top-level tree over the mesh roots. */
+int lucid_set_scene_build(lucid_ctx* ctx, int32_t mode);
+/* The scene's traversal data in host memory, for tests of the two builds:
+   n_nodes nodes of 64 bytes (pass nodes = NULL to read only the counts),
+   then per triangle in BVH order tri (12 floats: v0 | mesh, e1 | triangle,
+   e2 | light index, ids as int32 bits), fn (4 floats: face normal | smooth
+   flag), vn (12 floats: 3 vertex normals; zeros without smooth meshes) and
+   uv (6 floats; zeros without node materials).  Any of tri / fn / vn / uv
+   may be NULL.  stack_depth: the deepest node path; built_on_device: 1 for a
+   device build. */
+int lucid_scene_read_geometry(lucid_ctx* ctx, lucid_scene* scene, void* nodes, float* tri, float* fn, float* vn,
+                              float* uv, int64_t* n_nodes, int32_t* stack_depth, int32_t* built_on_device);
 
 /* Instrumentation flags (LUCID_PROFILE_*), 0 = off (default). */
 int lucid_set_profiling(lucid_ctx* ctx, int32_t flags);

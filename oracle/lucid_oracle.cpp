@@ -2852,6 +2852,27 @@ int orc_render_tile_diag(void* p, const char* algorithm, int tx, int ty, int tw,
 // ghash, gcount, gmean[3], gvar[3], gdepth, gtype0 (bsdf type of vertex 0);
 // totals: tmean[3], tvar[3]; coarse stats: ccount[8], cmean[24]; for
 // <16, 4>, the pixel in the reference's byte layout when bytes is given.
+// std::nth_element over triangle ids compared by their keys, as the
+// reference's BVH build calls it (core/scene.hpp:138-142); keys and ids are
+// permuted together.  depth_limit >= 0 runs the library's introselect with
+// that partition budget (GNU libstdc++; tests of the heap-select fallback).
+void orc_nth_element(float* keys, int32_t* idx, int n, int nth, int depth_limit) {
+    std::vector<int32_t> ord(n);
+    std::vector<float> k0(keys, keys + n);
+    std::vector<int32_t> i0(idx, idx + n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    auto comp = [&](int32_t a, int32_t b) { return k0[a] < k0[b]; };
+    if (depth_limit < 0)
+        std::nth_element(ord.begin(), ord.begin() + nth, ord.end(), comp);
+    else if (n > 0 && nth < n)
+        std::__introselect(ord.begin(), ord.begin() + nth, ord.end(), depth_limit,
+                           __gnu_cxx::__ops::__iter_comp_iter(comp));
+    for (int i = 0; i < n; ++i) {
+        keys[i] = k0[ord[i]];
+        idx[i] = i0[ord[i]];
+    }
+}
+
 void orc_diag_pixel_run(int max_groups, int max_outliers, int n, const int32_t* depth, const int32_t* obj,
                         const int16_t* mat, const uint8_t* type, const uint8_t* flags, const float* contrib,
                         const uint8_t* ltype, const uint8_t* strat, uint64_t* hash, uint8_t* coarse, long long* summary,

@@ -588,3 +588,14 @@ def diag_pixel_run(records, max_groups=16, max_outliers=4):
 def flip_to_top_first(rgba_flat: np.ndarray, tw: int, th: int) -> np.ndarray:
     """(th, tw, 4) image with row 0 = top, from a bottom-row-first buffer."""
     return rgba_flat.reshape(th, tw, 4)[::-1]
+
+
+def nth_element(keys, idx, nth, depth_limit=-1):
+    """std::nth_element of the reference's BVH build on (keys, ids): both
+    permuted as the library moves them.  depth_limit >= 0 runs the library's
+    introselect with that partition budget."""
+    k = np.ascontiguousarray(keys, np.float32).copy()
+    i = np.ascontiguousarray(idx, np.int32).copy()
+    lib().orc_nth_element(k.ctypes.data_as(C.POINTER(C.c_float)), i.ctypes.data_as(C.POINTER(C.c_int32)), len(k),
+                          int(nth), int(depth_limit))
+    return k, i
