@@ -1309,6 +1309,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
     // cancel requests are polled once per launch, i.e. between bounces;
     // wavefront-uniform, so the ballots in shade_one stay convergent
     const bool stop = __builtin_amdgcn_readfirstlane(stop_requested(W) ? 1 : 0) != 0;
+    // a wave whose paths this launch ends can never reach the film: latch it
+    // here, not only from the read after the wave's last launch
+    if (stop && threadIdx.x == 0) W.cnt[7] = 1;
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
         shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n, stop);
 }
@@ -1539,8 +1542,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W, int sd
 
 // ------------------------------------------------------------------ film --
 // Latches the cancel word once per wave, after its last shading launch:
-// cnt[7] = 1 once a cancel has been seen.  Every later kernel of the render
-// reads the latch, so all of them agree on one value.
+// cnt[7] = 1 once a cancel has been seen, here or by a shading launch that
+// ended paths because of it (which sets the latch itself: its read and this
+// one may run on different XCDs).  Every later kernel of the render reads
+// the latch, so all of them agree on one value.
 __global__ __launch_bounds__(64) void k_latch_stop(WaveDev W) {
     if (threadIdx.x == 0 && stop_requested(W)) W.cnt[7] = 1;
 }
@@ -2860,7 +2865,9 @@ int lucid_create(int32_t device, lucid_ctx** out) {
     }
     ctx->lanes[0].stream = ctx->stream;
     void* dp = nullptr;
-    if (hipHostMalloc((void**)&ctx->cancel_host, 64, hipHostMallocMapped) != hipSuccess ||
+    // coherent (uncached on the device): every kernel read sees the host's
+    // latest write, whichever XCD it runs on
+    if (hipHostMalloc((void**)&ctx->cancel_host, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer(&dp, ctx->cancel_host, 0) != hipSuccess) {
         if (ctx->cancel_host) (void)hipHostFree(ctx->cancel_host);
         (void)hipStreamDestroy(ctx->stream);
