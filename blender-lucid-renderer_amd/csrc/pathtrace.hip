@@ -3505,6 +3505,20 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
         double t = 0.0;
         if (!rc && v != sg && !(rc = measure(ro, tg, rf, v, &t)) && t < t_best) { t_best = t; sg = v; }
     }
+    // wavefront batch size (the queue length every launch works through):
+    // the automatic one wave, or the pass split into 2 or 4 waves of whole
+    // samples, when those still hold >= 2^20 paths
+    const int64_t P = (int64_t)tp.tile_w * tp.tile_h, total = P * tp.samples;
+    const int64_t wave_saved = ctx->wave_paths;
+    int64_t wp = wave_saved;
+    for (int split : {2, 4}) {
+        const int64_t per = ((tp.samples + split - 1) / split) * P;
+        if (rc || per >= total || per < ((int64_t)1 << 20)) continue;
+        ctx->wave_paths = per;
+        double t = 0.0;
+        if (!(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; wp = per; }
+    }
+    ctx->wave_paths = wp;
     ctx->ray_order = ro;
     ctx->traverse_groups = tg;
     ctx->refill = rf;
@@ -3519,6 +3533,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     best->ray_order = ro;
     best->shade_order = so;
     best->ms = t_best;
+    best->wave_paths = wp;
     return rc;
 }
 

@@ -615,6 +615,7 @@ public:
         d["shade_groups"] = b.shade_groups;
         d["ray_order"] = b.ray_order;
         d["shade_order"] = b.shade_order;
+        d["wave_paths"] = b.wave_paths;
         d["ms"] = b.ms;
         return d;
     }

@@ -104,7 +104,8 @@ class GuidingStats(C.Structure):
 
 class LaunchShape(C.Structure):
     _fields_ = [("traverse_groups", C.c_int32), ("refill", C.c_int32), ("shade_groups", C.c_int32),
-                ("ray_order", C.c_int32), ("shade_order", C.c_int32), ("ms", C.c_double)]
+                ("ray_order", C.c_int32), ("shade_order", C.c_int32), ("ms", C.c_double),
+                ("wave_paths", C.c_int64)]
 
 
 SIGNATURES = {

@@ -306,13 +306,17 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
    by coordinate descent -- the ray order (both values), traversal groups
    per CU in {8, 12, 16, 24}, then
    the refill threshold in {32, 48, 60}, then shading groups per CU in
-   {4, 8, 16}, two renders each -- and keeps the fastest on the context.
-   Launch shapes do not change results.  Not with diagnostics enabled. */
+   {4, 8, 16}, then the wavefront batch (one wave, or the pass split into 2
+   or 4 waves of whole samples, each of at least 2^20 paths:
+   lucid_set_wave_paths), two renders each -- and keeps the fastest on the
+   context.  Launch shapes do not change results.  Not with diagnostics
+   enabled. */
 typedef struct lucid_launch_shape {
     int32_t traverse_groups, refill, shade_groups;
     int32_t ray_order;  /* lucid_set_ray_order: 0 or 1 */
     int32_t shade_order;  /* lucid_set_shade_order: 0 or 1 */
     double ms;          /* device time of the chosen shape's render */
+    int64_t wave_paths; /* lucid_set_wave_paths: 0 automatic, or paths per batch */
 } lucid_launch_shape;
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);

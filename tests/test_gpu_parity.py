@@ -189,6 +189,21 @@ def test_autotune_picks_a_shape_and_keeps_results(lucid):
     shape = r.autotune(64, 48, 4, 8)
     assert shape["traverse_groups"] in (8, 12, 16, 24) and shape["refill"] in (32, 48, 60)
     assert shape["shade_groups"] in (4, 8, 16) and shape["ray_order"] in (0, 1) and shape["ms"] > 0
-    assert shape["shade_order"] in (0, 1)
+    assert shape["shade_order"] in (0, 1) and shape["wave_paths"] == 0  # too few paths to split
     after = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
     assert np.array_equal(before, after)
+
+
+def test_autotune_wave_size_keeps_results(lucid):
+    """The wavefront batch is searched too (one wave, or the pass in 2 or 4
+    waves of whole samples); the film adds samples in the same order either
+    way, so the image is bit-identical."""
+    r = make_renderer(lucid, BOX, BOX_CAM, "mis")
+    W = H = 512
+    before = np.asarray(r.render_tile(0, 0, W, H, W, H, 16, 3, 8))
+    shape = r.autotune(W, H, 16, 8)
+    assert shape["wave_paths"] in (0, W * H * 8, W * H * 4)
+    after = np.asarray(r.render_tile(0, 0, W, H, W, H, 16, 3, 8))
+    assert np.array_equal(before, after)
+    r.set_wave_paths(W * H * 4)  # four waves of 4 samples
+    assert np.array_equal(np.asarray(r.render_tile(0, 0, W, H, W, H, 16, 3, 8)), before)
