@@ -254,12 +254,22 @@ struct HostBounds {
 };
 #if defined(__HIP_DEVICE_COMPILE__)
 #define LUCID_HOST_CHECK(cond) ((void)0)
+#define LUCID_HOST_CHECK_VOID(cond) ((void)0)
 #else
 #define LUCID_HOST_CHECK(cond)                               \
     do {                                                      \
         if (hb && !(cond)) {                                  \
             hb->violated = true;                              \
             return false;                                     \
+        }                                                     \
+    } while (0)
+#define LUCID_HOST_CHECK_VOID(cond)                          \
+    do {                                                      \
+        if (hb && !(cond)) {                                  \
+            hb->violated = true;                              \
+            node = kPop;                                      \
+            sp = 0;                                           \
+            return;                                           \
         }                                                     \
     } while (0)
 #endif
@@ -271,57 +281,47 @@ LUCID_HD void note_overflow(int32_t* ovf) {
 #endif
 }
 
-// One step of the traversal: visits `node` (tests both child boxes, the
-// triangles of leaf children, then descends, pushes or pops).  Returns false
-// when the traversal has ended (stack empty, or an any-hit found).  The
-// state (node, sp, h) lives with the caller, so a persistent kernel can
-// interleave steps of different rays on one lane.
+// Traversal with postponed leaves, the form the persistent kernels run: a
+// ray's state is the internal node to visit next (or kPop: take the next
+// stack entry), at most one leaf waiting to be tested, and the stack, whose
+// entries are internal nodes or leaves.  A node visit tests both child
+// boxes, goes on to the nearer hit child (or holds it, if it is a leaf) and
+// pushes the farther one; so a wavefront can walk nodes until most of its
+// lanes hold a leaf and then test those leaves together, instead of leaving
+// most lanes idle while a few test triangles.  The result is the closest-hit
+// rule's (least t, then the larger BVH-order position), which does not
+// depend on the order of the tests: only a box whose entry distance exceeds
+// the best t so far by more than the 1 % pruning margin is skipped, and a
+// leaf taken from the stack is tested without that check (more work, the
+// same hit).
+constexpr int32_t kPop = INT32_MIN;  // no leaf encodes to it (count 0)
+
+LUCID_HD bool ray_done(int32_t node, int32_t leaf, int sp) { return node == kPop && leaf == 0 && sp == 0; }
+
 template <bool ANY>
-LUCID_HD bool trav_step(const SceneDev& S, v3f o, v3f d, v3f id, float tmin, int32_t* stk, int sd, int32_t* ovf,
-                        uint32_t& nn, uint32_t& nt, TraceHit& h, int32_t& node, int& sp, HostBounds* hb) {
+LUCID_HD void node_visit(const SceneDev& S, v3f o, v3f d, v3f id, int32_t* stk, int sd, int32_t* ovf, uint32_t& nn,
+                         const TraceHit& h, int32_t& node, int32_t& leaf, int& sp, HostBounds* hb) {
     (void)hb;
-    LUCID_HOST_CHECK(node >= 0 && node < hb->n_nodes);
+    if (node == kPop) {
+        if (sp == 0) return;
+        --sp;
+        const int32_t e = stk[sp * kBlock];
+        if (e < 0) leaf = e;
+        else node = e;
+        return;
+    }
+    LUCID_HOST_CHECK_VOID(node >= 0 && node < hb->n_nodes);
     const float4* n4 = reinterpret_cast<const float4*>(S.nodes + node);
     const float4 b0 = n4[0], b1 = n4[1], b2 = n4[2];
     const int2 cc = *reinterpret_cast<const int2*>(n4 + 3);
     ++nn;
     const float lim = prune_bound(h.t);
     float te0 = 0.0f, te1 = 0.0f;
-    bool h0 = cc.x != kEmptyChild && box_hit(o, d, id, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, te0) && te0 <= lim;
-    bool h1 = cc.y != kEmptyChild && box_hit(o, d, id, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w, te1) && te1 <= lim;
+    const bool h0 = cc.x != kEmptyChild && box_hit(o, d, id, b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, te0) && te0 <= lim;
+    const bool h1 = cc.y != kEmptyChild && box_hit(o, d, id, b1.z, b1.w, b2.x, b2.y, b2.z, b2.w, te1) && te1 <= lim;
     const bool sw = h1 && (!h0 || te1 < te0);
-    int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
-    bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
-    const float tf = sw ? te0 : te1;
-    // Leaf children are consumed here, whether tested or pruned, so only
-    // internal nodes (index >= 0) are ever descended into or pushed.
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int32_t c = k == 0 ? cn : cf;
-        const bool take = k == 0 ? hn : hf;
-        if (take && c < 0) {
-            if (k == 0) hn = false; else hf = false;
-            if (k == 1 && !(tf <= prune_bound(h.t))) continue;  // pruned by the near leaf's hit
-            const int32_t enc = ~c;
-            const int32_t start = enc >> 3, count = enc & 7;
-            nt += count;
-            for (int i = 0; i < count; ++i) {
-                const int32_t prim = start + i;
-                LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
-                float t, u, v;
-                const float4* tp = S.tri + 3 * (size_t)prim;
-                if (tri_test(o, d, tp[0], tp[1], tp[2], tmin, h.t, t, u, v)) {
-                    if (ANY) {
-                        h.t = t; h.u = u; h.v = v; h.prim = prim;
-                        return false;
-                    }
-                    if (t < h.t || prim > h.prim) {
-                        h.t = t; h.u = u; h.v = v; h.prim = prim;
-                    }
-                }
-            }
-        }
-    }
+    const int32_t cn = sw ? cc.y : cc.x, cf = sw ? cc.x : cc.y;
+    const bool hn = sw ? h1 : h0, hf = sw ? h0 : h1;
     if (hn && hf) {
         if (sp < sd) {
             stk[sp * kBlock] = cf;
@@ -329,15 +329,39 @@ LUCID_HD bool trav_step(const SceneDev& S, v3f o, v3f d, v3f id, float tmin, int
         } else {
             note_overflow(ovf);
         }
-        node = cn;
-        return true;
     }
-    if (hn) { node = cn; return true; }
-    if (hf) { node = cf; return true; }
-    if (sp == 0) return false;
-    --sp;
-    node = stk[sp * kBlock];
-    return true;
+    const int32_t next = hn ? cn : (hf ? cf : kPop);
+    node = kPop;
+    if (next != kPop) {
+        if (next < 0) leaf = next;
+        else node = next;
+    }
+}
+
+// Tests the leaf's triangles; true when an any-hit query has its answer.
+template <bool ANY>
+LUCID_HD bool leaf_visit(const SceneDev& S, v3f o, v3f d, float tmin, uint32_t& nt, TraceHit& h, int32_t leaf,
+                         HostBounds* hb) {
+    (void)hb;
+    const int32_t enc = ~leaf;
+    const int32_t start = enc >> 3, count = enc & 7;
+    nt += count;
+    for (int i = 0; i < count; ++i) {
+        const int32_t prim = start + i;
+        LUCID_HOST_CHECK(prim >= 0 && prim < hb->n_tris);
+        float t, u, v;
+        const float4* tp = S.tri + 3 * (size_t)prim;
+        if (tri_test(o, d, tp[0], tp[1], tp[2], tmin, h.t, t, u, v)) {
+            if (ANY) {
+                h.t = t; h.u = u; h.v = v; h.prim = prim;
+                return true;
+            }
+            if (t < h.t || prim > h.prim) {
+                h.t = t; h.u = u; h.v = v; h.prim = prim;
+            }
+        }
+    }
+    return false;
 }
 
 template <bool ANY>
@@ -346,8 +370,18 @@ LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, 
     h.t = tmax; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
     const v3f id = inv_dir(d);
     int sp = 0;
-    int32_t node = S.root;
-    while (trav_step<ANY>(S, o, d, id, tmin, stk, sd, ovf, nn, nt, h, node, sp, hb)) {
+    int32_t node = S.root, leaf = 0;
+    while (!ray_done(node, leaf, sp)) {
+        if (leaf) {
+            const bool any = leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, hb);
+            leaf = 0;
+            if (any) break;
+        } else {
+            node_visit<ANY>(S, o, d, id, stk, sd, ovf, nn, h, node, leaf, sp, hb);
+        }
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (hb && hb->violated) return false;
+#endif
     }
     return h.prim >= 0;
 }
@@ -572,7 +606,8 @@ __device__ __forceinline__ void shadow_unoccluded(const WaveDev& W, int p) {
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill) {
+__global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill,
+                                                         int walk_min) {
     constexpr bool SHADOW = MODE != 0, ANY = MODE == 1;
     extern __shared__ int32_t lds_stack[];
     int32_t* stk = lds_stack + threadIdx.x;
@@ -594,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
     float tmin = 0.0f, tmax = 0.0f, toff = 0.0f;  // MODE 2: shadow distance, offset of the moving origin
     TraceHit h;
     h.t = 0.0f; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-    int32_t node = 0;
+    int32_t node = kPop, leaf = 0;
     int sp = 0, requery = 0;
     for (;;) {
         const unsigned long long im = __ballot(p < 0);
@@ -626,6 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                     id = inv_dir(d);
                     h.u = 0.0f; h.v = 0.0f; h.prim = -1;
                     node = S.root;
+                    leaf = 0;
                     sp = 0;
                     requery = 0;
                     if (MODE == 2 && !(tmax > 0.0f)) {  // nothing left to test: unoccluded
@@ -639,8 +675,26 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             if (exhausted) break;
             continue;
         }
+        // node phase: lanes without a waiting leaf walk nodes, for as long as
+        // more than walk_min lanes of the wavefront still do (and at least
+        // until one lane holds a leaf, so every pass makes progress); then
+        // the waiting leaves are tested together
+        for (;;) {
+            const bool walk = p >= 0 && leaf == 0 && !ray_done(node, leaf, sp);
+            const unsigned long long wm = __ballot(walk);
+            if (wm == 0) break;
+            if (__popcll(wm) <= walk_min && __ballot(p >= 0 && leaf != 0) != 0) break;
+            if (walk) node_visit<ANY>(S, o, d, id, stk, sd, &W.cnt[3], nn, h, node, leaf, sp, nullptr);
+        }
         if (p < 0) continue;
-        if (trav_step<ANY>(S, o, d, id, tmin, stk, sd, &W.cnt[3], nn, nt, h, node, sp, nullptr)) continue;
+        if (leaf != 0) {
+            const bool found = leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, nullptr);
+            leaf = 0;
+            if (!found && !ray_done(node, leaf, sp)) continue;
+        } else if (!ray_done(node, leaf, sp)) {
+            continue;
+        }
+        // the ray's query has ended (stack empty, or an any-hit found)
         if (MODE == 0) {
             // intersect_skipping_volume_boundaries: re-query past a
             // boundary-only mesh; the result of an 8th re-query is discarded
@@ -651,6 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                     tmin = h.t + 1e-3f;
                     h.t = RAY_T_MAX; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
                     node = S.root;
+                    leaf = 0;
                     sp = 0;
                     continue;
                 }
@@ -673,6 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                 if (requery < 8 && remaining > 0.0f) {
                     h.t = remaining; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
                     node = S.root;
+                    leaf = 0;
                     sp = 0;
                     continue;
                 }
@@ -2131,6 +2187,7 @@ struct lucid_ctx {
     // use lets run at once.
     int32_t traverse_groups = 16, shade_groups = 8;
     int32_t refill = 48;  // idle lanes that make a wavefront fetch new rays
+    int32_t walk_min = 16;  // traversal: node walking stops once at most this many lanes still walk
     // 1: bounce and shadow rays regrouped by ray bin before their traversal
     // passes (depth >= 1); 0: never; 2: for waves of at least 2^24 paths.
     // The 1080p occluded room (33M paths a wave) renders 8 % faster with it
@@ -2577,6 +2634,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // the launch shape of this render, read once (callers hold ctx->mu)
     const int32_t traverse_groups = ctx->traverse_groups, shade_groups = ctx->shade_groups;
     const int32_t refill_lanes = ctx->refill, ray_order = ctx->ray_order, shade_order = ctx->shade_order;
+    const int32_t walk_min = ctx->walk_min;
     // traverse_groups > 0: persistent traversal with ray refill on that many
     // workgroups per CU; 0: one lane per queued ray
     const bool refill = traverse_groups > 0;
@@ -2670,7 +2728,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             int ks = kev_begin(ctx, 0, st);
             if (refill)
-                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, refill_lanes);
+                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, refill_lanes, walk_min);
             else
                 k_intersect<<<g, kBlock, lds, st>>>(SD, Wd, q, sd);
             kev_end(ctx, ks, st);
@@ -2742,9 +2800,9 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             ks = kev_begin(ctx, 2, st);
             if (refill && SD.any_vbo)
-                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes);
+                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min);
             else if (refill)
-                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes);
+                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min);
             else
                 k_shadow<<<g, kBlock, lds, st>>>(SD, Ws, sd);
             kev_end(ctx, ks, st);
@@ -3249,11 +3307,11 @@ int lucid_render_debug(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
     const unsigned g = (unsigned)std::min<int64_t>(blocks(P, kBlock), (int64_t)ctx->num_cu * 16);
     k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
     k_raygen<<<blocks(P, kBlock), kBlock, 0, st>>>(W, cd, (int32_t)P);
-    k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48);
+    k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48, ctx->walk_min);
     if (mode == kDebugVolume) {
         k_debug_vol_prep<<<blocks(P, 256), 256, 0, st>>>(SD, W, (int32_t)P);
         k_set_i32<<<1, 64, 0, st>>>(W.cnt + 4, 0);  // fetch counter of the second query
-        k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48);
+        k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48, ctx->walk_min);
     }
     k_debug_out<<<blocks(P, 256), 256, 0, st>>>(SD, W, ctx->out_dev, pr->tile_w, pr->tile_h, mode);
     HIP_OK(hipGetLastError());
@@ -3466,7 +3524,9 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     const int32_t prof = ctx->profiling;
     ctx->profiling = 0;
     int32_t so = 0;  // shading order, searched after the ray order
+    int32_t wk = ctx->walk_min;  // node-walk threshold, searched after the refill threshold
     auto measure = [&](int32_t ro, int32_t tg, int32_t rf, int32_t sg, double* ms) {
+        ctx->walk_min = wk;
         ctx->ray_order = ro;
         ctx->traverse_groups = tg;
         ctx->refill = rf;
@@ -3501,6 +3561,18 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
         double t = 0.0;
         if (!rc && v != rf && !(rc = measure(ro, tg, v, sg, &t)) && t < t_best) { t_best = t; rf = v; }
     }
+    {
+        const int32_t wks[] = {0, 8, 16, 32};
+        const int32_t first = wk;
+        int32_t best_wk = wk;
+        for (int32_t v : wks) {
+            if (rc || v == first) continue;
+            wk = v;
+            double t = 0.0;
+            if (!(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; best_wk = v; }
+        }
+        wk = best_wk;
+    }
     for (int32_t v : sgs) {
         double t = 0.0;
         if (!rc && v != sg && !(rc = measure(ro, tg, rf, v, &t)) && t < t_best) { t_best = t; sg = v; }
@@ -3519,6 +3591,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
         if (!(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; wp = per; }
     }
     ctx->wave_paths = wp;
+    ctx->walk_min = wk;
     ctx->ray_order = ro;
     ctx->traverse_groups = tg;
     ctx->refill = rf;
@@ -3534,7 +3607,15 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     best->shade_order = so;
     best->ms = t_best;
     best->wave_paths = wp;
+    best->walk_min = wk;
     return rc;
+}
+
+int lucid_set_walk_threshold(lucid_ctx* ctx, int32_t lanes) {
+    if (!ctx || lanes < 0 || lanes > 63) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->walk_min = lanes;
+    return LUCID_OK;
 }
 
 int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
@@ -3790,7 +3871,8 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     k_set_i32<<<1, 64, 0, st>>>(W.cnt + 1, n);
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
     const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
-    k_trace_refill<0><<<g, kBlock, (size_t)sd * kBlock * sizeof(int32_t), st>>>(scene->dev, W, 1, sd, 48);
+    k_trace_refill<0><<<g, kBlock, (size_t)sd * kBlock * sizeof(int32_t), st>>>(scene->dev, W, 1, sd, 48,
+                                                                                  ctx->walk_min);
     if (m > 0) HIP_OK(hipMemcpyAsync(W.q[0], order, (size_t)m * 4, hipMemcpyHostToDevice, st));
     k_set_i32<<<1, 64, 0, st>>>(W.cnt, m);
     // outputs start as NaN: entries the kernel must not write stay NaN
@@ -3881,7 +3963,7 @@ struct PassKey {
     lucid_camera cam;
     const float4* accum;
     int32_t w, h, max_depth, algorithm, spp, active_lanes;
-    int32_t traverse_groups, refill, shade_groups, ray_order, shade_order, num_cu;
+    int32_t traverse_groups, refill, shade_groups, ray_order, shade_order, num_cu, walk_min;
 };
 std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp, int active_lanes) {
     PassKey k;
@@ -3893,6 +3975,7 @@ std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp
     k.active_lanes = active_lanes;
     k.traverse_groups = ctx->traverse_groups; k.refill = ctx->refill; k.shade_groups = ctx->shade_groups;
     k.ray_order = ctx->ray_order; k.shade_order = ctx->shade_order; k.num_cu = ctx->num_cu;
+    k.walk_min = ctx->walk_min;
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
     return std::vector<uint8_t>(b, b + sizeof(k));
 }

@@ -616,6 +616,7 @@ public:
         d["ray_order"] = b.ray_order;
         d["shade_order"] = b.shade_order;
         d["wave_paths"] = b.wave_paths;
+        d["walk_min"] = b.walk_min;
         d["ms"] = b.ms;
         return d;
     }
