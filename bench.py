@@ -340,7 +340,7 @@ def main():
         ctx.check(L.lucid_autotune(ctx.h, sc, C.byref(camd), C.byref(tune_pr), C.byref(ls)))
         shape = {"traverse_groups_per_cu": ls.traverse_groups, "refill_idle_lanes": ls.refill,
                  "shade_groups_per_cu": ls.shade_groups, "ray_order": ls.ray_order, "shade_order": ls.shade_order,
-                 "wave_paths": int(ls.wave_paths), "walk_min": int(ls.walk_min)}
+                 "wave_paths": int(ls.wave_paths), "walk_min": int(ls.walk_min), "leaf_tris": int(ls.leaf_tris)}
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {shape}, {ls.ms:.1f} ms per pass")
     if args.shade_order != "auto":
         ctx.check(L.lucid_set_shade_order(ctx.h, int(args.shade_order)))

@@ -338,13 +338,17 @@ LUCID_HD void node_visit(const SceneDev& S, v3f o, v3f d, v3f id, int32_t* stk, 
     }
 }
 
-// Tests the leaf's triangles; true when an any-hit query has its answer.
+// Tests up to `batch` of the leaf's triangles (in order); `leaf` becomes the
+// rest of the leaf, 0 when none is left.  True when an any-hit query has its
+// answer.
 template <bool ANY>
-LUCID_HD bool leaf_visit(const SceneDev& S, v3f o, v3f d, float tmin, uint32_t& nt, TraceHit& h, int32_t leaf,
-                         HostBounds* hb) {
+LUCID_HD bool leaf_visit(const SceneDev& S, v3f o, v3f d, float tmin, uint32_t& nt, TraceHit& h, int32_t& leaf,
+                         int batch, HostBounds* hb) {
     (void)hb;
     const int32_t enc = ~leaf;
-    const int32_t start = enc >> 3, count = enc & 7;
+    const int32_t start = enc >> 3, all = enc & 7;
+    const int32_t count = all < batch ? all : batch;
+    leaf = count < all ? ~(((start + count) << 3) | (all - count)) : 0;
     nt += count;
     for (int i = 0; i < count; ++i) {
         const int32_t prim = start + i;
@@ -373,9 +377,7 @@ LUCID_HD bool traverse(const SceneDev& S, v3f o, v3f d, float tmin, float tmax, 
     int32_t node = S.root, leaf = 0;
     while (!ray_done(node, leaf, sp)) {
         if (leaf) {
-            const bool any = leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, hb);
-            leaf = 0;
-            if (any) break;
+            if (leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, 4, hb)) break;
         } else {
             node_visit<ANY>(S, o, d, id, stk, sd, ovf, nn, h, node, leaf, sp, hb);
         }
@@ -607,7 +609,7 @@ __device__ __forceinline__ void shadow_unoccluded(const WaveDev& W, int p) {
 
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill,
-                                                         int walk_min) {
+                                                         int walk_min, int leaf_tris) {
     constexpr bool SHADOW = MODE != 0, ANY = MODE == 1;
     extern __shared__ int32_t lds_stack[];
     int32_t* stk = lds_stack + threadIdx.x;
@@ -684,12 +686,18 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             const unsigned long long wm = __ballot(walk);
             if (wm == 0) break;
             if (__popcll(wm) <= walk_min && __ballot(p >= 0 && leaf != 0) != 0) break;
-            if (walk) node_visit<ANY>(S, o, d, id, stk, sd, &W.cnt[3], nn, h, node, leaf, sp, nullptr);
+            if (walk) {
+                node_visit<ANY>(S, o, d, id, stk, sd, &W.cnt[3], nn, h, node, leaf, sp, nullptr);
+                // a second visit per pass halves the ballots and branches
+                // (1-2 % on the room and the icospheres)
+                if (leaf == 0 && !ray_done(node, leaf, sp))
+                    node_visit<ANY>(S, o, d, id, stk, sd, &W.cnt[3], nn, h, node, leaf, sp, nullptr);
+            }
         }
         if (p < 0) continue;
         if (leaf != 0) {
-            const bool found = leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, nullptr);
-            leaf = 0;
+            const bool found = leaf_visit<ANY>(S, o, d, tmin, nt, h, leaf, leaf_tris, nullptr);
+            if (found) leaf = 0;
             if (!found && !ray_done(node, leaf, sp)) continue;
         } else if (!ray_done(node, leaf, sp)) {
             continue;
@@ -1357,8 +1365,8 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
 // Grid-stride over the live queue; the trip count is uniform per workgroup so
 // every lane reaches the wavefront ballots of shade_one.
 template <int ALGO, bool NODES>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::GuideDev G, int qin,
-                                                  int32_t max_depth) {
+__device__ __forceinline__ void shade_launch(const SceneDev& S, const WaveDev& W, const guide::GuideDev& G, int qin,
+                                             int32_t max_depth) {
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
     if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
@@ -1370,6 +1378,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
     if (stop && threadIdx.x == 0) W.cnt[7] = 1;
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
         shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n, stop);
+}
+template <int ALGO, bool NODES>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::GuideDev G, int qin,
+                                                  int32_t max_depth) {
+    shade_launch<ALGO, NODES>(S, W, G, qin, max_depth);
 }
 
 template <int ALGO>
@@ -2188,6 +2201,7 @@ struct lucid_ctx {
     int32_t traverse_groups = 16, shade_groups = 8;
     int32_t refill = 48;  // idle lanes that make a wavefront fetch new rays
     int32_t walk_min = 16;  // traversal: node walking stops once at most this many lanes still walk
+    int32_t leaf_tris = 4;  // traversal: triangles of a waiting leaf tested per leaf phase
     // 1: bounce and shadow rays regrouped by ray bin before their traversal
     // passes (depth >= 1); 0: never; 2: for waves of at least 2^24 paths.
     // The 1080p occluded room (33M paths a wave) renders 8 % faster with it
@@ -2634,7 +2648,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     // the launch shape of this render, read once (callers hold ctx->mu)
     const int32_t traverse_groups = ctx->traverse_groups, shade_groups = ctx->shade_groups;
     const int32_t refill_lanes = ctx->refill, ray_order = ctx->ray_order, shade_order = ctx->shade_order;
-    const int32_t walk_min = ctx->walk_min;
+    const int32_t walk_min = ctx->walk_min, leaf_tris = ctx->leaf_tris;
     // traverse_groups > 0: persistent traversal with ray refill on that many
     // workgroups per CU; 0: one lane per queued ray
     const bool refill = traverse_groups > 0;
@@ -2728,7 +2742,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             int ks = kev_begin(ctx, 0, st);
             if (refill)
-                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, refill_lanes, walk_min);
+                k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, Wd, q, sd, refill_lanes, walk_min, leaf_tris);
             else
                 k_intersect<<<g, kBlock, lds, st>>>(SD, Wd, q, sd);
             kev_end(ctx, ks, st);
@@ -2800,9 +2814,9 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             }
             ks = kev_begin(ctx, 2, st);
             if (refill && SD.any_vbo)
-                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min);
+                k_trace_refill<2><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min, leaf_tris);
             else if (refill)
-                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min);
+                k_trace_refill<1><<<g, kBlock, lds, st>>>(SD, Ws, 0, sd, refill_lanes, walk_min, leaf_tris);
             else
                 k_shadow<<<g, kBlock, lds, st>>>(SD, Ws, sd);
             kev_end(ctx, ks, st);
@@ -3307,11 +3321,11 @@ int lucid_render_debug(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
     const unsigned g = (unsigned)std::min<int64_t>(blocks(P, kBlock), (int64_t)ctx->num_cu * 16);
     k_zero_counters<<<1, 64, 0, st>>>(W.cnt, W.rays);
     k_raygen<<<blocks(P, kBlock), kBlock, 0, st>>>(W, cd, (int32_t)P);
-    k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48, ctx->walk_min);
+    k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48, ctx->walk_min, ctx->leaf_tris);
     if (mode == kDebugVolume) {
         k_debug_vol_prep<<<blocks(P, 256), 256, 0, st>>>(SD, W, (int32_t)P);
         k_set_i32<<<1, 64, 0, st>>>(W.cnt + 4, 0);  // fetch counter of the second query
-        k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48, ctx->walk_min);
+        k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48, ctx->walk_min, ctx->leaf_tris);
     }
     k_debug_out<<<blocks(P, 256), 256, 0, st>>>(SD, W, ctx->out_dev, pr->tile_w, pr->tile_h, mode);
     HIP_OK(hipGetLastError());
@@ -3525,8 +3539,10 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     ctx->profiling = 0;
     int32_t so = 0;  // shading order, searched after the ray order
     int32_t wk = ctx->walk_min;  // node-walk threshold, searched after the refill threshold
+    int32_t lt = ctx->leaf_tris;  // triangles per leaf phase, searched after it
     auto measure = [&](int32_t ro, int32_t tg, int32_t rf, int32_t sg, double* ms) {
         ctx->walk_min = wk;
+        ctx->leaf_tris = lt;
         ctx->ray_order = ro;
         ctx->traverse_groups = tg;
         ctx->refill = rf;
@@ -3572,6 +3588,16 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
             if (!(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; best_wk = v; }
         }
         wk = best_wk;
+        const int32_t lts[] = {1, 2, 4};
+        const int32_t first_lt = lt;
+        int32_t best_lt = lt;
+        for (int32_t v : lts) {
+            if (rc || v == first_lt) continue;
+            lt = v;
+            double t = 0.0;
+            if (!(rc = measure(ro, tg, rf, sg, &t)) && t < t_best) { t_best = t; best_lt = v; }
+        }
+        lt = best_lt;
     }
     for (int32_t v : sgs) {
         double t = 0.0;
@@ -3592,6 +3618,7 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     }
     ctx->wave_paths = wp;
     ctx->walk_min = wk;
+    ctx->leaf_tris = lt;
     ctx->ray_order = ro;
     ctx->traverse_groups = tg;
     ctx->refill = rf;
@@ -3608,13 +3635,15 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     best->ms = t_best;
     best->wave_paths = wp;
     best->walk_min = wk;
+    best->leaf_tris = lt;
     return rc;
 }
 
-int lucid_set_walk_threshold(lucid_ctx* ctx, int32_t lanes) {
-    if (!ctx || lanes < 0 || lanes > 63) return LUCID_ERR_INVALID;
+int lucid_set_traversal_shape(lucid_ctx* ctx, int32_t walk_lanes, int32_t leaf_tris) {
+    if (!ctx || walk_lanes < 0 || walk_lanes > 63 || leaf_tris < 1 || leaf_tris > 4) return LUCID_ERR_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->walk_min = lanes;
+    ctx->walk_min = walk_lanes;
+    ctx->leaf_tris = leaf_tris;
     return LUCID_OK;
 }
 
@@ -3872,7 +3901,7 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     const int sd = std::min(kStackMax, scene->host.max_depth + 2);
     const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
     k_trace_refill<0><<<g, kBlock, (size_t)sd * kBlock * sizeof(int32_t), st>>>(scene->dev, W, 1, sd, 48,
-                                                                                  ctx->walk_min);
+                                                                                  ctx->walk_min, ctx->leaf_tris);
     if (m > 0) HIP_OK(hipMemcpyAsync(W.q[0], order, (size_t)m * 4, hipMemcpyHostToDevice, st));
     k_set_i32<<<1, 64, 0, st>>>(W.cnt, m);
     // outputs start as NaN: entries the kernel must not write stay NaN
@@ -3963,7 +3992,7 @@ struct PassKey {
     lucid_camera cam;
     const float4* accum;
     int32_t w, h, max_depth, algorithm, spp, active_lanes;
-    int32_t traverse_groups, refill, shade_groups, ray_order, shade_order, num_cu, walk_min;
+    int32_t traverse_groups, refill, shade_groups, ray_order, shade_order, num_cu, walk_min, leaf_tris;
 };
 std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp, int active_lanes) {
     PassKey k;
@@ -3976,6 +4005,7 @@ std::vector<uint8_t> pass_key(const lucid_session* s, const Lane& L, int32_t spp
     k.traverse_groups = ctx->traverse_groups; k.refill = ctx->refill; k.shade_groups = ctx->shade_groups;
     k.ray_order = ctx->ray_order; k.shade_order = ctx->shade_order; k.num_cu = ctx->num_cu;
     k.walk_min = ctx->walk_min;
+    k.leaf_tris = ctx->leaf_tris;
     const uint8_t* b = reinterpret_cast<const uint8_t*>(&k);
     return std::vector<uint8_t>(b, b + sizeof(k));
 }

@@ -617,6 +617,7 @@ public:
         d["shade_order"] = b.shade_order;
         d["wave_paths"] = b.wave_paths;
         d["walk_min"] = b.walk_min;
+        d["leaf_tris"] = b.leaf_tris;
         d["ms"] = b.ms;
         return d;
     }

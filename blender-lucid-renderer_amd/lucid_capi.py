@@ -105,7 +105,7 @@ class GuidingStats(C.Structure):
 class LaunchShape(C.Structure):
     _fields_ = [("traverse_groups", C.c_int32), ("refill", C.c_int32), ("shade_groups", C.c_int32),
                 ("ray_order", C.c_int32), ("shade_order", C.c_int32), ("ms", C.c_double),
-                ("wave_paths", C.c_int64), ("walk_min", C.c_int32), ("pad", C.c_int32)]
+                ("wave_paths", C.c_int64), ("walk_min", C.c_int32), ("leaf_tris", C.c_int32)]
 
 
 SIGNATURES = {
@@ -167,7 +167,7 @@ SIGNATURES = {
     "lucid_set_grid_format": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_scene_grid_bytes": (C.c_int64, [C.c_void_p]),
     "lucid_set_scene_build": (C.c_int, [C.c_void_p, C.c_int32]),
-    "lucid_set_walk_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_set_traversal_shape": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "lucid_scene_read_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
                                             C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                             C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

@@ -306,8 +306,8 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
    by coordinate descent -- the ray order (both values), traversal groups
    per CU in {8, 12, 16, 24}, then
    the refill threshold in {32, 48, 60}, then the node-walk threshold in
-   {0, 8, 16, 32}, then shading groups per CU in {4, 8, 16}, then the
-   wavefront batch (one wave, or the pass split into 2
+   {0, 8, 16, 32} and the triangles per leaf phase in {1, 2, 4}, then
+   shading groups per CU in {4, 8, 16}, then the wavefront batch (one wave, or the pass split into 2
    or 4 waves of whole samples, each of at least 2^20 paths:
    lucid_set_wave_paths), two renders each -- and keeps the fastest on the
    context.  Launch shapes do not change results.  Not with diagnostics
@@ -318,8 +318,8 @@ typedef struct lucid_launch_shape {
     int32_t shade_order;  /* lucid_set_shade_order: 0 or 1 */
     double ms;          /* device time of the chosen shape's render */
     int64_t wave_paths; /* lucid_set_wave_paths: 0 automatic, or paths per batch */
-    int32_t walk_min;   /* lucid_set_walk_threshold */
-    int32_t pad;
+    int32_t walk_min;   /* lucid_set_traversal_shape: walk_lanes */
+    int32_t leaf_tris;  /* lucid_set_traversal_shape: leaf_tris */
 } lucid_launch_shape;
 int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, const lucid_render_params* params,
                    lucid_launch_shape* best);
@@ -349,10 +349,11 @@ int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on);
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order);
 
 /* Persistent traversal (no counterpart in the reference): a wavefront walks
-   BVH nodes while more than `lanes` of its lanes still look for a leaf, then
-   tests the leaves its lanes hold together (0..63; default 16).  Results do
-   not depend on it. */
-int lucid_set_walk_threshold(lucid_ctx* ctx, int32_t lanes);
+   BVH nodes while more than `walk_lanes` of its lanes still look for a leaf
+   (0..63; default 16), then tests up to `leaf_tris` triangles (1..4; default
+   4) of the leaf each lane holds; a lane keeps the rest of its leaf for the
+   next leaf phase.  Results do not depend on either. */
+int lucid_set_traversal_shape(lucid_ctx* ctx, int32_t walk_lanes, int32_t leaf_tris);
 /* Persistent traversal: a wavefront fetches new rays once this many of its
    64 lanes are idle (1..64, default 48).  A tuning knob, no counterpart in
    the reference. */
