@@ -111,13 +111,18 @@ struct WaveDev {
     int32_t tstride;
     int64_t tnrec;  // records (paths with slots); slot arrays are [k * 5 + field][record]
     // render_tile: the context's cancel word in host memory the device reads
-    // (null elsewhere).  Set, it ends every path at its next shading; the
-    // film then skips the wave (cnt[7] latches what each wave saw).
+    // (null elsewhere).  One thread of each bounce's closest-hit launch reads
+    // it and latches a set word in cnt[7] (device memory, cleared at the
+    // render's start, never cleared within it).  Shading ends every path of
+    // a latched wave and the film skips it.  Every other kernel reads only
+    // the latch, written by an earlier launch of the same stream, so all of
+    // them agree: a path ends early only if the film drops its wave.
     const int32_t* stop;
 };
-__device__ __forceinline__ bool stop_requested(const WaveDev& W) {
-    return W.stop && *reinterpret_cast<const volatile int32_t*>(W.stop) != 0;
+__device__ __forceinline__ void poll_stop(const WaveDev& W) {
+    if (W.stop && *reinterpret_cast<const volatile int32_t*>(W.stop) != 0) W.cnt[7] = 1;
 }
+__device__ __forceinline__ bool stop_latched(const WaveDev& W) { return W.stop && W.cnt[7] != 0; }
 constexpr int kTrainSlots = 6;
 // Paths that keep training slots: one per block of `tstride` consecutive
 // paths, at a hashed offset so that every pixel column contributes.
@@ -571,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int
         W.cnt[1 - qin] = 0;
         W.cnt[2] = 0;
         atomicAdd(&W.rays[0], (unsigned long long)n);
+        poll_stop(W);
     }
     uint32_t nn = 0, nt = 0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -622,6 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             W.cnt[1 - qin] = 0;
             W.cnt[2] = 0;
             atomicAdd(&W.rays[0], (unsigned long long)n);
+            poll_stop(W);
         }
     }
     uint32_t nn = 0, nt = 0;
@@ -1370,12 +1377,10 @@ __device__ __forceinline__ void shade_launch(const SceneDev& S, const WaveDev& W
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
     if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
-    // cancel requests are polled once per launch, i.e. between bounces;
-    // wavefront-uniform, so the ballots in shade_one stay convergent
-    const bool stop = __builtin_amdgcn_readfirstlane(stop_requested(W) ? 1 : 0) != 0;
-    // a wave whose paths this launch ends can never reach the film: latch it
-    // here, not only from the read after the wave's last launch
-    if (stop && threadIdx.x == 0) W.cnt[7] = 1;
+    // the cancel latch of this bounce (set by its closest-hit launch, not
+    // written during this one): uniform over the launch, so the ballots in
+    // shade_one stay convergent and every wavefront takes the same decision
+    const bool stop = stop_latched(W);
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock)
         shade_one<ALGO, NODES>(S, W, G, qin, max_depth, base + (int)threadIdx.x, n, stop);
 }
@@ -1610,18 +1615,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, WaveDev W, int sd
 }
 
 // ------------------------------------------------------------------ film --
-// Latches the cancel word once per wave, after its last shading launch:
-// cnt[7] = 1 once a cancel has been seen, here or by a shading launch that
-// ended paths because of it (which sets the latch itself: its read and this
-// one may run on different XCDs).  Every later kernel of the render reads
-// the latch, so all of them agree on one value.
-__global__ __launch_bounds__(64) void k_latch_stop(WaveDev W) {
-    if (threadIdx.x == 0 && stop_requested(W)) W.cnt[7] = 1;
-}
 __global__ __launch_bounds__(256) void k_film(WaveDev W, float4* film, int32_t P, int32_t B) {
     int pix = blockIdx.x * 256 + threadIdx.x;
     if (pix >= P) return;
-    if (W.stop && W.cnt[7]) return;  // a cancelled wave: the film keeps the complete waves only
+    // a latched wave may hold paths that shading ended early: the film keeps
+    // the waves that finished before the latch only
+    if (stop_latched(W)) return;
     float4 a = film[pix];
     for (int b = 0; b < B; ++b) {
         float4 r = W.rad[(size_t)b * P + pix];
@@ -2830,7 +2829,6 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             kev_end(ctx, ks, st);
         }
         if (mode == 0 || mode == 3) {
-            if (W.stop) k_latch_stop<<<1, 64, 0, st>>>(W);
             k_film<<<blocks(P, 256), 256, 0, st>>>(W, film, (int32_t)P, b);
             if (diag && dsw > 0 && dsh > 0)
                 k_diag_film<<<blocks((int64_t)dsw * dsh, 64), 64, 0, st>>>(
