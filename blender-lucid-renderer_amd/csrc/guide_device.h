@@ -33,13 +33,18 @@ __device__ __forceinline__ float feature(int idx, const float pn[3], v3f n, v3f 
     return dot(n, wo);
 }
 
-// All 64 features at once: sin / cos (2^k pi pn_c) from one sincos per
-// coordinate and the double-angle recurrence (the values feature() defines,
-// to a few float ulps).
+// sin(pi x) and cos(pi x) by the hardware's v_sin_f32 / v_cos_f32 (their
+// argument is in revolutions; absolute error about 1e-6 for |x| <= 2, far
+// below the BF16 rounding the network inputs take).
+__device__ __forceinline__ float hw_sinpi(float x) { return __builtin_amdgcn_sinf(0.5f * x); }
+__device__ __forceinline__ float hw_cospi(float x) { return __builtin_amdgcn_cosf(0.5f * x); }
+
+// All 64 features at once: sin / cos (2^k pi pn_c) from one hardware sine
+// and cosine per coordinate and the double-angle recurrence (the values
+// feature() defines, to about 1e-5 absolute).
 __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float f[kIn]) {
     for (int c = 0; c < 3; ++c) {
-        float sn, cs;
-        sincospif(pn[c], &sn, &cs);
+        float sn = hw_sinpi(pn[c]), cs = hw_cospi(pn[c]);
         for (int k = 0; k < kOctaves; ++k) {
             f[3 * (2 * k) + c] = sn;
             f[3 * (2 * k + 1) + c] = cs;
@@ -51,8 +56,8 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
     f[48] = 2.0f * pn[0] - 1.0f; f[49] = 2.0f * pn[1] - 1.0f; f[50] = 2.0f * pn[2] - 1.0f;
     f[51] = n.x; f[52] = n.y; f[53] = n.z;
     f[54] = wo.x; f[55] = wo.y; f[56] = wo.z;
-    f[57] = sinpif(n.x); f[58] = sinpif(n.y); f[59] = sinpif(n.z);
-    f[60] = sinpif(wo.x); f[61] = sinpif(wo.y); f[62] = sinpif(wo.z);
+    f[57] = hw_sinpi(n.x); f[58] = hw_sinpi(n.y); f[59] = hw_sinpi(n.z);
+    f[60] = hw_sinpi(wo.x); f[61] = hw_sinpi(wo.y); f[62] = hw_sinpi(wo.z);
     f[63] = dot(n, wo);
 }
 
@@ -61,8 +66,7 @@ __device__ __forceinline__ void features(const float pn[3], v3f n, v3f wo, float
 __device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, __bf16* row) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float sn, cs;
-        sincospif(pn[c], &sn, &cs);
+        float sn = hw_sinpi(pn[c]), cs = hw_cospi(pn[c]);
 #pragma unroll
         for (int k = 0; k < kOctaves; ++k) {
             row[3 * (2 * k) + c] = (__bf16)sn;
@@ -76,8 +80,8 @@ __device__ __forceinline__ void features_bf16(const float pn[3], v3f n, v3f wo, 
     row[50] = (__bf16)(2.0f * pn[2] - 1.0f);
     row[51] = (__bf16)n.x; row[52] = (__bf16)n.y; row[53] = (__bf16)n.z;
     row[54] = (__bf16)wo.x; row[55] = (__bf16)wo.y; row[56] = (__bf16)wo.z;
-    row[57] = (__bf16)sinpif(n.x); row[58] = (__bf16)sinpif(n.y); row[59] = (__bf16)sinpif(n.z);
-    row[60] = (__bf16)sinpif(wo.x); row[61] = (__bf16)sinpif(wo.y); row[62] = (__bf16)sinpif(wo.z);
+    row[57] = (__bf16)hw_sinpi(n.x); row[58] = (__bf16)hw_sinpi(n.y); row[59] = (__bf16)hw_sinpi(n.z);
+    row[60] = (__bf16)hw_sinpi(wo.x); row[61] = (__bf16)hw_sinpi(wo.y); row[62] = (__bf16)hw_sinpi(wo.z);
     row[63] = (__bf16)dot(n, wo);
 }
 
@@ -167,6 +171,138 @@ __device__ __forceinline__ v3f mixture_sample(const Mixture& m, float u0, float 
     const v3f t = mk(1.0f + s * mu.x * mu.x * a, s * b, -s * mu.x);
     const v3f bb = mk(b, s + mu.y * mu.y * a, -mu.y);
     return t * (r * cosf(phi)) + bb * (r * sinf(phi)) + mu * wz;
+}
+
+// ---- render side: the FP16 outputs of one path, decoded where they are
+// used.  A guided vertex keeps the 44 raw outputs packed in 22 registers
+// (instead of 48 decoded floats), takes the lobe weights in one pass,
+// decodes only the chosen lobe to sample it, and evaluates the density at
+// both of its directions (the NEE direction and the sampled one) in one
+// more pass.  The transcendentals are the hardware's (v_exp_f32,
+// v_log_f32, v_sqrt_f32, v_rcp_f32; about 1 ulp): the guide is this
+// build's own, and sampling and density read the same decoded values, so
+// the density stays the one the sampler draws from up to float rounding.
+struct GuideOut {
+    uint32_t h[kGoutStride / 2];  // outputs 2i (low half) and 2i + 1 (high half)
+};
+constexpr float kLog2E = 1.4426950408889634f, kLn2 = 0.6931471805599453f;
+constexpr float kLog2TwoPi = 2.6514961294723187f;  // log2(2 pi)
+
+__device__ __forceinline__ GuideOut load_out(const __half* go) {
+    GuideOut g;
+    const uint2* p = reinterpret_cast<const uint2*>(go);  // rows are 88 bytes: 8-byte aligned
+#pragma unroll
+    for (int i = 0; i < kGoutStride / 4; ++i) {
+        const uint2 v = p[i];
+        g.h[2 * i] = v.x;
+        g.h[2 * i + 1] = v.y;
+    }
+    return g;
+}
+__device__ __forceinline__ float out_at(const GuideOut& g, int i) {
+    const uint32_t w = g.h[i >> 1];
+    const unsigned short b = (unsigned short)((i & 1) ? (w >> 16) : (w & 0xffffu));
+    return __half2float(__ushort_as_half(b));
+}
+// false if any of the trained outputs is not finite (exponent all ones):
+// such a vertex is left to plain sampling, as decode() leaves it
+__device__ __forceinline__ bool out_finite(const GuideOut& g) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < kOutTrained; ++i) {
+        const uint32_t b = (i & 1) ? (g.h[i >> 1] >> 16) : g.h[i >> 1];
+        ok = ok && (b & 0x7c00u) != 0x7c00u;
+    }
+    return ok;
+}
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// softmax of the weight logits: w_k = exp2((o_k - mx) log2 e) * inv_sum
+struct LobeWeights {
+    float mx, inv_sum, log2_sum;
+};
+__device__ __forceinline__ LobeWeights lobe_weights(const GuideOut& g) {
+    LobeWeights lw;
+    float mx = -1e30f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) mx = fmaxf(mx, out_at(g, 5 * k + 4));
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) sum += fast_exp2((out_at(g, 5 * k + 4) - mx) * kLog2E);
+    lw.mx = mx;
+    lw.inv_sum = fast_rcp(sum);
+    lw.log2_sum = fast_log2(sum);
+    return lw;
+}
+// mean direction r / (|r| + 1e-6) and concentration exp(clamp(o, -2, 7)) of
+// a lobe from its raw outputs (decode()'s mapping)
+__device__ __forceinline__ v3f lobe_mu(float rx, float ry, float rz) {
+    const float il = fast_rcp(fast_sqrt(rx * rx + ry * ry + rz * rz) + 1e-6f);
+    return mk(rx * il, ry * il, rz * il);
+}
+__device__ __forceinline__ float lobe_log_kappa(float o) { return fminf(fmaxf(o, kLogKappaMin), kLogKappaMax); }
+
+// The mixture density at directions a and b (one pass over the lobes):
+// p = sum_k exp2(log2(w_k C(kappa_k)) + kappa_k log2 e (mu_k . x - 1)),
+// C(kappa) = kappa / (2 pi (1 - exp(-2 kappa))).
+__device__ __forceinline__ void mixture_pdf2(const GuideOut& g, const LobeWeights& lw, v3f a, v3f b, float& pa,
+                                             float& pb) {
+    pa = 0.0f;
+    pb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kLobes; ++k) {
+        const v3f mu = lobe_mu(out_at(g, 5 * k), out_at(g, 5 * k + 1), out_at(g, 5 * k + 2));
+        const float lk = lobe_log_kappa(out_at(g, 5 * k + 3));
+        const float ka = fast_exp2(lk * kLog2E), k2 = ka * kLog2E;
+        const float logc2 = lk * kLog2E - kLog2TwoPi - fast_log2(1.0f - fast_exp2(-2.0f * k2));
+        const float lw2 = logc2 + (out_at(g, 5 * k + 4) - lw.mx) * kLog2E - lw.log2_sum;
+        pa += fast_exp2(lw2 + k2 * (dot(mu, a) - 1.0f));
+        pb += fast_exp2(lw2 + k2 * (dot(mu, b) - 1.0f));
+    }
+}
+
+// Draws a direction: the lobe with u0 (the first whose running weight sum
+// exceeds u0, the last if none does), then the lobe's cosine with u1 by
+// inversion, w = 1 + log(u1 + (1 - u1) exp(-2 kappa)) / kappa, and its
+// azimuth 2 pi u2.
+__device__ __forceinline__ v3f mixture_sample_out(const GuideOut& g, const LobeWeights& lw, float u0, float u1,
+                                                  float u2) {
+    float c = 0.0f;
+    float rx = out_at(g, 5 * (kLobes - 1)), ry = out_at(g, 5 * (kLobes - 1) + 1);
+    float rz = out_at(g, 5 * (kLobes - 1) + 2), olk = out_at(g, 5 * (kLobes - 1) + 3);
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < kLobes - 1; ++i) {
+        c += fast_exp2((out_at(g, 5 * i + 4) - lw.mx) * kLog2E) * lw.inv_sum;
+        const bool take = !found && u0 < c;
+        rx = take ? out_at(g, 5 * i) : rx;
+        ry = take ? out_at(g, 5 * i + 1) : ry;
+        rz = take ? out_at(g, 5 * i + 2) : rz;
+        olk = take ? out_at(g, 5 * i + 3) : olk;
+        found = found || take;
+    }
+    const v3f mu = lobe_mu(rx, ry, rz);
+    const float k2 = fast_exp2(lobe_log_kappa(olk) * kLog2E) * kLog2E;  // kappa log2 e
+    // 1 + log(x) / kappa = 1 + log2(x) / (kappa log2 e)
+    const float wz = fminf(fmaxf(1.0f + fast_log2(u1 + (1.0f - u1) * fast_exp2(-2.0f * k2)) * fast_rcp(k2), -1.0f),
+                           1.0f);
+    const float r = fast_sqrt(fmaxf(0.0f, 1.0f - wz * wz));
+    // v_cos_f32 / v_sin_f32 take their argument in revolutions
+    const float cp = __builtin_amdgcn_cosf(u2), sp = __builtin_amdgcn_sinf(u2);
+    const float s = copysignf(1.0f, mu.z);
+    const float a = -1.0f / (s + mu.z), bb0 = mu.x * mu.y * a;
+    const v3f t = mk(1.0f + s * mu.x * mu.x * a, s * bb0, -s * mu.x);
+    const v3f bb = mk(bb0, s + mu.y * mu.y * a, -mu.y);
+    return t * (r * cp) + bb * (r * sp) + mu * wz;
+}
+// the guide-selection probability from the packed outputs (selection())
+__device__ __forceinline__ float selection_out(const GuideOut& g, float alpha, int learn_sel) {
+    if (!learn_sel) return alpha;
+    const float a = 1.0f / (1.0f + expf(-out_at(g, kSel)));
+    return isfinite(a) ? fminf(fmaxf(a, kSelMin), kSelMax) : alpha;
 }
 
 }  // namespace guide

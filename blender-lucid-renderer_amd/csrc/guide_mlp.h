@@ -41,6 +41,29 @@ __device__ __forceinline__ bf16x8 pack_b(f32x4 lo, f32x4 hi) {
     return r;
 }
 
+// ReLU, then the B operand of the next layer, on the packed BF16 values: a
+// BF16 has the sign in its top bit, so max(x, 0) as signed 16-bit integers
+// is the ReLU (v_pk_max_i16, two values per instruction; -0 becomes +0).
+// Rounding commutes with it, so this equals pack_b of the ReLU'd floats.
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef short s16x2 __attribute__((ext_vector_type(2)));
+    const f32x2 f = {a, b};
+    const bf16x2 p = __builtin_convertvector(f, bf16x2);  // v_cvt_pk_bf16_f32
+    s16x2 v;
+    __builtin_memcpy(&v, &p, 4);
+    v = __builtin_elementwise_max(v, (s16x2)0);
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    return u;
+}
+__device__ __forceinline__ bf16x8 pack_b_relu(f32x4 lo, f32x4 hi) {
+    const uint4 u = {relu_bf16x2(lo[0], lo[1]), relu_bf16x2(lo[2], lo[3]), relu_bf16x2(hi[0], hi[1]),
+                     relu_bf16x2(hi[2], hi[3])};
+    return as_bf16x8(u);
+}
+
 // Forward pass.  `x` holds the lane's input operand for K blocks 0 and 1
 // (features perm(kb, g, i) of its sample).  Outputs: the three output tiles
 // (features 16m + 4g + j).  When `store` is given, the post-ReLU hidden
@@ -112,10 +135,8 @@ __device__ __forceinline__ void mlp_forward_tiles_at(const uint4* fwd, BiasOf bi
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             if (l < kLayers - 1) {
-                for (int m = 0; m < 4; ++m)
-                    for (int j = 0; j < 4; ++j) acc[t][m][j] = fmaxf(acc[t][m][j], 0.0f);
-                b0[t] = pack_b(acc[t][0], acc[t][1]);
-                b1[t] = pack_b(acc[t][2], acc[t][3]);
+                b0[t] = pack_b_relu(acc[t][0], acc[t][1]);
+                b1[t] = pack_b_relu(acc[t][2], acc[t][3]);
             } else {
                 out[t][0] = acc[t][0];
                 out[t][1] = acc[t][1];

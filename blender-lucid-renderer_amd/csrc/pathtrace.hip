@@ -963,6 +963,16 @@ __device__ __forceinline__ void put_record(const WaveDev& W, int p, int slot, v3
     W.dr1[k] = make_float4(lpos.x, lpos.y, lpos.z, __int_as_float(light_obj));
 }
 
+// The NEE contribution of a guided vertex once the guide's density at the
+// light sample's direction is known: c = bsdf_weight * emission, weighed by
+// the power heuristic of the light pdf pl against the density q of the
+// vertex's hemisphere technique, times the throughput.
+__device__ __forceinline__ void finish_guided_nee(const WaveDev& W, int p, float q, float pl, v3f c, v3f thr) {
+    const float mw = power_heuristic(pl, q);
+    const v3f cc = cmul(c * mw, thr);
+    W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+}
+
 // One bounce of traceSimple / traceNEE / traceMIS (path_tracer.hpp:128-570),
 // their volume variants (:209-283, 586-835), or the diagnostic tracers.
 // NODES: the scene has node-tree programs (separate instantiations keep
@@ -1164,15 +1174,24 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             // evaluated f / pdf pair there)
             const bool guidable = GUIDED && mat.transmission == 0.0f && !(mat.roughness < 0.05f && mat.metallic > 0.5f);
             bool guided = guidable && G.alpha > 0.0f;
-            guide::Mixture gmx;
+            // the vertex's network outputs, packed (decoded where used)
+            guide::GuideOut gout = {};
+            guide::LobeWeights glw = {0.0f, 0.0f, 0.0f};
             float ga = G.alpha;  // guide-selection probability of this vertex
             // a degenerate learned density (non-finite outputs) leaves this
             // vertex to plain BSDF sampling and NEE
             if (GUIDED && guided) {
-                const __half* go = W.gout + (size_t)p * guide::kGoutStride;
-                guided = guide::decode(go, gmx);
-                ga = guide::selection(go, G.alpha, G.learn_sel);
+                gout = guide::load_out(W.gout + (size_t)p * guide::kGoutStride);
+                guided = guide::out_finite(gout);
+                glw = guide::lobe_weights(gout);
+                ga = guide::selection_out(gout, G.alpha, G.learn_sel);
             }
+            // guided vertex: the NEE sample's MIS weight needs the guide's
+            // density at its direction, which the density pass after the
+            // BSDF sampling evaluates together with the sampled direction's
+            bool nee_wait = false;
+            v3f nee_dir = mk(0.0f, 0.0f, 1.0f), nee_c = mk(0.0f, 0.0f, 0.0f);
+            float nee_pl = 0.0f, nee_pt = 0.0f;
             if (SIMPLE) {
                 if (DIAG) {
                     if (has_em) {
@@ -1230,13 +1249,18 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                             v3f f = eval_bsdf(mat, wo, ls.dir, sn);
                             float pl = ls.pdf * sp;
                             v3f cc;
-                            if (A == LUCID_ALGO_MIS) {
-                                float pb = pdf_bsdf(mat, wo, ls.dir, sn);
+                            if (A == LUCID_ALGO_MIS && GUIDED && guided) {
                                 // guided bounce: weighted against the density of its
-                                // hemisphere technique (see the BSDF sampling below)
-                                if (GUIDED && guided)
-                                    pb = ga * guide::mixture_pdf(gmx, ls.dir) +
-                                         (1.0f - ga) * pdf_bsdf_sampled(mat, wo, ls.dir, sn);
+                                // hemisphere technique (see the BSDF sampling below);
+                                // the contribution is written after the density pass
+                                nee_wait = true;
+                                nee_dir = ls.dir;
+                                nee_pl = pl;
+                                nee_pt = pdf_bsdf_sampled(mat, wo, ls.dir, sn);
+                                nee_c = cmul(bsdf_weight(f, nl, pl), ls.emission);
+                                cc = mk(0.0f, 0.0f, 0.0f);
+                            } else if (A == LUCID_ALGO_MIS) {
+                                float pb = pdf_bsdf(mat, wo, ls.dir, sn);
                                 float mw = power_heuristic(pl, pb);
                                 cc = cmul(bsdf_weight(f, nl, pl) * mw, ls.emission);
                             } else if (A == LUCID_ALGO_VOLUME_MIS) {
@@ -1264,7 +1288,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                             }
                             W.sh_o[p] = make_float4(point.x, point.y, point.z, ls.dist);
                             W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tag);
-                            W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
+                            if (!(GUIDED && nee_wait)) W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
                             nee = true;
                         }
                     }
@@ -1295,7 +1319,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 // of the throughput's f_eff: rho w'_b + (1 - rho) w_b, rho = pb / pt.
                 if (randf(rng) < ga) {
                     const float g0 = randf(rng), g1 = randf(rng), g2 = randf(rng);
-                    bs.wi = guide::mixture_sample(gmx, g0, g1, g2);
+                    bs.wi = guide::mixture_sample_out(gout, glw, g0, g1, g2);
                     bs.kind = 0;
                     bs.type = 0;
                     bs.fw = eval_bsdf(mat, wo, bs.wi, smp_n);
@@ -1306,7 +1330,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 }
                 const float pt = pdf_bsdf_sampled(mat, wo, bs.wi, smp_n);
                 pt_wi = pt;
-                const float q = ga * guide::mixture_pdf(gmx, bs.wi) + (1.0f - ga) * pt;
+                float qg = 0.0f, qg_nee = 0.0f;
+                guide::mixture_pdf2(gout, glw, bs.wi, nee_dir, qg, qg_nee);
+                if (nee_wait) {
+                    nee_wait = false;
+                    finish_guided_nee(W, p, ga * qg_nee + (1.0f - ga) * nee_pt, nee_pl, nee_c, thr);
+                }
+                const float q = ga * qg + (1.0f - ga) * pt;
                 pw = (pt > 0.0f && q > 0.0f) ? q * (bs.pdf / pt) : 0.0f;
                 mis_pdf = q;
                 next_1m_rho = pt > 0.0f ? 1.0f - bs.pdf / pt : 0.0f;
@@ -1356,6 +1386,11 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                 W.meta[p] = make_int4(meta.x, meta.y, depth + 1, GUIDED && bs.kind == 0 ? __float_as_int(next_1m_rho) : 0);
             }
             }
+            if (GUIDED && nee_wait) {
+                float qg_nee = 0.0f, unused = 0.0f;
+                guide::mixture_pdf2(gout, glw, nee_dir, nee_dir, qg_nee, unused);
+                finish_guided_nee(W, p, ga * qg_nee + (1.0f - ga) * nee_pt, nee_pl, nee_c, thr);
+            }
         }
         W.rad[p] = make_float4(L.x, L.y, L.z, 0.0f);
         if (DIAG) W.dcnt[p] = nvert | ((depth + 1) << 8);
@@ -1399,6 +1434,13 @@ void launch_shade(unsigned gs, hipStream_t st, const SceneDev& S, const WaveDev&
         k_shade<ALGO, false><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
 }
 
+// Shading workgroups per CU of the guided passes: the guided shading kernel
+// holds about 170 VGPRs, so two of its workgroups fit a CU at once, and it
+// runs best with twice that many (measured on the room at 1080p with a
+// trained network: 60.9 ms per pass at 4 groups per CU, 61.6 at 3, 69.3 at 2,
+// 70.1 at 8).
+constexpr int kGuidedShadeGroups = 4;
+
 // Surface point and shading normal of a closest hit (as the shade kernel
 // computes them).
 __device__ __forceinline__ void hit_frame(const SceneDev& S, v3f o, v3f d, float4 hv, v3f& point, v3f& nrm) {
@@ -1434,6 +1476,17 @@ __global__ __launch_bounds__(256) void k_debug_frame(SceneDev S, WaveDev W, int3
     W.sh_d[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
+// release/acquire fence on all memory, so it would wait for this
+// wavefront's outstanding global loads too, and with them for the loads
+// issued for later trips; fences restricted to the "local" address space
+// wait for the LDS counter alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 // Wavefronts per workgroup: they share one LDS copy of the weights (31 KB);
 // with their feature rows a workgroup takes 65 KB, two per CU.
@@ -1444,51 +1497,94 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
     constexpr int kRow = guide::kIn + 8;  // bf16 per row: 144 bytes
     __shared__ __attribute__((aligned(16))) __bf16 feats[kInferWaves][64][kRow];
     __shared__ int32_t paths[kInferWaves][64];
+    const int n = W.cnt[qin];
+    if (n <= 0) return;  // uniform: every path of the wave has ended
     guide::stage_weights(G, wfwd, wbias);
     __syncthreads();
-    const int n = W.cnt[qin];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, s = lane & 15;
     __bf16 (*feat)[kRow] = feats[wave];
     int32_t* path = paths[wave];
     const auto bias_of = [&](int l) { return wbias + guide::kBiasStride * l; };
-    // Loads are software-pipelined over the trip loop: the queue entry two
-    // trips ahead and the hit and ray of the next trip are issued before this
-    // trip's MLP, so their latency overlaps the matrix work; only the hit's
-    // normal gathers stay on the critical path.
+    // Loads are software-pipelined over the trip loop, so that a trip's
+    // feature encoding waits on no memory: before each trip's MLP, whose
+    // matrix work then covers their latency, the wavefront issues the queue
+    // entries two trips ahead, the hits and rays of the next trip but one,
+    // and the triangle normals of the next trip (whose hits arrived a trip
+    // ago).  No register of a load issued in this trip is copied before the
+    // next trip (a copy would wait for the load), and the barriers order
+    // LDS accesses only (lds_barrier).
     const int stride = gridDim.x * kInferWaves * 64, lane0 = wave * 64 + lane;
-    const auto load_q = [&](int b0) { return b0 + lane0 < n ? W.q[qin][b0 + lane0] : -1; };
-    // Branch-free, and only the words that are used: a load whose register
-    // is dead or copied at once would be waited for right away.  Slot -1
-    // reads path 0, which every launched wave has.
+    // Every load below is issued unconditionally, from a valid address, and
+    // its register is not read until it is consumed a trip later: a load
+    // under a branch, or a select on a loaded register, would make the
+    // compiler wait for it (and for every load issued after it) at once.
+    // The queue entry of slot b0 + lane0 (the last entry past the end; the
+    // slot's validity is applied when the entry is consumed).
+    const auto load_q = [&](int b0) { return W.q[qin][min(b0 + lane0, n - 1)]; };
     struct PathIn {
-        int pp;
+        int pp;  // path, or -1 for a slot past the end of the queue
         float4 hv;
         v3f o, d;
     };
-    const auto load_path = [&](int pp) {
-        const int i = pp >= 0 ? pp : 0;
+    const auto load_path = [&](int raw, int b0) {
+        PathIn r;
+        r.pp = b0 + lane0 < n ? raw : -1;
+        const int i = r.pp >= 0 ? r.pp : 0;
         const float* po = reinterpret_cast<const float*>(W.ray_o + i);
         const float* pd = reinterpret_cast<const float*>(W.ray_d + i);
-        PathIn r;
-        r.pp = pp;
         r.hv = W.hit[i];
         r.o = mk(po[0], po[1], po[2]);
         r.d = mk(pd[0], pd[1], pd[2]);
         return r;
     };
+    // the hit triangle's face normal (w: smooth flag) and vertex normals; a
+    // path without a surface hit reads path 0's hit record instead (the
+    // words are not used)
+    struct Frame {
+        float4 fn, n0, n1, n2;
+    };
+    const auto load_frame = [&](const PathIn& c) {
+        const int32_t prim = __float_as_int(c.hv.y);
+        const bool hit = c.pp >= 0 && prim >= 0;
+        const bool smooth = hit && S.vn != nullptr;
+        const float4* fs = hit ? S.fn + prim : W.hit;
+        const float4* vs = smooth ? S.vn + 3 * (size_t)prim : W.hit;
+        const int k = smooth ? 1 : 0;
+        Frame f;
+        f.fn = fs[0];
+        f.n0 = vs[0];
+        f.n1 = vs[k];
+        f.n2 = vs[2 * k];
+        return f;
+    };
     int base0 = blockIdx.x * kInferWaves * 64;
-    PathIn cur = load_path(load_q(base0));
-    int q_next = load_q(base0 + stride);
-    // trip counts are uniform over the workgroup (its waves take 64 paths each)
-    for (; base0 < n; base0 += stride) {
+    // Two hit / ray buffers take turns: a trip encodes the features of
+    // `cur` (its normals in `fr`), then loads the normals of `nxt` into `fr`
+    // and the hits and rays of the trip after next into `cur`, whose words
+    // it has consumed.  The loop is unrolled by two so that the buffers
+    // swap roles without register copies.
+    PathIn pa = load_path(load_q(base0), base0);
+    Frame fr = load_frame(pa);
+    PathIn pb = load_path(load_q(base0 + stride), base0 + stride);
+    int q_next = load_q(base0 + 2 * stride);
+    // drain the prologue's loads: the loop is then entered with none
+    // pending, as at its back edge, and the compiler can count the waits
+    // of the loop body exactly
+    __builtin_amdgcn_s_waitcnt(0);
+    const auto trip = [&](PathIn& cur, const PathIn& nxt, int b0) {
         {
             int p = -1;
             float pn[3] = {0.0f, 0.0f, 0.0f};
             v3f nrm = mk(0.0f, 0.0f, 1.0f), wo = mk(0.0f, 0.0f, 1.0f);
             if (cur.pp >= 0 && __float_as_int(cur.hv.y) >= 0) {
+                // hit_frame's point and shading normal, from the loaded words
                 const v3f o = cur.o, d = cur.d;
-                v3f point;
-                hit_frame(S, o, d, cur.hv, point, nrm);
+                const v3f point = o + d * cur.hv.x;
+                nrm = mk(fr.fn);
+                if (__float_as_int(fr.fn.w)) {
+                    const float u = cur.hv.z, v = cur.hv.w, w = 1.0f - u - v;
+                    nrm = make_dir_or(mk(fr.n0) * w + mk(fr.n1) * u + mk(fr.n2) * v, nrm);
+                }
                 wo = -d;
                 pn[0] = (point.x - G.bmin[0]) * G.binv[0];
                 pn[1] = (point.y - G.bmin[1]) * G.binv[1];
@@ -1506,9 +1602,13 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
             }
             path[lane] = p;
         }
-        __syncthreads();
-        const PathIn nxt = load_path(q_next);
-        q_next = load_q(base0 + 2 * stride);
+        lds_barrier();
+        fr = load_frame(nxt);
+        cur = load_path(q_next, b0 + 2 * stride);
+        q_next = load_q(b0 + 3 * stride);
+        // keep the loads ahead of the MLP (the scheduler would sink them
+        // towards their use in the next trip)
+        __builtin_amdgcn_sched_barrier(0);
         // the 4 tiles of 16 samples in passes of kInferTiles tiles (fewer
         // live accumulators per pass, more wavefronts per SIMD)
 #pragma unroll
@@ -1544,8 +1644,13 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
                 }
             }
         }
-        __syncthreads();  // the next iteration overwrites feat / path
-        cur = nxt;
+        lds_barrier();  // the next trip overwrites feat / path
+    };
+    // trip counts are uniform over the workgroup (its waves take 64 paths each)
+    for (; base0 < n; base0 += 2 * stride) {
+        trip(pa, pb, base0);
+        if (base0 + stride >= n) break;
+        trip(pb, pa, base0 + stride);
     }
 }
 
@@ -2658,7 +2763,11 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     const int64_t cap_trav = refill ? std::min<int64_t>((int64_t)ctx->num_cu * tgroups,
                                                         blocks(P * B, kBlock * kMinRaysPerLane))
                                     : INT64_MAX;
-    const int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
+    int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
+    // guided passes shade with their own group count, whatever shape the
+    // plain kernels (which hardly depend on it) were tuned to
+    if (pr->algorithm == LUCID_ALGO_GUIDED_MIS && shade_groups > 0)
+        cap_shade = (int64_t)ctx->num_cu * kGuidedShadeGroups;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     W.stop = (mode == 0 && cancel_flag == &ctx->cancel && !capturing) ? ctx->cancel_dev : nullptr;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
