@@ -107,6 +107,7 @@ struct WaveDev {
     // `tstride`, training slots of kTrainSlots bounces x 5 float4:
     // (p, T_in.x) (n, T_in.y) (wo, T_in.z) (wi, valid) (L_after, valid)
     __half* gout;
+    int64_t gspare;  // a spare row of gout that slots past the live queue write
     float4* tslot;
     int32_t tstride;
     int64_t tnrec;  // records (paths with slots); slot arrays are [k * 5 + field][record]
@@ -1476,114 +1477,150 @@ __global__ __launch_bounds__(256) void k_debug_frame(SceneDev S, WaveDev W, int3
     W.sh_d[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
 }
 
-// Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
-// release/acquire fence on all memory, so it would wait for this
-// wavefront's outstanding global loads too, and with them for the loads
-// issued for later trips; fences restricted to the "local" address space
-// wait for the LDS counter alone.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+// Orders this wavefront's LDS accesses across its lanes (a lane reads what
+// another lane of the same wavefront wrote, or overwrites what another lane
+// read): LDS instructions of one wavefront execute in order, so waiting for
+// the LDS counter is enough; no workgroup barrier is involved.
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt and expcnt untouched
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 // Wavefronts per workgroup: they share one LDS copy of the weights (31 KB);
-// with their feature rows a workgroup takes 65 KB, two per CU.
+// with their rows a workgroup takes 68 KB, two per CU.
 constexpr int kInferWaves = 4;
+// A lane's LDS row (36 dwords, 144 bytes) holds, between trips, the staged
+// inputs of its next two trips, and during a trip its path's 64 BF16
+// features (dwords 0..31):
+//   0..10   this trip's path: index (-1: slot past the queue), hit, ray origin, direction
+//   11..23  this trip's triangle normals: face normal + smooth flag, 3 vertex normals
+//   24..34  the next trip's path, as 0..10
+//   35      the queue entry of the trip after next
+constexpr int kRowDw = 36;
+struct StagedPath {
+    int32_t pp;
+    float4 hv;
+    float o[3], d[3];
+};
 __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, WaveDev W, guide::GuideDev G, int qin) {
     __shared__ uint4 wfwd[guide::kPackedFwd / 8];
     __shared__ float wbias[guide::kLayers * guide::kBiasStride];
-    constexpr int kRow = guide::kIn + 8;  // bf16 per row: 144 bytes
-    __shared__ __attribute__((aligned(16))) __bf16 feats[kInferWaves][64][kRow];
+    __shared__ __attribute__((aligned(16))) uint32_t rows[kInferWaves][64][kRowDw];
     __shared__ int32_t paths[kInferWaves][64];
     const int n = W.cnt[qin];
     if (n <= 0) return;  // uniform: every path of the wave has ended
     guide::stage_weights(G, wfwd, wbias);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, s = lane & 15;
-    __bf16 (*feat)[kRow] = feats[wave];
+    uint32_t* row = rows[wave][lane];
     int32_t* path = paths[wave];
     const auto bias_of = [&](int l) { return wbias + guide::kBiasStride * l; };
-    // Loads are software-pipelined over the trip loop, so that a trip's
-    // feature encoding waits on no memory: before each trip's MLP, whose
-    // matrix work then covers their latency, the wavefront issues the queue
-    // entries two trips ahead, the hits and rays of the next trip but one,
-    // and the triangle normals of the next trip (whose hits arrived a trip
-    // ago).  No register of a load issued in this trip is copied before the
-    // next trip (a copy would wait for the load), and the barriers order
-    // LDS accesses only (lds_barrier).
     const int stride = gridDim.x * kInferWaves * 64, lane0 = wave * 64 + lane;
-    // Every load below is issued unconditionally, from a valid address, and
-    // its register is not read until it is consumed a trip later: a load
-    // under a branch, or a select on a loaded register, would make the
-    // compiler wait for it (and for every load issued after it) at once.
-    // The queue entry of slot b0 + lane0 (the last entry past the end; the
-    // slot's validity is applied when the entry is consumed).
+    // Software pipeline through LDS: a trip reads its inputs from the row,
+    // issues the loads of later trips (the triangle normals of the next
+    // trip, the hits and rays of the trip after it, a queue entry), runs the
+    // MLP, and only then writes what arrived to the row.  Every load is
+    // issued and waited for inside one trip, from a valid address (a slot
+    // past the end reads the queue's last entry, a path without a hit reads
+    // path 0's hit record; neither is used), so the compiler places exact
+    // waits after the MLP; the wavefronts of a workgroup never wait for one
+    // another.
     const auto load_q = [&](int b0) { return W.q[qin][min(b0 + lane0, n - 1)]; };
-    struct PathIn {
-        int pp;  // path, or -1 for a slot past the end of the queue
-        float4 hv;
-        v3f o, d;
-    };
     const auto load_path = [&](int raw, int b0) {
-        PathIn r;
+        StagedPath r;
         r.pp = b0 + lane0 < n ? raw : -1;
         const int i = r.pp >= 0 ? r.pp : 0;
         const float* po = reinterpret_cast<const float*>(W.ray_o + i);
         const float* pd = reinterpret_cast<const float*>(W.ray_d + i);
         r.hv = W.hit[i];
-        r.o = mk(po[0], po[1], po[2]);
-        r.d = mk(pd[0], pd[1], pd[2]);
+        r.o[0] = po[0]; r.o[1] = po[1]; r.o[2] = po[2];
+        r.d[0] = pd[0]; r.d[1] = pd[1]; r.d[2] = pd[2];
         return r;
     };
-    // the hit triangle's face normal (w: smooth flag) and vertex normals; a
-    // path without a surface hit reads path 0's hit record instead (the
-    // words are not used)
     struct Frame {
-        float4 fn, n0, n1, n2;
+        float4 fn;
+        float n0[3], n1[3], n2[3];
     };
-    const auto load_frame = [&](const PathIn& c) {
+    const auto load_frame = [&](const StagedPath& c) {
         const int32_t prim = __float_as_int(c.hv.y);
         const bool hit = c.pp >= 0 && prim >= 0;
         const bool smooth = hit && S.vn != nullptr;
         const float4* fs = hit ? S.fn + prim : W.hit;
-        const float4* vs = smooth ? S.vn + 3 * (size_t)prim : W.hit;
-        const int k = smooth ? 1 : 0;
+        const float* vs = reinterpret_cast<const float*>(smooth ? S.vn + 3 * (size_t)prim : W.hit);
+        const int k = smooth ? 4 : 0;  // floats between vertex normals
         Frame f;
         f.fn = fs[0];
-        f.n0 = vs[0];
-        f.n1 = vs[k];
-        f.n2 = vs[2 * k];
+        f.n0[0] = vs[0]; f.n0[1] = vs[1]; f.n0[2] = vs[2];
+        f.n1[0] = vs[k]; f.n1[1] = vs[k + 1]; f.n1[2] = vs[k + 2];
+        f.n2[0] = vs[2 * k]; f.n2[1] = vs[2 * k + 1]; f.n2[2] = vs[2 * k + 2];
         return f;
     };
-    int base0 = blockIdx.x * kInferWaves * 64;
-    // Two hit / ray buffers take turns: a trip encodes the features of
-    // `cur` (its normals in `fr`), then loads the normals of `nxt` into `fr`
-    // and the hits and rays of the trip after next into `cur`, whose words
-    // it has consumed.  The loop is unrolled by two so that the buffers
-    // swap roles without register copies.
-    PathIn pa = load_path(load_q(base0), base0);
-    Frame fr = load_frame(pa);
-    PathIn pb = load_path(load_q(base0 + stride), base0 + stride);
-    int q_next = load_q(base0 + 2 * stride);
-    // drain the prologue's loads: the loop is then entered with none
-    // pending, as at its back edge, and the compiler can count the waits
-    // of the loop body exactly
-    __builtin_amdgcn_s_waitcnt(0);
-    const auto trip = [&](PathIn& cur, const PathIn& nxt, int b0) {
+    const auto put_path = [&](uint32_t* d, const StagedPath& c) {
+        d[0] = (uint32_t)c.pp;
+        d[1] = __float_as_uint(c.hv.x); d[2] = __float_as_uint(c.hv.y);
+        d[3] = __float_as_uint(c.hv.z); d[4] = __float_as_uint(c.hv.w);
+        for (int k = 0; k < 3; ++k) { d[5 + k] = __float_as_uint(c.o[k]); d[8 + k] = __float_as_uint(c.d[k]); }
+    };
+    const auto get_path = [&](const uint32_t* d) {
+        StagedPath c;
+        c.pp = (int32_t)d[0];
+        c.hv = make_float4(__uint_as_float(d[1]), __uint_as_float(d[2]), __uint_as_float(d[3]), __uint_as_float(d[4]));
+        for (int k = 0; k < 3; ++k) { c.o[k] = __uint_as_float(d[5 + k]); c.d[k] = __uint_as_float(d[8 + k]); }
+        return c;
+    };
+    const auto put_frame = [&](uint32_t* d, const Frame& f) {
+        d[0] = __float_as_uint(f.fn.x); d[1] = __float_as_uint(f.fn.y);
+        d[2] = __float_as_uint(f.fn.z); d[3] = __float_as_uint(f.fn.w);
+        for (int k = 0; k < 3; ++k) {
+            d[4 + k] = __float_as_uint(f.n0[k]); d[7 + k] = __float_as_uint(f.n1[k]); d[10 + k] = __float_as_uint(f.n2[k]);
+        }
+    };
+    // a row as 9 16-byte words
+    const auto put_row = [&](const uint32_t (&v)[kRowDw]) {
+        uint4* r4 = reinterpret_cast<uint4*>(row);
+#pragma unroll
+        for (int k = 0; k < kRowDw / 4; ++k) r4[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    };
+    int b0 = blockIdx.x * kInferWaves * 64;
+    {  // prologue: the first trip's inputs, the second trip's path
+        uint32_t v[kRowDw];
+        const StagedPath a0 = load_path(load_q(b0), b0);
+        put_path(v, a0);
+        put_frame(v + 11, load_frame(a0));
+        put_path(v + 24, load_path(load_q(b0 + stride), b0 + stride));
+        v[35] = (uint32_t)load_q(b0 + 2 * stride);
+        put_row(v);
+    }
+    // a wavefront's trips end when its first slot is past the queue
+    for (; b0 + wave * 64 < n; b0 += stride) {
+        wave_lds_fence();
+        uint32_t v[kRowDw];
+        {
+            const uint4* r4 = reinterpret_cast<const uint4*>(row);
+#pragma unroll
+            for (int k = 0; k < kRowDw / 4; ++k) {
+                const uint4 t = r4[k];
+                v[4 * k] = t.x; v[4 * k + 1] = t.y; v[4 * k + 2] = t.z; v[4 * k + 3] = t.w;
+            }
+        }
+        const StagedPath cur = get_path(v);
         {
             int p = -1;
             float pn[3] = {0.0f, 0.0f, 0.0f};
             v3f nrm = mk(0.0f, 0.0f, 1.0f), wo = mk(0.0f, 0.0f, 1.0f);
             if (cur.pp >= 0 && __float_as_int(cur.hv.y) >= 0) {
-                // hit_frame's point and shading normal, from the loaded words
-                const v3f o = cur.o, d = cur.d;
+                // hit_frame's point and shading normal, from the staged words
+                const v3f o = mk(cur.o[0], cur.o[1], cur.o[2]), d = mk(cur.d[0], cur.d[1], cur.d[2]);
                 const v3f point = o + d * cur.hv.x;
-                nrm = mk(fr.fn);
-                if (__float_as_int(fr.fn.w)) {
-                    const float u = cur.hv.z, v = cur.hv.w, w = 1.0f - u - v;
-                    nrm = make_dir_or(mk(fr.n0) * w + mk(fr.n1) * u + mk(fr.n2) * v, nrm);
+                nrm = mk(__uint_as_float(v[11]), __uint_as_float(v[12]), __uint_as_float(v[13]));
+                if ((int32_t)v[14]) {
+                    const float u = cur.hv.z, vv = cur.hv.w, w = 1.0f - u - vv;
+                    const v3f n0 = mk(__uint_as_float(v[15]), __uint_as_float(v[16]), __uint_as_float(v[17]));
+                    const v3f n1 = mk(__uint_as_float(v[18]), __uint_as_float(v[19]), __uint_as_float(v[20]));
+                    const v3f n2 = mk(__uint_as_float(v[21]), __uint_as_float(v[22]), __uint_as_float(v[23]));
+                    nrm = make_dir_or(n0 * w + n1 * u + n2 * vv, nrm);
                 }
                 wo = -d;
                 pn[0] = (point.x - G.bmin[0]) * G.binv[0];
@@ -1591,24 +1628,24 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
                 pn[2] = (point.z - G.bmin[2]) * G.binv[2];
                 p = cur.pp;
             }
-            __bf16 row[guide::kIn];
-            guide::features_bf16(pn, nrm, wo, row);
-            uint4* dst = reinterpret_cast<uint4*>(&feat[lane][0]);
+            __bf16 frow[guide::kIn];
+            guide::features_bf16(pn, nrm, wo, frow);
+            uint4* dst = reinterpret_cast<uint4*>(row);
 #pragma unroll
             for (int q = 0; q < guide::kIn / 8; ++q) {
-                uint4 v;
-                __builtin_memcpy(&v, row + 8 * q, 16);
-                dst[q] = v;
+                uint4 t;
+                __builtin_memcpy(&t, frow + 8 * q, 16);
+                dst[q] = t;
             }
             path[lane] = p;
         }
-        lds_barrier();
-        fr = load_frame(nxt);
-        cur = load_path(q_next, b0 + 2 * stride);
-        q_next = load_q(b0 + 3 * stride);
-        // keep the loads ahead of the MLP (the scheduler would sink them
-        // towards their use in the next trip)
+        // the loads of later trips, ahead of the MLP
+        const StagedPath nxt = get_path(v + 24);
+        const Frame fnew = load_frame(nxt);
+        const StagedPath pnew = load_path((int32_t)v[35], b0 + 2 * stride);
+        const int32_t qnew = load_q(b0 + 3 * stride);
         __builtin_amdgcn_sched_barrier(0);
+        wave_lds_fence();  // every lane's features are in its row
         // the 4 tiles of 16 samples in passes of kInferTiles tiles (fewer
         // live accumulators per pass, more wavefronts per SIMD)
 #pragma unroll
@@ -1619,38 +1656,39 @@ __global__ __launch_bounds__(kInferWaves * 64) void k_guide_infer(SceneDev S, Wa
             for (int t = 0; t < kInferTiles; ++t) {
                 // perm(kb, g, i): features 32 kb + 4 g + i (i < 4) and
                 // 32 kb + 16 + 4 g + i - 4, i.e. 8-byte runs 8 kb + g and 8 kb + 4 + g
-                const uint2* r = reinterpret_cast<const uint2*>(&feat[16 * (t0 + t) + s][0]);
+                const uint2* r = reinterpret_cast<const uint2*>(rows[wave][16 * (t0 + t) + s]);
                 const uint4 a = {r[g].x, r[g].y, r[4 + g].x, r[4 + g].y};
-                const uint4 b = {r[8 + g].x, r[8 + g].y, r[12 + g].x, r[12 + g].y};
+                const uint4 bq = {r[8 + g].x, r[8 + g].y, r[12 + g].x, r[12 + g].y};
                 __builtin_memcpy(&x0[t], &a, 16);
-                __builtin_memcpy(&x1[t], &b, 16);
+                __builtin_memcpy(&x1[t], &bq, 16);
                 pt[t] = path[16 * (t0 + t) + s];
             }
             guide::f32x4 o[kInferTiles][3];
             guide::mlp_forward_tiles_at<kInferTiles>(wfwd, bias_of, x0, x1, o);
 #pragma unroll
             for (int t = 0; t < kInferTiles; ++t) {
-                if (pt[t] < 0) continue;
-                __half* out = W.gout + (size_t)pt[t] * guide::kGoutStride;
+                // a slot past the queue writes the spare row
+                const size_t dst_path = pt[t] >= 0 ? (size_t)pt[t] : (size_t)W.gspare;
+                __half* out = W.gout + dst_path * guide::kGoutStride;
                 for (int m = 0; m < 3; ++m) {
                     const int f = 16 * m + 4 * g;  // 4 outputs per lane and tile, 8-byte aligned
                     if (f >= guide::kGoutStride) continue;
                     __half2 lo = __floats2half2_rn(o[t][m][0], o[t][m][1]);
                     __half2 hi = __floats2half2_rn(o[t][m][2], o[t][m][3]);
-                    uint2 v;
-                    __builtin_memcpy(&v.x, &lo, 4);
-                    __builtin_memcpy(&v.y, &hi, 4);
-                    *reinterpret_cast<uint2*>(out + f) = v;
+                    uint2 u2;
+                    __builtin_memcpy(&u2.x, &lo, 4);
+                    __builtin_memcpy(&u2.y, &hi, 4);
+                    *reinterpret_cast<uint2*>(out + f) = u2;
                 }
             }
         }
-        lds_barrier();  // the next trip overwrites feat / path
-    };
-    // trip counts are uniform over the workgroup (its waves take 64 paths each)
-    for (; base0 < n; base0 += 2 * stride) {
-        trip(pa, pb, base0);
-        if (base0 + stride >= n) break;
-        trip(pb, pa, base0 + stride);
+        // the next trip's inputs (every lane has read its operands)
+        wave_lds_fence();
+        for (int k = 0; k < 11; ++k) v[k] = v[24 + k];
+        put_frame(v + 11, fnew);
+        put_path(v + 24, pnew);
+        v[35] = (uint32_t)qnew;
+        put_row(v);
     }
 }
 
@@ -2723,6 +2761,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     guide::GuideDev G;
     std::memset(&G, 0, sizeof(G));
     W.gout = nullptr;
+    W.gspare = 0;
     W.tslot = nullptr;
     W.tstride = 1;
     W.tnrec = 0;
@@ -2730,9 +2769,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     if (guided) {
         rc = ensure_guide(ctx);
         if (rc) return rc;
-        rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, P * B * guide::kGoutStride);
+        rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, (P * B + 1) * guide::kGoutStride);
         if (rc) return rc;
         W.gout = ctx->gout;
+        W.gspare = P * B;
         if (train) {
             W.tstride = std::max(1, ctx->gcfg.record_stride);
             const int64_t nrec = (P * B + W.tstride - 1) / W.tstride;
@@ -3983,13 +4023,14 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     if (scene->device != ctx->device) return fail(ctx, LUCID_ERR_INVALID, "scene belongs to another device");
     int rc = ensure_guide(ctx);
     if (!rc) rc = ensure_wave(ctx, ctx->lanes[0], n);
-    if (!rc) rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, (int64_t)n * guide::kGoutStride);
+    if (!rc) rc = ensure_buf(ctx, &ctx->gout, &ctx->gout_cap, ((int64_t)n + 1) * guide::kGoutStride);
     if (rc) return rc;
     if (guide::set_params(ctx->gstate, params, ctx->stream)) return fail(ctx, LUCID_ERR_DEVICE, "guiding: copy failed");
     ctx->gsteps = std::max<int64_t>(ctx->gsteps, 1);
     hipStream_t st = ctx->stream;
     WaveDev W = ctx->lanes[0].W;
     W.gout = ctx->gout;
+    W.gspare = n;
     W.dcnt = nullptr;
     W.count_work = 0;
     std::vector<float4> o4((size_t)n), d4((size_t)n);
