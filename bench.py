@@ -157,7 +157,9 @@ def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
     """Batched sessions: `count` sessions (edges log-uniform on [128, 1024],
     scenes log-uniform on [10k, 500k] triangles, Principled and Emission
     materials, 1-8 lights), one batched pass of `spp` samples per session
-    per step, against the same passes issued one session at a time."""
+    per step: fused (one launch per kernel and bounce over all sessions, the
+    library default), spread over the four stream lanes, and issued one
+    session at a time."""
     import lucid_capi
     import workloads
 
@@ -191,19 +193,28 @@ def sessions_leg(ctx, passes, spp=16, count=32, seed=7):
         return rays / t / 1e6, t / passes * 1e3
 
     b_mrays, b_ms = timed(True)
+    ctx.check(ctx.L.lucid_set_session_fusion(ctx.h, 0))
+    for _ in range(2):  # the lane passes' warm-up (allocation, graph capture)
+        lucid_capi.render_sessions(ctx, sessions, spp)
+    l_mrays, l_ms = timed(True)
     for _ in range(2):  # the one-at-a-time passes' own warm-up
         for s in sessions:
             lucid_capi.render_sessions(ctx, [s], spp)
     s_mrays, s_ms = timed(False)
-    log(f"sessions: batched {b_mrays:.0f} Mrays/s ({b_ms:.0f} ms/pass), one at a time {s_mrays:.0f} Mrays/s")
+    ctx.check(ctx.L.lucid_set_session_fusion(ctx.h, 1))
+    log(f"sessions: fused {b_mrays:.0f} Mrays/s ({b_ms:.0f} ms/pass), four lanes {l_mrays:.0f} Mrays/s "
+        f"({l_ms:.0f} ms/pass), one at a time {s_mrays:.0f} Mrays/s")
     for s in sessions:
         s.close()
     for h in scenes:
         ctx.L.lucid_scene_release(h)
     return {"count": count, "seed": seed, "triangles": tris, "pixels": pixels, "pass_spp": spp, "max_depth": 8,
-            "algorithm": "mis", "passes": passes, "batched_mrays_s": round(b_mrays, 1),
-            "batched_ms_per_pass": round(b_ms, 1), "serial_mrays_s": round(s_mrays, 1),
-            "serial_ms_per_pass": round(s_ms, 1), "speedup": round(b_mrays / max(s_mrays, 1e-9), 3)}
+            "algorithm": "mis", "passes": passes, "batched": "fused",
+            "batched_mrays_s": round(b_mrays, 1), "batched_ms_per_pass": round(b_ms, 1),
+            "lanes_mrays_s": round(l_mrays, 1), "lanes_ms_per_pass": round(l_ms, 1),
+            "serial_mrays_s": round(s_mrays, 1), "serial_ms_per_pass": round(s_ms, 1),
+            "speedup": round(b_mrays / max(s_mrays, 1e-9), 3),
+            "fused_over_lanes": round(b_mrays / max(l_mrays, 1e-9), 3)}
 
 
 def volumes_leg(ctx, passes, spp=16):

@@ -533,14 +533,13 @@ __device__ __forceinline__ v3f cam_dir(const CamDev& c, float nx, float ny) {
     return make_dir_or(d, mk(c.fwd[0], c.fwd[1], c.fwd[2]));
 }
 
-__global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
-    int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
-    if (p >= n_paths) return;
-    int b = p / c.P, pix = p - b * c.P;
+// One camera path: wave slot p, pixel `pix` of the tile, sample index s.
+// meta.w and the direction's w carry `tag` and `dw` (0 for render_tile;
+// the session and its BVH root in fused session passes).
+__device__ __forceinline__ void raygen_path(const WaveDev& W, const CamDev& c, int p, int pix, int s, int tag,
+                                            float dw) {
     int px = pix % c.tw, py = pix / c.tw;
     int gx = c.tx + px, gy = c.ty + py;
-    int s = (c.s0p ? *c.s0p : c.s0) + b;
     float ndc_x = (2.0f * (gx + 0.5f) / c.fw - 1.0f) * c.aspect;
     float ndc_y = 1.0f - 2.0f * (gy + 0.5f) / c.fh;
     Pcg rng = pcg_seed((uint32_t)gx, (uint32_t)gy, (uint32_t)s);
@@ -553,11 +552,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
         d = cam_dir(c, ndc_x, ndc_y);
     }
     W.ray_o[p] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
-    W.ray_d[p] = make_float4(d.x, d.y, d.z, 0.0f);
+    W.ray_d[p] = make_float4(d.x, d.y, d.z, dw);
     W.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     W.rad[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     W.rng[p] = make_ulonglong2(rng.state, rng.inc);
-    W.meta[p] = make_int4(pix, s, 0, 0);
+    W.meta[p] = make_int4(pix, s, 0, tag);
     W.q[0][p] = p;
     if (W.dcnt) W.dcnt[p] = 0;
     if (keeps_record(W, p)) {
@@ -567,6 +566,13 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
             *tslot_at(W, r, k, 4) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
+}
+__global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (p >= n_paths) return;
+    int b = p / c.P, pix = p - b * c.P;
+    raygen_path(W, c, p, pix, (c.s0p ? *c.s0p : c.s0) + b, 0, 0.0f);
 }
 
 // ------------------------------------------------------------- intersect --
@@ -603,9 +609,11 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int
 // MODE 1: shadow any-hit; MODE 2: shadow past boundary-only meshes
 // (shadowed with re-queries from a moving origin).  Per ray the steps and
 // their order are those of closest_real / shadowed.
+template <bool FUSED = false>
 __device__ __forceinline__ void shadow_unoccluded(const WaveDev& W, int p) {
     const float4 L = W.rad[p], c = W.sh_c[p];
     W.rad[p] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.0f);
+    if (FUSED) return;  // the direction's w holds the BVH root there
     // diagnostic runs: the pending NEE record of this bounce is kept
     const int32_t tag = __float_as_int(W.sh_d[p].w);
     if (tag > 0) {
@@ -614,7 +622,9 @@ __device__ __forceinline__ void shadow_unoccluded(const WaveDev& W, int p) {
     }
 }
 
-template <int MODE>
+// FUSED: a fused session pass over the sessions' merged scenes; every ray
+// starts at its own scene's BVH root, stored in its direction's w.
+template <int MODE, bool FUSED = false>
 __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, int qin, int sd, int refill,
                                                          int walk_min, int leaf_tris) {
     constexpr bool SHADOW = MODE != 0, ANY = MODE == 1;
@@ -639,7 +649,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
     float tmin = 0.0f, tmax = 0.0f, toff = 0.0f;  // MODE 2: shadow distance, offset of the moving origin
     TraceHit h;
     h.t = 0.0f; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-    int32_t node = kPop, leaf = 0;
+    int32_t node = kPop, leaf = 0, root = S.root;
     int sp = 0, requery = 0;
     for (;;) {
         const unsigned long long im = __ballot(p < 0);
@@ -661,21 +671,23 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                         tmax = o4.w;
                         toff = 0.0f;
                         h.t = o4.w;
+                        if (FUSED) root = __float_as_int(d4.w);
                     } else {
                         p = W.q[qin][i];
                         const float4 o4 = W.ray_o[p], d4 = W.ray_d[p];
                         o = mk(o4); d = mk(d4);
                         tmin = o4.w;
                         h.t = RAY_T_MAX;
+                        if (FUSED) root = __float_as_int(d4.w);
                     }
                     id = inv_dir(d);
                     h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-                    node = S.root;
+                    node = root;
                     leaf = 0;
                     sp = 0;
                     requery = 0;
                     if (MODE == 2 && !(tmax > 0.0f)) {  // nothing left to test: unoccluded
-                        shadow_unoccluded(W, p);
+                        shadow_unoccluded<FUSED>(W, p);
                         p = -1;
                     }
                 }
@@ -720,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                     ++requery;
                     tmin = h.t + 1e-3f;
                     h.t = RAY_T_MAX; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-                    node = S.root;
+                    node = root;
                     leaf = 0;
                     sp = 0;
                     continue;
@@ -729,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             }
             W.hit[p] = make_float4(h.t, __int_as_float(h.prim), h.u, h.v);
         } else if (MODE == 1) {
-            if (h.prim < 0) shadow_unoccluded(W, p);
+            if (h.prim < 0) shadow_unoccluded<FUSED>(W, p);
         } else {
             // is_shadowed_skipping_volume_boundaries: a real surface within
             // the remaining distance occludes; a boundary-only one moves the
@@ -743,14 +755,14 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
                 const float remaining = tmax - toff;
                 if (requery < 8 && remaining > 0.0f) {
                     h.t = remaining; h.u = 0.0f; h.v = 0.0f; h.prim = -1;
-                    node = S.root;
+                    node = root;
                     leaf = 0;
                     sp = 0;
                     continue;
                 }
                 unoccluded = true;
             }
-            if (unoccluded) shadow_unoccluded(W, p);
+            if (unoccluded) shadow_unoccluded<FUSED>(W, p);
         }
         p = -1;
     }
@@ -978,7 +990,10 @@ __device__ __forceinline__ void finish_guided_nee(const WaveDev& W, int p, float
 // their volume variants (:209-283, 586-835), or the diagnostic tracers.
 // NODES: the scene has node-tree programs (separate instantiations keep
 // their registers out of the kernels of scenes without them).
-template <int ALGO, bool NODES>
+// FUSED: a fused session pass; S is the path's session's view of the merged
+// scenes (see batch_view), the path keeps its session in meta.w and its
+// scene's BVH root in the w of its ray and shadow-ray directions.
+template <int ALGO, bool NODES, bool FUSED = false>
 __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, const guide::GuideDev& G, int qin,
                                           int32_t max_depth, int i, int n, bool stop) {
     constexpr bool DIAG = ALGO >= kAlgoDiagBase;
@@ -1074,7 +1089,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                                 // the transmittance march runs in k_vol_march
                                 defer_transmittance(W, p, pw * mw, ls.emission, thr);
                                 W.sh_o[p] = make_float4(x.x, x.y, x.z, ls.dist);
-                                W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, 0.0f);
+                                W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, FUSED ? __int_as_float(S.root) : 0.0f);
                                 nee = true;
                             }
                         }
@@ -1093,10 +1108,10 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                     if (ok && finite3(thr) && depth + 1 < max_depth) {
                         cont = true;
                         W.ray_o[p] = make_float4(x.x, x.y, x.z, RAY_T_MIN);
-                        W.ray_d[p] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+                        W.ray_d[p] = make_float4(wi.x, wi.y, wi.z, FUSED ? __int_as_float(S.root) : 0.0f);
                         W.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                         W.rng[p] = make_ulonglong2(rng.state, rng.inc);
-                        W.meta[p] = make_int4(meta.x, meta.y, depth + 1, 0);
+                        W.meta[p] = make_int4(meta.x, meta.y, depth + 1, FUSED ? meta.w : 0);
                     }
                 }
             }
@@ -1288,7 +1303,7 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
                                 tag = __int_as_float(depth + 1);
                             }
                             W.sh_o[p] = make_float4(point.x, point.y, point.z, ls.dist);
-                            W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tag);
+                            W.sh_d[p] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, FUSED ? __int_as_float(S.root) : tag);
                             if (!(GUIDED && nee_wait)) W.sh_c[p] = make_float4(cc.x, cc.y, cc.z, 0.0f);
                             nee = true;
                         }
@@ -1381,10 +1396,12 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
             if (ok && finite3(thr)) {
                 cont = true;
                 W.ray_o[p] = make_float4(point.x, point.y, point.z, RAY_T_MIN);
-                W.ray_d[p] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z, GUIDED && bs.kind == 0 ? bs.pdf : 0.0f);
+                W.ray_d[p] = make_float4(bs.wi.x, bs.wi.y, bs.wi.z,
+                                         FUSED ? __int_as_float(S.root) : (GUIDED && bs.kind == 0 ? bs.pdf : 0.0f));
                 W.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 W.rng[p] = make_ulonglong2(rng.state, rng.inc);
-                W.meta[p] = make_int4(meta.x, meta.y, depth + 1, GUIDED && bs.kind == 0 ? __float_as_int(next_1m_rho) : 0);
+                W.meta[p] = make_int4(meta.x, meta.y, depth + 1,
+                                      FUSED ? meta.w : (GUIDED && bs.kind == 0 ? __float_as_int(next_1m_rho) : 0));
             }
             }
             if (GUIDED && nee_wait) {
@@ -1771,6 +1788,120 @@ __global__ __launch_bounds__(256) void k_film(WaveDev W, float4* film, int32_t P
     }
     film[pix] = a;
 }
+// ---- fused session passes: the sessions of a batched pass, their scenes
+// merged into one set of arrays (their node and triangle ranges laid end to
+// end; node links and leaf triangle ranges shifted to the merged ranges),
+// share one launch per kernel and bounce.  A path carries its session in
+// meta.w and its scene's BVH root in the w of its ray directions; the
+// per-scene values (light list, CDF, materials, node programs, environment)
+// are read through the session's view of the merged arrays, so each path
+// computes exactly what a render of its session alone computes.
+struct FusedSession {
+    CamDev cam;            // the session's camera; cam.s0 = its first sample index of the pass
+    float4* accum;         // its radiance sums
+    int64_t pix_off;       // its first pixel in the pass's pixel range
+    int32_t root, max_depth;
+    int32_t mat_off, prog_off, light_off, native_off;
+    int32_t n_lights, n_native, native_start, pad;
+    float total_area, env[3];
+};
+struct FusedDev {
+    const FusedSession* ses;
+    int32_t n_ses;
+    int64_t P;  // pixels of all sessions
+};
+// the session of pixel `pix` of the pass (sessions in pixel order)
+__device__ __forceinline__ int fused_session_of(const FusedDev& F, int64_t pix) {
+    int lo = 0, hi = F.n_ses - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (F.ses[mid].pix_off <= pix) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// a session's view of the merged scene arrays: the per-triangle and node
+// arrays are shared (global triangle and node indices), the per-scene lists
+// start at the session's offsets, so the indices stored in the scene's own
+// records (mesh, light, node program, native light) stay valid
+__device__ __forceinline__ SceneDev fused_view(const SceneDev& A, const FusedSession& f) {
+    SceneDev v = A;
+    v.root = f.root;
+    v.mats = A.mats + f.mat_off;
+    v.prog = A.prog ? A.prog + f.prog_off : nullptr;
+    v.lights = A.lights + f.light_off;
+    v.cdf = A.cdf + f.light_off;
+    v.native_ids = A.native_ids + f.native_off;
+    v.n_lights = f.n_lights;
+    v.n_native = f.n_native;
+    v.native_start = f.native_start;
+    v.total_area = f.total_area;
+    v.env[0] = f.env[0]; v.env[1] = f.env[1]; v.env[2] = f.env[2];
+    v.n_vols = 0;
+    v.any_vbo = 0;
+    return v;
+}
+// wave slot p = b * P + pixel: sample b of every pixel of every session
+__global__ __launch_bounds__(kBlock) void k_raygen_fused(WaveDev W, FusedDev F, int32_t s_base, int64_t n_paths) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) { W.cnt[0] = (int32_t)n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (p >= n_paths) return;
+    const int64_t b = p / F.P, pix = p - b * F.P;
+    const int sid = fused_session_of(F, pix);
+    const FusedSession& f = F.ses[sid];
+    raygen_path(W, f.cam, (int)p, (int)(pix - f.pix_off), f.cam.s0 + s_base + (int)b, sid, __int_as_float(f.root));
+}
+template <int ALGO, bool NODES>
+__global__ __launch_bounds__(kBlock) void k_shade_fused(SceneDev A, WaveDev W, FusedDev F, int qin) {
+    const int n = W.cnt[qin];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    guide::GuideDev G;
+    __builtin_memset(&G, 0, sizeof(G));
+    for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const int i = base + (int)threadIdx.x;
+        int sid = 0;
+        if (i < n) sid = W.meta[W.q[qin][i]].w;
+        const FusedSession& f = F.ses[sid];
+        shade_one<ALGO, NODES, true>(fused_view(A, f), W, G, qin, f.max_depth, i, n, false);
+    }
+}
+template <int ALGO>
+void launch_shade_fused(unsigned gs, hipStream_t st, const SceneDev& A, const WaveDev& W, const FusedDev& F, int qin) {
+    if (A.prog)
+        k_shade_fused<ALGO, true><<<gs, kBlock, 0, st>>>(A, W, F, qin);
+    else
+        k_shade_fused<ALGO, false><<<gs, kBlock, 0, st>>>(A, W, F, qin);
+}
+// k_film for the sessions of a fused pass: each pixel's B samples in order
+// onto its session's sums
+__global__ __launch_bounds__(256) void k_film_fused(WaveDev W, FusedDev F, int32_t B) {
+    const int64_t pix = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pix >= F.P) return;
+    const FusedSession& f = F.ses[fused_session_of(F, pix)];
+    float4* film = f.accum + (pix - f.pix_off);
+    float4 a = *film;
+    for (int b = 0; b < B; ++b) {
+        const float4 r = W.rad[(size_t)b * F.P + pix];
+        a.x = a.x + r.x; a.y = a.y + r.y; a.z = a.z + r.z;
+    }
+    *film = a;
+}
+// A scene's nodes copied into the merged array: child node indices and
+// leaf triangle ranges shifted by the scene's offsets.
+__global__ __launch_bounds__(256) void k_rebase_nodes(GNode* nodes, int64_t n, int32_t node_off, int32_t tri_off) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < 2; ++c) {
+        const int32_t ch = nodes[i].child[c];
+        if (ch == kEmptyChild) continue;
+        if (ch >= 0) {
+            nodes[i].child[c] = ch + node_off;
+        } else {
+            const int32_t start = (~ch) >> 3, count = (~ch) & 7;
+            nodes[i].child[c] = ~(((start + tri_off) << 3) | count);  // encode_leaf
+        }
+    }
+}
+
 // ---- ray binning: a ray queue regrouped before its traversal pass, so
 // that a wavefront's rays start in the same region (and head the same way).
 // Counting sort on a 12-bit key; the order within a bin is arbitrary.
@@ -2289,9 +2420,11 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+std::atomic<uint64_t> g_scene_serial{0};
 struct lucid_scene {
     std::atomic<int> refs{1};
     int device = 0;
+    uint64_t serial = ++g_scene_serial;  // identifies the scene in the fused-pass arena cache
     PackedScene host;
     std::vector<void*> allocs;
     SceneDev dev;
@@ -2317,6 +2450,15 @@ struct Lane {
     int64_t vcap = 0;
 };
 
+// The merged scenes of a fused pass (cached: the next pass over the same
+// scenes reuses them).
+struct FusedArena {
+    std::vector<uint64_t> key;  // serials of the scenes, in first-use order
+    std::vector<void*> allocs;
+    SceneDev dev{};
+    std::vector<int32_t> node_off, tri_off, mat_off, prog_off, light_off, native_off;
+    int sd = 2;  // traversal stack entries: the deepest scene's
+};
 struct lucid_ctx {
     int device = 0;
     int num_cu = 256;
@@ -2355,6 +2497,13 @@ struct lucid_ctx {
     // hits before the shading launch (class_order); 0: compaction order
     int32_t shade_order = 0;
     int32_t session_graphs = 1;  // replay plain one-wave session passes from captured graphs
+    // 1: a batched pass whose sessions all qualify runs fused (one launch per
+    // kernel and bounce over all of them, lucid_set_session_fusion); 0: the
+    // sessions are spread over the stream lanes
+    int32_t session_fusion = 1;
+    FusedArena* arena = nullptr;  // the merged scenes of the last fused pass
+    FusedSession* fses = nullptr;        // its per-session records
+    int64_t fses_cap = 0;
     int32_t grid_bits = 32;  // density-grid storage of scenes created from now on
     int32_t scene_build = 1;  // 1: BVHs and triangle records built on the device, 0: on the host
     float4* film = nullptr;
@@ -3110,6 +3259,12 @@ int lucid_create(int32_t device, lucid_ctx** out) {
     return LUCID_OK;
 }
 
+}  // extern "C"
+namespace {
+void free_arena(lucid_ctx* ctx);
+}  // namespace
+extern "C" {
+
 void lucid_destroy(lucid_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
@@ -3128,6 +3283,8 @@ void lucid_destroy(lucid_ctx* ctx) {
     if (ctx->dtotal) (void)hipFree(ctx->dtotal);
     if (ctx->film) (void)hipFree(ctx->film);
     if (ctx->out_dev) (void)hipFree(ctx->out_dev);
+    free_arena(ctx);
+    if (ctx->fses) (void)hipFree(ctx->fses);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     for (int i = 0; i < ctx->lanes_ready; ++i) {
@@ -3801,6 +3958,13 @@ int lucid_set_refill_threshold(lucid_ctx* ctx, int32_t idle_lanes) {
     return LUCID_OK;
 }
 
+int lucid_set_session_fusion(lucid_ctx* ctx, int32_t enabled) {
+    if (!ctx || enabled < 0 || enabled > 1) return LUCID_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->session_fusion = enabled;
+    return LUCID_OK;
+}
+
 int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on) {
     if (!ctx || on < 0 || on > 1) return LUCID_ERR_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -4334,6 +4498,274 @@ void session_stop(lucid_session* s) {
     s->running.store(false);
 }
 
+void free_arena(lucid_ctx* ctx) {
+    if (!ctx->arena) return;
+    for (void* q : ctx->arena->allocs) (void)hipFree(q);
+    delete ctx->arena;
+    ctx->arena = nullptr;
+}
+
+// A pass over these sessions can run fused: persistent traversal, one
+// surface algorithm for all (MIS, NEE or simple), scenes with geometry and
+// without participating media or volume-boundary meshes.
+bool fusable(const lucid_ctx* ctx, lucid_session* const* ss, int32_t n) {
+    if (!ctx->session_fusion || n < 2 || ctx->traverse_groups <= 0 || ctx->diag_on) return false;
+    const int32_t algo = ss[0]->algorithm;
+    if (algo != LUCID_ALGO_MIS && algo != LUCID_ALGO_NEE && algo != LUCID_ALGO_SIMPLE) return false;
+    int64_t P = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const lucid_session* s = ss[i];
+        if (s->algorithm != algo || s->scene->n_nodes <= 0 || s->scene->dev.any_vbo || s->scene->dev.n_vols)
+            return false;
+        P += (int64_t)s->w * s->h;
+    }
+    return P > 0 && P < ((int64_t)1 << 30);
+}
+
+// the scenes of `ss` merged (or the cached merge of the same scenes);
+// scene_of[i] = the merged scene index of session i
+int ensure_arena(lucid_ctx* ctx, lucid_session* const* ss, int32_t n, std::vector<int>& scene_of) {
+    std::vector<uint64_t> key;
+    std::vector<lucid_scene*> scenes;
+    scene_of.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        lucid_scene* sc = ss[i]->scene;
+        int k = 0;
+        while (k < (int)key.size() && key[k] != sc->serial) ++k;
+        if (k == (int)key.size()) {
+            key.push_back(sc->serial);
+            scenes.push_back(sc);
+        }
+        scene_of[i] = k;
+    }
+    if (ctx->arena && ctx->arena->key == key) return LUCID_OK;
+    free_arena(ctx);
+    FusedArena* A = new FusedArena();
+    ctx->arena = A;
+    A->key = key;
+    const int ns = (int)scenes.size();
+    int64_t N = 0, T = 0, M = 0, G = 0, Lt = 0, Nt = 0;
+    bool any_vn = false, any_uv = false;
+    for (lucid_scene* sc : scenes) {
+        A->node_off.push_back((int32_t)N);
+        A->tri_off.push_back((int32_t)T);
+        A->mat_off.push_back((int32_t)M);
+        A->prog_off.push_back((int32_t)G);
+        A->light_off.push_back((int32_t)Lt);
+        A->native_off.push_back((int32_t)Nt);
+        N += sc->n_nodes;
+        T += sc->host.num_triangles;
+        M += (int64_t)sc->host.mats.size();
+        G += (int64_t)sc->host.node_ops.size();
+        Lt += (int64_t)sc->host.lights.size();
+        Nt += (int64_t)sc->host.native_light_ids.size();
+        any_vn = any_vn || sc->dev.vn;
+        any_uv = any_uv || sc->dev.tuv;
+        A->sd = std::max(A->sd, std::min(kStackMax, sc->host.max_depth + 2));
+    }
+    if (N + 1 >= INT32_MAX || T >= ((int64_t)1 << 28)) return fail(ctx, LUCID_ERR_INVALID, "fused pass: scenes too large");
+    const auto get = [&](int64_t bytes, void** out) {
+        *out = nullptr;
+        if (bytes <= 0) return LUCID_OK;
+        if (hipMalloc(out, (size_t)bytes) != hipSuccess) return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (fused scenes)");
+        A->allocs.push_back(*out);
+        return LUCID_OK;
+    };
+    void *nodes, *tri, *fn, *vn, *uv, *mats, *prog, *lights, *cdf, *nat;
+    int rc = get(N * 64, &nodes);
+    if (!rc) rc = get(T * 48, &tri);
+    if (!rc) rc = get(T * 16, &fn);
+    if (!rc) rc = get(any_vn ? T * 48 : 0, &vn);
+    if (!rc) rc = get(any_uv ? T * 24 : 0, &uv);
+    if (!rc) rc = get(M * (int64_t)sizeof(GMat), &mats);
+    if (!rc) rc = get(G * (int64_t)sizeof(GNodeOp), &prog);
+    if (!rc) rc = get(Lt * (int64_t)sizeof(GLight), &lights);
+    if (!rc) rc = get(Lt * 4, &cdf);
+    if (!rc) rc = get(Nt * 4, &nat);
+    if (rc) {
+        free_arena(ctx);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    const auto cp = [&](void* dst, int64_t off_bytes, const void* src, int64_t bytes) {
+        if (bytes <= 0) return LUCID_OK;
+        if (!src) return hipMemsetAsync(static_cast<char*>(dst) + off_bytes, 0, (size_t)bytes, st) == hipSuccess
+                             ? LUCID_OK
+                             : fail(ctx, LUCID_ERR_DEVICE, "fused pass: clear failed");
+        return hipMemcpyAsync(static_cast<char*>(dst) + off_bytes, src, (size_t)bytes, hipMemcpyDeviceToDevice, st) ==
+                       hipSuccess
+                   ? LUCID_OK
+                   : fail(ctx, LUCID_ERR_DEVICE, "fused pass: copy failed");
+    };
+    for (int k = 0; k < ns && !rc; ++k) {
+        const lucid_scene* sc = scenes[k];
+        const SceneDev& D = sc->dev;
+        const int64_t t = sc->host.num_triangles;
+        rc = cp(nodes, (int64_t)A->node_off[k] * 64, D.nodes, sc->n_nodes * 64);
+        if (!rc) rc = cp(tri, (int64_t)A->tri_off[k] * 48, D.tri, t * 48);
+        if (!rc) rc = cp(fn, (int64_t)A->tri_off[k] * 16, D.fn, t * 16);
+        if (!rc && any_vn) rc = cp(vn, (int64_t)A->tri_off[k] * 48, D.vn, t * 48);
+        if (!rc && any_uv) rc = cp(uv, (int64_t)A->tri_off[k] * 24, D.tuv, t * 24);
+        if (!rc) rc = cp(mats, (int64_t)A->mat_off[k] * (int64_t)sizeof(GMat), D.mats,
+                         (int64_t)sc->host.mats.size() * (int64_t)sizeof(GMat));
+        if (!rc) rc = cp(prog, (int64_t)A->prog_off[k] * (int64_t)sizeof(GNodeOp), D.prog,
+                         (int64_t)sc->host.node_ops.size() * (int64_t)sizeof(GNodeOp));
+        if (!rc) rc = cp(lights, (int64_t)A->light_off[k] * (int64_t)sizeof(GLight), D.lights,
+                         (int64_t)sc->host.lights.size() * (int64_t)sizeof(GLight));
+        if (!rc) rc = cp(cdf, (int64_t)A->light_off[k] * 4, D.cdf, (int64_t)sc->host.lights.size() * 4);
+        if (!rc) rc = cp(nat, (int64_t)A->native_off[k] * 4, D.native_ids, (int64_t)sc->host.native_light_ids.size() * 4);
+        if (!rc && sc->n_nodes > 0)
+            k_rebase_nodes<<<blocks(sc->n_nodes, 256), 256, 0, st>>>(static_cast<GNode*>(nodes) + A->node_off[k],
+                                                                     sc->n_nodes, A->node_off[k], A->tri_off[k]);
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, LUCID_ERR_DEVICE, "fused pass: merge failed");
+    if (rc) {
+        free_arena(ctx);
+        return rc;
+    }
+    SceneDev& D = A->dev;
+    std::memset(&D, 0, sizeof(D));
+    D.nodes = static_cast<const GNode*>(nodes);
+    D.tri = static_cast<const float4*>(tri);
+    D.fn = static_cast<const float4*>(fn);
+    D.vn = static_cast<const float4*>(vn);
+    D.tuv = static_cast<const float2*>(uv);
+    D.mats = static_cast<const GMat*>(mats);
+    D.prog = G > 0 ? static_cast<const GNodeOp*>(prog) : nullptr;
+    D.lights = static_cast<const GLight*>(lights);
+    D.cdf = static_cast<const float*>(cdf);
+    D.native_ids = static_cast<const int32_t*>(nat);
+    return LUCID_OK;
+}
+
+// One pass of `spp` samples over fusable sessions (see fusable): waves of
+// whole samples of every session's pixels, each bounce one traversal, one
+// shading and one shadow launch for all of them.
+int fused_sessions_pass(lucid_ctx* ctx, lucid_session* const* ss, int32_t n, int32_t spp, bool* cancelled) {
+    *cancelled = false;
+    std::vector<int> scene_of;
+    int rc = ensure_arena(ctx, ss, n, scene_of);
+    if (rc) return rc;
+    const FusedArena& A = *ctx->arena;
+    std::vector<FusedSession> hs((size_t)n);
+    int64_t P = 0;
+    int32_t max_depth = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const lucid_session* s = ss[i];
+        if (s->samples + spp > INT32_MAX) return fail(ctx, LUCID_ERR_INVALID, "session sample count overflows");
+        FusedSession& f = hs[i];
+        std::memset(&f, 0, sizeof(f));
+        const CameraBasis cb = camera_basis(s->cam, s->w, s->h);
+        std::memcpy(f.cam.pos, cb.pos, 12); std::memcpy(f.cam.fwd, cb.fwd, 12);
+        std::memcpy(f.cam.right, cb.right, 12); std::memcpy(f.cam.up, cb.up, 12);
+        f.cam.aspect = cb.aspect; f.cam.scale = cb.scale;
+        f.cam.fw = s->w; f.cam.fh = s->h; f.cam.tx = 0; f.cam.ty = 0; f.cam.tw = s->w; f.cam.th = s->h;
+        f.cam.samples_in_call = std::max(spp, 2);  // a session's samples are always jittered
+        f.cam.s0 = (int32_t)s->samples;
+        f.cam.P = s->w * s->h;
+        f.cam.max_depth = s->max_depth;
+        f.cam.s0p = nullptr;
+        f.accum = s->accum;
+        f.pix_off = P;
+        const int k = scene_of[i];
+        const SceneDev& D = s->scene->dev;
+        f.root = A.node_off[k] + D.root;
+        f.max_depth = s->max_depth;
+        f.mat_off = A.mat_off[k]; f.prog_off = A.prog_off[k];
+        f.light_off = A.light_off[k]; f.native_off = A.native_off[k];
+        f.n_lights = D.n_lights; f.n_native = D.n_native; f.native_start = D.native_start;
+        f.total_area = D.total_area;
+        f.env[0] = D.env[0]; f.env[1] = D.env[1]; f.env[2] = D.env[2];
+        P += (int64_t)s->w * s->h;
+        max_depth = std::max(max_depth, s->max_depth);
+    }
+    rc = ensure_buf(ctx, &ctx->fses, &ctx->fses_cap, (int64_t)n);
+    if (rc) return rc;
+    Lane& L = ctx->lanes[0];
+    hipStream_t st = L.stream;
+    HIP_OK(hipMemcpyAsync(ctx->fses, hs.data(), hs.size() * sizeof(FusedSession), hipMemcpyHostToDevice, st));
+    // waves of whole samples: as in run_waves, at most 2^26 paths and a
+    // quarter of free memory
+    int64_t target = ctx->wave_paths;
+    if (target <= 0) {
+        size_t free_b = 0, total_b = 0;
+        target = (int64_t)1 << 26;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) {
+            const int64_t fit = (int64_t)(free_b / 4 / kWaveBytesPerPath) + L.cap_paths;
+            target = std::max<int64_t>(std::min<int64_t>(target, fit), (int64_t)1 << 16);
+        }
+    }
+    const int64_t B = std::max<int64_t>(1, std::min<int64_t>(spp, target / P));
+    if (P * B >= INT32_MAX) return fail(ctx, LUCID_ERR_INVALID, "fused pass: wave too large");
+    rc = ensure_wave(ctx, L, P * B);
+    if (rc) return rc;
+    WaveDev W = L.W;
+    W.sh_a = W.sh_e = W.sh_t = nullptr;
+    W.dv0 = W.dv1 = W.dr0 = W.dr1 = nullptr;
+    W.dcnt = nullptr;
+    W.dstride = 0;
+    W.gout = nullptr;
+    W.gspare = 0;
+    W.tslot = nullptr;
+    W.tstride = 1;
+    W.tnrec = 0;
+    W.stop = nullptr;
+    W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
+    FusedDev F;
+    F.ses = ctx->fses;
+    F.n_ses = n;
+    F.P = P;
+    const SceneDev& SD = A.dev;
+    const int sd = A.sd;
+    const size_t lds = (size_t)sd * kBlock * sizeof(int32_t);
+    const int64_t cap_trav = (int64_t)ctx->num_cu * ctx->traverse_groups;
+    const int64_t cap_shade = ctx->shade_groups > 0 ? (int64_t)ctx->num_cu * ctx->shade_groups : INT64_MAX;
+    const int32_t refill = ctx->refill, walk_min = ctx->walk_min, leaf_tris = ctx->leaf_tris;
+    const int32_t algo = ss[0]->algorithm;
+    int64_t done = 0;
+    uint64_t waves = 0;
+    for (int64_t s0 = 0; s0 < spp; s0 += B) {
+        if (waves >= 2) (void)hipEventSynchronize(L.wave_ev[waves & 1]);
+        if (ctx->cancel.load(std::memory_order_relaxed)) {
+            *cancelled = true;
+            break;
+        }
+        const int64_t b = std::min<int64_t>(B, spp - s0);
+        const int64_t np = P * b;
+        k_raygen_fused<<<blocks(np, kBlock), kBlock, 0, st>>>(W, F, (int32_t)s0, np);
+        int q = 0;
+        for (int depth = 0; depth < max_depth; ++depth) {
+            const unsigned g = (unsigned)std::min<int64_t>(blocks(np, kBlock), cap_trav);
+            const unsigned gs = (unsigned)std::min<int64_t>(blocks(np, kBlock), cap_shade);
+            int ks = kev_begin(ctx, 0, st);
+            k_trace_refill<0, true><<<g, kBlock, lds, st>>>(SD, W, q, sd, refill, walk_min, leaf_tris);
+            kev_end(ctx, ks, st);
+            ks = kev_begin(ctx, 1, st);
+            if (algo == LUCID_ALGO_MIS)
+                launch_shade_fused<LUCID_ALGO_MIS>(gs, st, SD, W, F, q);
+            else if (algo == LUCID_ALGO_NEE)
+                launch_shade_fused<LUCID_ALGO_NEE>(gs, st, SD, W, F, q);
+            else
+                launch_shade_fused<LUCID_ALGO_SIMPLE>(gs, st, SD, W, F, q);
+            kev_end(ctx, ks, st);
+            ks = kev_begin(ctx, 2, st);
+            k_trace_refill<1, true><<<g, kBlock, lds, st>>>(SD, W, 0, sd, refill, walk_min, leaf_tris);
+            kev_end(ctx, ks, st);
+            q = 1 - q;
+        }
+        k_film_fused<<<blocks(P, 256), 256, 0, st>>>(W, F, (int32_t)b);
+        (void)hipEventRecord(L.wave_ev[waves & 1], st);
+        ++waves;
+        done += b;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, LUCID_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    for (int32_t i = 0; i < n; ++i) ss[i]->samples += done;
+    ctx->stats.paths += (uint64_t)P * (uint64_t)done;
+    ctx->stats.waves += waves;
+    return LUCID_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -4451,6 +4883,22 @@ int lucid_sessions_render(lucid_ctx* ctx, lucid_session* const* sessions, int32_
     ctx->stats = lucid_stats{};
     ctx->kev_used = 0;
     HIP_OK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (fusable(ctx, sessions, n)) {
+        // the sessions' graphs describe lane passes: a fused pass makes them stale
+        for (int32_t i = 0; i < n; ++i) drop_graph(sessions[i]);
+        rc = ensure_counters(ctx, ctx->lanes[0]);
+        if (rc) return rc;
+        k_zero_counters<<<1, 64, 0, ctx->stream>>>(ctx->lanes[0].W.cnt, ctx->lanes[0].W.rays);
+        bool cancelled = false;
+        rc = fused_sessions_pass(ctx, sessions, n, samples, &cancelled);
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc)
+            rc = fail(ctx, LUCID_ERR_DEVICE, "fused session pass: synchronize failed");
+        if (rc) return rc;
+        rc = finish_stats(ctx, 1);
+        if (rc) return rc;
+        return cancelled ? LUCID_ERR_CANCELLED : LUCID_OK;
+    }
     for (int k = 0; k < nl; ++k) {
         Lane& L = ctx->lanes[k];
         rc = ensure_counters(ctx, L);

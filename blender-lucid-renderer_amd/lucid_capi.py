@@ -141,6 +141,7 @@ SIGNATURES = {
     "lucid_set_ray_order": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_shade_order": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_set_session_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "lucid_set_session_fusion": (C.c_int, [C.c_void_p, C.c_int32]),
     "lucid_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Camera), C.POINTER(RenderParams),
                                  C.POINTER(LaunchShape)]),
     "lucid_set_diagnostics": (C.c_int, [C.c_void_p, C.POINTER(DiagConfig)]),

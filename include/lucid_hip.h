@@ -340,6 +340,19 @@ int lucid_set_shade_order(lucid_ctx* ctx, int32_t order);
    launches every kernel on its own.  Results do not depend on it. */
 int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on);
 
+/* Fused session passes (no counterpart in the reference, which renders one
+   session at a time behind gpu_mutex_, include/pybind_renderer.hpp:1118):
+   with 1 (default) a lucid_sessions_render call whose sessions all use the
+   same surface algorithm (MIS, NEE or simple) on scenes with geometry and
+   without participating media or volume-boundary meshes runs as one fused
+   pass: the sessions' scenes merged into one set of device arrays (cached
+   for the next pass over the same scenes), every bounce one traversal, one
+   shading and one shadow launch over the paths of all sessions.  Other
+   batches, and every batch with 0, spread their sessions over the stream
+   lanes.  Results do not depend on it: each session's sums equal those of
+   its own render_tile bit for bit. */
+int lucid_set_session_fusion(lucid_ctx* ctx, int32_t enabled);
+
 /* Ray order (no counterpart in the reference): 1 regroups, from the second
    bounce on, the bounce rays of each depth by direction octant and origin
    cell (8x8x8 over the scene box) before the closest-hit pass, and the

@@ -108,6 +108,47 @@ def test_batched_sessions_equal_their_own_renders(ctx):
         ctx.L.lucid_scene_release(sh)
 
 
+@pytest.mark.parametrize("algo", ["mis", "nee", "simple"])
+def test_fused_pass_equals_lanes_and_render_tile(ctx, algo):
+    """Sessions of one algorithm on different scenes (one scene shared by two
+    sessions), sizes and depths: a fused pass (one launch per kernel and
+    bounce over all of them) and the stream-lane batch both leave every
+    session with the sums of its own render_tile, bit for bit, over passes
+    of several sizes (a pass of 7 samples spans two waves)."""
+    cases = []
+    scenes = {}
+    for i, (sc, cam, w, h) in enumerate(workloads.batched_sessions(seed=5, count=6, tri_range=(2e3, 3e4),
+                                                                    edge_range=(16, 80))):
+        key = i if i != 5 else 1  # sessions 1 and 5 render the same scene
+        if key not in scenes:
+            scenes[key] = (ctx.scene(sc), lucid_capi.camera(cam))
+        sh, c = scenes[key]
+        depth = (8, 5, 12, 3, 8, 6)[i]
+        cases.append((sh, c, w, h, depth))
+    P = sum(w * h for _, _, w, h, _ in cases)
+    for fused in (1, 0):
+        ctx.check(ctx.L.lucid_set_session_fusion(ctx.h, fused))
+        if fused:  # waves of 3 samples: the 7-sample pass takes three
+            ctx.check(ctx.L.lucid_set_wave_paths(ctx.h, 3 * P))
+        sessions = [lucid_capi.Session(ctx, sh, c, w, h, depth, algo) for sh, c, w, h, depth in cases]
+        for spp in (4, 7, 1):
+            lucid_capi.render_sessions(ctx, sessions, spp)
+            st = ctx.stats()
+            assert st["paths"] == P * spp and st["rays_closest"] >= P * spp
+        ctx.check(ctx.L.lucid_set_wave_paths(ctx.h, 0))
+        for (sh, c, w, h, depth), s in zip(cases, sessions):
+            assert s.samples == 12
+            got = s.read(average=False)
+            ref = _render_tile(ctx, sh, c, w, h, 12, depth, algo)
+            assert np.array_equal(got, ref), (fused, w, h, np.abs(got - ref).max())
+            assert got[0::4].sum() > 0
+        for s in sessions:
+            s.close()
+    ctx.check(ctx.L.lucid_set_session_fusion(ctx.h, 1))
+    for sh, _ in scenes.values():
+        ctx.L.lucid_scene_release(sh)
+
+
 def test_session_set_camera_clears(ctx):
     scene = workloads.closed_box()
     sh = ctx.scene(scene)
