@@ -120,10 +120,18 @@ struct WaveDev {
     // them agree: a path ends early only if the film drops its wave.
     const int32_t* stop;
 };
-__device__ __forceinline__ void poll_stop(const WaveDev& W) {
-    if (W.stop && *reinterpret_cast<const volatile int32_t*>(W.stop) != 0) W.cnt[7] = 1;
+// The latch is written and read with device-scope atomics: a plain store
+// can sit in the writer's L2 and a plain load of this uniform address
+// compiles to a scalar-cache load, and neither cache is coherent with the
+// CUs of the other XCDs that run the later kernels.
+__device__ __forceinline__ int32_t latch_read(const int32_t* w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ bool stop_latched(const WaveDev& W) { return W.stop && W.cnt[7] != 0; }
+__device__ __forceinline__ void poll_stop(const WaveDev& W) {
+    if (W.stop && *reinterpret_cast<const volatile int32_t*>(W.stop) != 0)
+        __hip_atomic_store(&W.cnt[7], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool stop_latched(const WaveDev& W) { return W.stop && latch_read(&W.cnt[7]) != 0; }
 constexpr int kTrainSlots = 6;
 // Paths that keep training slots: one per block of `tstride` consecutive
 // paths, at a hashed offset so that every pixel column contributes.
@@ -232,7 +240,14 @@ LUCID_HD float prune_bound(float best) { return best + smax(best * 0.01f, 1e-6f)
 struct TraceHit {
     float t, u, v;
     int32_t prim;
+    int32_t mesh;  // of `prim`: the .w of its first record, which the test loads anyway
 };
+// the bits of a float as an int, on the host and the device
+LUCID_HD int32_t float_bits(float x) {
+    int32_t i;
+    __builtin_memcpy(&i, &x, 4);
+    return i;
+}
 
 // Slab test of one child box given as scalars (no local arrays, so nothing
 // is spilled to scratch).
@@ -363,11 +378,11 @@ LUCID_HD bool leaf_visit(const SceneDev& S, v3f o, v3f d, float tmin, uint32_t& 
         const float4* tp = S.tri + 3 * (size_t)prim;
         if (tri_test(o, d, tp[0], tp[1], tp[2], tmin, h.t, t, u, v)) {
             if (ANY) {
-                h.t = t; h.u = u; h.v = v; h.prim = prim;
+                h.t = t; h.u = u; h.v = v; h.prim = prim; h.mesh = float_bits(tp[0].w);
                 return true;
             }
             if (t < h.t || prim > h.prim) {
-                h.t = t; h.u = u; h.v = v; h.prim = prim;
+                h.t = t; h.u = u; h.v = v; h.prim = prim; h.mesh = float_bits(tp[0].w);
             }
         }
     }
@@ -408,7 +423,7 @@ __device__ __forceinline__ TraceHit closest_real(const SceneDev& S, v3f o, v3f d
     float smin_ = tmin;
     for (int peek = 0; peek < 8; ++peek) {
         if (h.prim < 0) return h;
-        if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return h;
+        if (!(S.mats[h.mesh].flags & MF_VOLUME_BOUNDARY)) return h;
         smin_ = h.t + 1e-3f;
         traverse<false>(S, o, d, smin_, RAY_T_MAX, stk, sd, ovf, nn, nt, h);
     }
@@ -427,7 +442,7 @@ __device__ __forceinline__ bool shadowed(const SceneDev& S, v3f o, v3f d, float 
         if (remaining <= 0.0f) return false;
         traverse<false>(S, po, d, RAY_T_MIN, remaining, stk, sd, ovf, nn, nt, h);
         if (h.prim < 0) return false;
-        if (!(S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) return true;
+        if (!(S.mats[h.mesh].flags & MF_VOLUME_BOUNDARY)) return true;
         float skip = h.t + 1e-3f;
         toff += skip;
         po = po + d * skip;
@@ -727,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             // intersect_skipping_volume_boundaries: re-query past a
             // boundary-only mesh; the result of an 8th re-query is discarded
             // by the reference, so a 7th boundary hit ends with no hit
-            if (S.any_vbo && h.prim >= 0 && (S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) {
+            if (S.any_vbo && h.prim >= 0 && (S.mats[h.mesh].flags & MF_VOLUME_BOUNDARY)) {
                 if (requery < 7) {
                     ++requery;
                     tmin = h.t + 1e-3f;
@@ -747,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
             // the remaining distance occludes; a boundary-only one moves the
             // origin 1e-3 past its hit, at most 8 queries in all
             bool unoccluded = h.prim < 0;
-            if (!unoccluded && (S.mats[prim_mesh(S, h.prim)].flags & MF_VOLUME_BOUNDARY)) {
+            if (!unoccluded && (S.mats[h.mesh].flags & MF_VOLUME_BOUNDARY)) {
                 const float skip = h.t + 1e-3f;
                 toff += skip;
                 o = o + d * skip;
@@ -2152,7 +2167,7 @@ __global__ __launch_bounds__(256) void k_output(const float4* film, float* out, 
     if (pix >= tw * th) return;
     // a cancelled render returns its partial sums in row order, unflipped
     // (pybind_renderer.hpp:225-233 returns before the flip of :351-360)
-    if (latch && *latch) flip = 0;
+    if (latch && latch_read(latch)) flip = 0;
     int px = pix % tw, py = pix / tw;
     float4 a = film[pix];
     float r = smax(a.x * sens, 0.0f), g = smax(a.y * sens, 0.0f), b = smax(a.z * sens, 0.0f);

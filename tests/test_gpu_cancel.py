@@ -64,15 +64,21 @@ def test_cancel_from_another_thread_returns_completed_batches_unflipped():
     assert got["t"] < 0.8 * t_full, (got["t"], t_full)
     part = got["img"]
     assert np.all(part[..., 3] == 1.0)
-    # find k: the number of batches in the buffer (sums grow with every batch)
-    total = float(part[..., :3].sum())
+    # find k: the number of batches in the buffer (sums grow with every batch);
+    # the sums are taken over arrays of one layout (the partial buffer turned
+    # to the row order of a full render), so that equal buffers give equal sums
+    # (a float sum depends on the order of its terms)
+    def image_sum(a):
+        return float(np.ascontiguousarray(a[..., :3], dtype=np.float64).sum())
+
+    total = image_sum(part[::-1])
     assert total > 0.0
     lo, hi = 1, TOTAL // BATCH - 1
     while lo < hi:
         k = (lo + hi) // 2
         rc, img = _render(ctx, sh, camd, BATCH * k)
         ctx.check(rc)
-        if float(img[..., :3].sum()) < total:
+        if image_sum(img) < total:
             lo = k + 1
         else:
             hi = k
