@@ -46,7 +46,34 @@ namespace lucid {
 constexpr int kBlock = 128;      // 2 wavefronts per workgroup
 constexpr int kStackMax = 64;    // LDS traversal stack entries per lane, upper bound
 constexpr int kMinRaysPerLane = 8;  // persistent traversal: grid shrinks for small waves
-constexpr int64_t kWaveBytesPerPath = 10 * 16 + 5 * 4;  // WaveDev SoA bytes per path slot
+constexpr int64_t kWaveBytesPerPath = 10 * 16 + 7 * 4;  // WaveDev SoA bytes per path slot
+
+// Queue counters.  Device-scope atomics on one 128-byte line serialize (about
+// 11 ns each on MI355X, measured with 4,096 wavefronts; 64 lines take 0.25 ns
+// per atomic in aggregate), and a pass issues millions of them: the queue
+// appends of every shading wavefront and the ray fetches of the persistent
+// traversal.  So each launch spreads them over kShards counters with a line
+// each.  The counter block (cnt):
+//   [0], [1]  live-queue counts      [2] shadow-queue count
+//   [3]       stack overflows        [7] cancel latch      bytes 64..127: ray counters
+//   cnt[kFetchBase + (kind * kShards + s) * kLineInts]: fetch counter of range s
+//             of a persistent launch's queue (kind 0 closest hit, 1 shadow, 2 volume march)
+//   cnt[kAppendBase + (w * kShards + s) * kLineInts]: entries in segment s of
+//             the next live queue (w 0) or of the shadow queue (w 1)
+constexpr int kShardBits = 6;
+constexpr int kShards = 1 << kShardBits;  // one per lane of a wavefront
+constexpr int kLineInts = 32;             // 128 bytes
+constexpr int kFetchBase = 32;
+constexpr int kAppendBase = kFetchBase + 3 * kShards * kLineInts;
+constexpr int kCounterInts = kAppendBase + 2 * kShards * kLineInts;
+constexpr int64_t kCounterBytes = (int64_t)kCounterInts * 4;
+// entries per append segment of a wave of n paths: a shading launch appends
+// the entries of input chunk c (kBlock consecutive queue positions) to
+// segment c % kShards, at most one per position and queue
+inline int64_t segment_capacity(int64_t n) {
+    const int64_t chunks = (n + kBlock - 1) / kBlock;
+    return (chunks + kShards - 1) / kShards * kBlock;
+}
 
 // ------------------------------------------------------------- device scene --
 struct SceneDev {
@@ -97,7 +124,13 @@ struct WaveDev {
     int32_t* sq;     // the live queue regrouped by ray bin (lucid_set_ray_order)
     uint32_t* skey;  // bin of queue entry i
     uint32_t* sbins; // kRayBins counters, then start offsets
-    int32_t* cnt;    // [0],[1] queue counts, [2] shadow count, [3] stack overflows
+    // append segments of the next live queue (aq) and of the shadow queue
+    // (aqs): kShards segments of seg_cap entries, concatenated into q / qs
+    // by k_queue_concat after each shading launch
+    int32_t* aq;
+    int32_t* aqs;
+    int64_t seg_cap;
+    int32_t* cnt;    // counter block: queue counts, fetch and append counters (kCounterInts)
     unsigned long long* rays;  // [0] closest, [1] shadow, [2..3] closest nodes/tris, [4..5] shadow nodes/tris
     int32_t count_work;        // accumulate node / triangle fetch counts
     float4 *dv0, *dv1, *dr0, *dr1;
@@ -176,6 +209,65 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
     base = __shfl(base, leader);
     int rank = __popcll(mask & ((1ull << lane_id()) - 1ull));
     return base + rank;
+}
+
+// ---- sharded queue counters (see kShards) ----
+// Persistent launches hand out the positions [0, n) of their input queue in
+// kShards contiguous ranges, range s = [range_lo(n, s), range_lo(n, s + 1)),
+// each with its own fetch counter.  A wavefront fetches from one range until
+// it is handed out, then moves on to the next range that has positions left.
+__device__ __forceinline__ int range_lo(int n, int s) { return (int)(((int64_t)n * s) >> kShardBits); }
+__device__ __forceinline__ int32_t* fetch_counters(const WaveDev& W, int kind) {
+    return W.cnt + kFetchBase + kind * kShards * kLineInts;
+}
+struct RangeFetch {
+    int cur;                     // range the wavefront fetches from (uniform)
+    unsigned long long drained;  // ranges it has seen handed out
+};
+__device__ __forceinline__ RangeFetch first_range() {
+    RangeFetch f;
+    f.cur = (int)((blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) & (kShards - 1));
+    f.drained = 0ull;
+    return f;
+}
+// Range f.cur is handed out: move to the first range after it (cyclically)
+// that still has positions, reading all kShards fetch counters at once (one
+// per lane).  Returns true when no range has any left.  Every lane must call
+// it.  A range is left at most once, so a wavefront's fetches end.
+__device__ __forceinline__ bool next_range(const int32_t* fetch, int n, RangeFetch& f) {
+    f.drained |= 1ull << f.cur;
+    const int s = lane_id();
+    const int len = range_lo(n, s + 1) - range_lo(n, s);
+    const int taken = __hip_atomic_load(fetch + s * kLineInts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long avail = __ballot(taken < len) & ~f.drained;
+    if (avail == 0) return true;
+    const unsigned long long after = avail & (~0ull << f.cur);
+    f.cur = __ffsll((long long)(after ? after : avail)) - 1;
+    return false;
+}
+// One wavefront fetch of up to `want` positions (the lanes of `idle` that
+// ask); lane k of `idle` (in lane order) gets position *lo + base + k when
+// base + k < len.  Every lane must call it.
+__device__ __forceinline__ int range_fetch(int32_t* fetch, int n, RangeFetch& f, unsigned long long idle, int want,
+                                           bool& exhausted, int& lo, int& len) {
+    const int leader = __ffsll((long long)idle) - 1;
+    lo = range_lo(n, f.cur);
+    len = range_lo(n, f.cur + 1) - lo;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(fetch + f.cur * kLineInts, want);
+    base = __shfl(base, leader);
+    if (base + want >= len) exhausted = next_range(fetch, n, f);
+    return base;
+}
+// Zeroes the fetch counters of the next persistent launches (called by one
+// workgroup of the launch before them).
+__device__ __forceinline__ void reset_fetch(const WaveDev& W) {
+    for (int j = threadIdx.x; j < 3 * kShards; j += blockDim.x) W.cnt[kFetchBase + j * kLineInts] = 0;
+}
+// Zeroes the append counters of the next shading launch (one workgroup of
+// the closest-hit launch before it).
+__device__ __forceinline__ void reset_append(const WaveDev& W) {
+    for (int j = threadIdx.x; j < 2 * kShards; j += blockDim.x) W.cnt[kAppendBase + j * kLineInts] = 0;
 }
 
 // Same for a 64-bit running total whose positions wrap around a ring of
@@ -584,7 +676,8 @@ __device__ __forceinline__ void raygen_path(const WaveDev& W, const CamDev& c, i
 }
 __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t n_paths) {
     int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (p == 0) { W.cnt[0] = n_paths; W.cnt[1] = 0; W.cnt[2] = 0; }
+    if (blockIdx.x == 0) reset_fetch(W);
     if (p >= n_paths) return;
     int b = p / c.P, pix = p - b * c.P;
     raygen_path(W, c, p, pix, (c.s0p ? *c.s0p : c.s0) + b, 0, 0.0f);
@@ -594,6 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(WaveDev W, CamDev c, int32_t 
 __global__ __launch_bounds__(kBlock) void k_intersect(SceneDev S, WaveDev W, int qin, int sd) {
     extern __shared__ int32_t lds_stack[];
     const int n = W.cnt[qin];
+    if (blockIdx.x == 0) reset_append(W);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         W.cnt[1 - qin] = 0;
         W.cnt[2] = 0;
@@ -646,7 +740,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
     extern __shared__ int32_t lds_stack[];
     int32_t* stk = lds_stack + threadIdx.x;
     const int n = SHADOW ? W.cnt[2] : W.cnt[qin];
-    int32_t* fetch = &W.cnt[SHADOW ? 5 : 4];
+    int32_t* fetch = fetch_counters(W, SHADOW ? 1 : 0);
+    RangeFetch rf = first_range();
+    if (!SHADOW && blockIdx.x == 0) reset_append(W);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (SHADOW) {
             atomicAdd(&W.rays[1], (unsigned long long)n);
@@ -670,14 +766,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
         const unsigned long long im = __ballot(p < 0);
         const int nidle = __popcll(im);
         if (!exhausted && (nidle >= refill || nidle == 64)) {
-            const int leader = __ffsll((long long)im) - 1;
-            int base = 0;
-            if (lane_id() == leader) base = atomicAdd(fetch, nidle);
-            base = __shfl(base, leader);
-            exhausted = base + nidle >= n;
+            int lo, len;
+            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, lo, len);
             if (p < 0) {
-                const int i = base + __popcll(im & ((1ull << lane_id()) - 1ull));
-                if (i < n) {
+                const int j = base + __popcll(im & ((1ull << lane_id()) - 1ull));
+                if (j < len) {
+                    const int i = lo + j;
                     if (SHADOW) {
                         p = W.qs[i];
                         const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
@@ -796,7 +890,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
 constexpr int kMarchChunk = 8;
 __global__ __launch_bounds__(kBlock) void k_vol_march(SceneDev S, WaveDev W, int refill) {
     const int n = W.cnt[2];
-    int32_t* fetch = &W.cnt[6];
+    int32_t* fetch = fetch_counters(W, 2);
+    RangeFetch rf = first_range();
     int p = -1;
     bool exhausted = false;
     v3f o = mk(0.0f, 0.0f, 0.0f), d = o;
@@ -807,15 +902,12 @@ __global__ __launch_bounds__(kBlock) void k_vol_march(SceneDev S, WaveDev W, int
         const unsigned long long im = __ballot(p < 0);
         const int nidle = __popcll(im);
         if (!exhausted && (nidle >= refill || nidle == 64)) {
-            const int leader = __ffsll((long long)im) - 1;
-            int base = 0;
-            if (lane_id() == leader) base = atomicAdd(fetch, nidle);
-            base = __shfl(base, leader);
-            exhausted = base + nidle >= n;
+            int lo, len;
+            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, lo, len);
             if (p < 0) {
-                const int q = base + __popcll(im & ((1ull << lane_id()) - 1ull));
-                if (q < n) {
-                    p = W.qs[q];
+                const int j = base + __popcll(im & ((1ull << lane_id()) - 1ull));
+                if (j < len) {
+                    p = W.qs[lo + j];
                     const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
                     o = mk(o4); d = mk(d4);
                     tmax = o4.w;
@@ -1431,10 +1523,13 @@ __device__ __forceinline__ void shade_one(const SceneDev& S, const WaveDev& W, c
     // a cancelled render ends its paths here (no next bounce, no shadow ray)
     cont = cont && !stop;
     nee = nee && !stop;
-    int slot = wave_append(&W.cnt[1 - qin], cont);
-    if (cont) W.q[1 - qin][slot] = p;
-    int sslot = wave_append(&W.cnt[2], nee);
-    if (nee) W.qs[sslot] = p;
+    // into the segments of this input chunk (the same for the whole
+    // workgroup: i = chunk * kBlock + threadIdx.x); k_queue_concat joins them
+    const int seg = (i / kBlock) & (kShards - 1);
+    int slot = wave_append(W.cnt + kAppendBase + seg * kLineInts, cont);
+    if (cont) W.aq[seg * W.seg_cap + slot] = p;
+    int sslot = wave_append(W.cnt + kAppendBase + (kShards + seg) * kLineInts, nee);
+    if (nee) W.aqs[seg * W.seg_cap + sslot] = p;
 }
 
 // Grid-stride over the live queue; the trip count is uniform per workgroup so
@@ -1444,7 +1539,7 @@ __device__ __forceinline__ void shade_launch(const SceneDev& S, const WaveDev& W
                                              int32_t max_depth) {
     const int n = W.cnt[qin];
     // ray fetch counters of the next persistent traversal launches
-    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (blockIdx.x == 0) reset_fetch(W);
     // the cancel latch of this bounce (set by its closest-hit launch, not
     // written during this one): uniform over the launch, so the ballots in
     // shade_one stay convergent and every wavefront takes the same decision
@@ -1457,6 +1552,39 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
                                                   int32_t max_depth) {
     shade_launch<ALGO, NODES>(S, W, G, qin, max_depth);
 }
+
+// Joins the append segments of a shading launch into the dense next live
+// queue q[qout] (blockIdx.y 0) and shadow queue qs (blockIdx.y 1), and
+// writes their counts.  Workgroup (s, r) copies slice r of segment s to the
+// segment's offset, the sum of the counts before it.
+constexpr int kConcatSlices = 16;
+__global__ __launch_bounds__(256) void k_queue_concat(WaveDev W, int qout) {
+    __shared__ int s_off, s_cnt, s_total;
+    const int w = blockIdx.y;
+    const int s = blockIdx.x / kConcatSlices, r = blockIdx.x % kConcatSlices;
+    if (threadIdx.x < 64) {
+        // lane l: count of segment l (written by the atomics of the shading launch)
+        const int c = __hip_atomic_load(W.cnt + kAppendBase + (w * kShards + (int)threadIdx.x) * kLineInts,
+                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int x = c;  // inclusive prefix sum over the wavefront
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o);
+            if ((int)threadIdx.x >= o) x += y;
+        }
+        if ((int)threadIdx.x == s) { s_off = x - c; s_cnt = c; }
+        if (threadIdx.x == 63) s_total = x;
+    }
+    __syncthreads();
+    const int32_t* src = (w ? W.aqs : W.aq) + (int64_t)s * W.seg_cap;
+    int32_t* dst = (w ? W.qs : W.q[qout]) + s_off;
+    const int lo = (int)((int64_t)s_cnt * r / kConcatSlices), hi = (int)((int64_t)s_cnt * (r + 1) / kConcatSlices);
+    for (int j = lo + (int)threadIdx.x; j < hi; j += 256) dst[j] = src[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) W.cnt[w ? 2 : qout] = s_total;
+}
+inline void concat_queues(const WaveDev& W, int qout, hipStream_t st) {
+    k_queue_concat<<<dim3(kShards * kConcatSlices, 2), 256, 0, st>>>(W, qout);
+}
+static_assert(kShards == 64, "one shard per lane of a wavefront");
 
 template <int ALGO>
 void launch_shade(unsigned gs, hipStream_t st, const SceneDev& S, const WaveDev& W, const guide::GuideDev& G,
@@ -1858,7 +1986,8 @@ __device__ __forceinline__ SceneDev fused_view(const SceneDev& A, const FusedSes
 // wave slot p = b * P + pixel: sample b of every pixel of every session
 __global__ __launch_bounds__(kBlock) void k_raygen_fused(WaveDev W, FusedDev F, int32_t s_base, int64_t n_paths) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (p == 0) { W.cnt[0] = (int32_t)n_paths; W.cnt[1] = 0; W.cnt[2] = 0; W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (p == 0) { W.cnt[0] = (int32_t)n_paths; W.cnt[1] = 0; W.cnt[2] = 0; }
+    if (blockIdx.x == 0) reset_fetch(W);
     if (p >= n_paths) return;
     const int64_t b = p / F.P, pix = p - b * F.P;
     const int sid = fused_session_of(F, pix);
@@ -1868,7 +1997,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen_fused(WaveDev W, FusedDev F, 
 template <int ALGO, bool NODES>
 __global__ __launch_bounds__(kBlock) void k_shade_fused(SceneDev A, WaveDev W, FusedDev F, int qin) {
     const int n = W.cnt[qin];
-    if (blockIdx.x == 0 && threadIdx.x == 0) { W.cnt[4] = 0; W.cnt[5] = 0; W.cnt[6] = 0; }
+    if (blockIdx.x == 0) reset_fetch(W);
     guide::GuideDev G;
     __builtin_memset(&G, 0, sizeof(G));
     for (int base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -2636,11 +2765,11 @@ void free_wave(Lane& L) {
 
 int ensure_counters(lucid_ctx* ctx, Lane& L) {
     if (L.ctr) return LUCID_OK;
-    if (hipMalloc(&L.ctr, 128) != hipSuccess) {
+    if (hipMalloc(&L.ctr, kCounterBytes) != hipSuccess) {
         L.ctr = nullptr;
         return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (counters)");
     }
-    HIP_OK(hipMemset(L.ctr, 0, 128));
+    HIP_OK(hipMemset(L.ctr, 0, kCounterBytes));
     L.W.cnt = static_cast<int32_t*>(L.ctr);
     L.W.rays = reinterpret_cast<unsigned long long*>(static_cast<char*>(L.ctr) + 64);
     return LUCID_OK;
@@ -2666,11 +2795,13 @@ int ensure_wave(lucid_ctx* ctx, Lane& L, int64_t n) {
               alloc(n * 16, (void**)&W.hit) && alloc(n * 16, (void**)&W.sh_o) && alloc(n * 16, (void**)&W.sh_d) &&
               alloc(n * 16, (void**)&W.sh_c) && alloc(n * 4, (void**)&W.q[0]) && alloc(n * 4, (void**)&W.q[1]) &&
               alloc(n * 4, (void**)&W.qs) && alloc(n * 4, (void**)&W.sq) && alloc(n * 4, (void**)&W.skey) &&
-              alloc(kRayBins * 4, (void**)&W.sbins);
+              alloc(kShards * segment_capacity(n) * 4, (void**)&W.aq) &&
+              alloc(kShards * segment_capacity(n) * 4, (void**)&W.aqs) && alloc(kRayBins * 4, (void**)&W.sbins);
     if (!ok) {
         free_wave(L);
         return fail(ctx, LUCID_ERR_NOMEM, "hipMalloc failed (wave buffers)");
     }
+    W.seg_cap = segment_capacity(n);
     L.cap_paths = n;
     return LUCID_OK;
 }
@@ -2776,6 +2907,11 @@ __global__ void k_zero_u64(unsigned long long* p) {
 __global__ void k_zero_counters(int32_t* cnt, unsigned long long* rays) {
     if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
     if (threadIdx.x < 8) rays[threadIdx.x] = 0ull;
+    for (int j = threadIdx.x; j < 5 * kShards; j += blockDim.x) cnt[kFetchBase + j * kLineInts] = 0;
+}
+// zeroes the closest-hit fetch counters (a second query over the same queue)
+__global__ void k_zero_fetch(int32_t* cnt) {
+    for (int j = threadIdx.x; j < kShards; j += blockDim.x) cnt[kFetchBase + j * kLineInts] = 0;
 }
 
 inline unsigned blocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -3111,6 +3247,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                     launch_shade<LUCID_ALGO_NEE>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
             }
+            concat_queues(W, 1 - q, st);
             if (march) {
                 const unsigned gm = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
                 k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, refill_lanes);
@@ -3504,12 +3641,14 @@ int lucid_render_tile_device_noalloc(lucid_ctx* ctx, lucid_scene* scene, const l
     const bool vol = pr->algorithm == LUCID_ALGO_VOLUME_MIS;
     const int64_t per_path = kWaveBytesPerPath + (vol ? 48 : 0);
     const auto up = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
-    const int64_t head = 256 + up((int64_t)kRayBins * 4) + up(P * 16);
-    const int64_t slack = 18 * 256;  // alignment of the path arrays
+    // the append segments hold up to kShards * kBlock entries more than the paths
+    const int64_t head = up(kCounterBytes) + up((int64_t)kRayBins * 4) + up(P * 16) + 2 * up(kShards * kBlock * 4);
+    const int64_t slack = 20 * 256;  // alignment of the path arrays
     const int64_t avail = (int64_t)kScratchBytes - head - slack;
     const int64_t cap = avail > 0 ? avail / per_path : 0;
     if (cap < P) {
-        const int64_t max_px = ((int64_t)kScratchBytes - 256 - up((int64_t)kRayBins * 4) - slack) / (per_path + 16);
+        const int64_t max_px = ((int64_t)kScratchBytes - up(kCounterBytes) - up((int64_t)kRayBins * 4) -
+                                2 * up(kShards * kBlock * 4) - slack) / (per_path + 16);
         return fail(ctx, LUCID_ERR_INVALID,
                     "tile of " + std::to_string(P) + " pixels exceeds the no-allocation scratch (" +
                         std::to_string(kScratchBytes >> 20) + " MiB: at most " + std::to_string(max_px) +
@@ -3524,7 +3663,7 @@ int lucid_render_tile_device_noalloc(lucid_ctx* ctx, lucid_scene* scene, const l
     };
     FixedWave fw{};
     WaveDev& W = fw.W;
-    W.cnt = static_cast<int32_t*>(take(256));
+    W.cnt = static_cast<int32_t*>(take(kCounterBytes));
     W.rays = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(W.cnt) + 64);
     W.sbins = static_cast<uint32_t*>(take((int64_t)kRayBins * 4));
     fw.film = static_cast<float4*>(take(P * 16));
@@ -3543,6 +3682,9 @@ int lucid_render_tile_device_noalloc(lucid_ctx* ctx, lucid_scene* scene, const l
     W.qs = static_cast<int32_t*>(take(cap * 4));
     W.sq = static_cast<int32_t*>(take(cap * 4));
     W.skey = static_cast<uint32_t*>(take(cap * 4));
+    W.seg_cap = segment_capacity(cap);
+    W.aq = static_cast<int32_t*>(take(kShards * W.seg_cap * 4));
+    W.aqs = static_cast<int32_t*>(take(kShards * W.seg_cap * 4));
     if (vol) {
         W.sh_a = static_cast<float4*>(take(cap * 16));
         W.sh_e = static_cast<float4*>(take(cap * 16));
@@ -3643,7 +3785,7 @@ int lucid_render_debug(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* c
     k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 0, sd, 48, ctx->walk_min, ctx->leaf_tris);
     if (mode == kDebugVolume) {
         k_debug_vol_prep<<<blocks(P, 256), 256, 0, st>>>(SD, W, (int32_t)P);
-        k_set_i32<<<1, 64, 0, st>>>(W.cnt + 4, 0);  // fetch counter of the second query
+        k_zero_fetch<<<1, 64, 0, st>>>(W.cnt);  // fetch counters of the second query
         k_trace_refill<0><<<g, kBlock, lds, st>>>(SD, W, 1, sd, 48, ctx->walk_min, ctx->leaf_tris);
     }
     k_debug_out<<<blocks(P, 256), 256, 0, st>>>(SD, W, ctx->out_dev, pr->tile_w, pr->tile_h, mode);
@@ -4762,6 +4904,7 @@ int fused_sessions_pass(lucid_ctx* ctx, lucid_session* const* ss, int32_t n, int
                 launch_shade_fused<LUCID_ALGO_NEE>(gs, st, SD, W, F, q);
             else
                 launch_shade_fused<LUCID_ALGO_SIMPLE>(gs, st, SD, W, F, q);
+            concat_queues(W, 1 - q, st);
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
             k_trace_refill<1, true><<<g, kBlock, lds, st>>>(SD, W, 0, sd, refill, walk_min, leaf_tris);
