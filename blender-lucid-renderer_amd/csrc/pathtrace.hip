@@ -212,11 +212,26 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
 }
 
 // ---- sharded queue counters (see kShards) ----
-// Persistent launches hand out the positions [0, n) of their input queue in
-// kShards contiguous ranges, range s = [range_lo(n, s), range_lo(n, s + 1)),
-// each with its own fetch counter.  A wavefront fetches from one range until
-// it is handed out, then moves on to the next range that has positions left.
-__device__ __forceinline__ int range_lo(int n, int s) { return (int)(((int64_t)n * s) >> kShardBits); }
+// Persistent launches hand out the positions [0, n) of their input queue
+// through kShards fetch counters.  The queue is cut into chunks of
+// kFetchChunk positions dealt round-robin to kShards ranges: range s holds
+// chunks s, s + kShards, s + 2 kShards, ... in that order.  All ranges
+// advance together, so the rays in flight stay near the front of the queue
+// (the order the queue was built in: neighbouring pixels, or paths queued
+// together) as with one counter, while each counter takes 1/kShards of the
+// atomics.  A wavefront fetches from one range until it is handed out, then
+// moves on to the next range that has positions left.
+constexpr int kFetchChunkBits = 8;
+constexpr int kFetchChunk = 1 << kFetchChunkBits;
+__device__ __forceinline__ int range_len(int n, int s) {
+    const int full = n >> (kFetchChunkBits + kShardBits);
+    const int rem = n - (full << (kFetchChunkBits + kShardBits));
+    return (full << kFetchChunkBits) + min(max(rem - (s << kFetchChunkBits), 0), kFetchChunk);
+}
+// queue position of entry j of range s
+__device__ __forceinline__ int range_pos(int s, int j) {
+    return ((j >> kFetchChunkBits) << (kFetchChunkBits + kShardBits)) + (s << kFetchChunkBits) + (j & (kFetchChunk - 1));
+}
 __device__ __forceinline__ int32_t* fetch_counters(const WaveDev& W, int kind) {
     return W.cnt + kFetchBase + kind * kShards * kLineInts;
 }
@@ -237,24 +252,23 @@ __device__ __forceinline__ RangeFetch first_range() {
 __device__ __forceinline__ bool next_range(const int32_t* fetch, int n, RangeFetch& f) {
     f.drained |= 1ull << f.cur;
     const int s = lane_id();
-    const int len = range_lo(n, s + 1) - range_lo(n, s);
     const int taken = __hip_atomic_load(fetch + s * kLineInts, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long avail = __ballot(taken < len) & ~f.drained;
+    const unsigned long long avail = __ballot(taken < range_len(n, s)) & ~f.drained;
     if (avail == 0) return true;
     const unsigned long long after = avail & (~0ull << f.cur);
     f.cur = __ffsll((long long)(after ? after : avail)) - 1;
     return false;
 }
-// One wavefront fetch of up to `want` positions (the lanes of `idle` that
-// ask); lane k of `idle` (in lane order) gets position *lo + base + k when
-// base + k < len.  Every lane must call it.
+// One wavefront fetch of `want` entries of range f.cur (for the lanes that
+// ask); entries base + k < len are valid (queue position range_pos(s, base + k)).
+// Every lane must call it.
 __device__ __forceinline__ int range_fetch(int32_t* fetch, int n, RangeFetch& f, unsigned long long idle, int want,
-                                           bool& exhausted, int& lo, int& len) {
+                                           bool& exhausted, int& s, int& len) {
     const int leader = __ffsll((long long)idle) - 1;
-    lo = range_lo(n, f.cur);
-    len = range_lo(n, f.cur + 1) - lo;
+    s = f.cur;
+    len = range_len(n, s);
     int base = 0;
-    if (lane_id() == leader) base = atomicAdd(fetch + f.cur * kLineInts, want);
+    if (lane_id() == leader) base = atomicAdd(fetch + s * kLineInts, want);
     base = __shfl(base, leader);
     if (base + want >= len) exhausted = next_range(fetch, n, f);
     return base;
@@ -766,12 +780,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_refill(SceneDev S, WaveDev W, 
         const unsigned long long im = __ballot(p < 0);
         const int nidle = __popcll(im);
         if (!exhausted && (nidle >= refill || nidle == 64)) {
-            int lo, len;
-            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, lo, len);
+            int rs, len;
+            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, rs, len);
             if (p < 0) {
                 const int j = base + __popcll(im & ((1ull << lane_id()) - 1ull));
                 if (j < len) {
-                    const int i = lo + j;
+                    const int i = range_pos(rs, j);
                     if (SHADOW) {
                         p = W.qs[i];
                         const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
@@ -902,12 +916,12 @@ __global__ __launch_bounds__(kBlock) void k_vol_march(SceneDev S, WaveDev W, int
         const unsigned long long im = __ballot(p < 0);
         const int nidle = __popcll(im);
         if (!exhausted && (nidle >= refill || nidle == 64)) {
-            int lo, len;
-            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, lo, len);
+            int rs, len;
+            const int base = range_fetch(fetch, n, rf, im, nidle, exhausted, rs, len);
             if (p < 0) {
                 const int j = base + __popcll(im & ((1ull << lane_id()) - 1ull));
                 if (j < len) {
-                    p = W.qs[lo + j];
+                    p = W.qs[range_pos(rs, j)];
                     const float4 o4 = W.sh_o[p], d4 = W.sh_d[p];
                     o = mk(o4); d = mk(d4);
                     tmax = o4.w;
@@ -1552,6 +1566,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, WaveDev W, guide::
                                                   int32_t max_depth) {
     shade_launch<ALGO, NODES>(S, W, G, qin, max_depth);
 }
+// The guided shading kernel, held to kGuidedWaves wavefronts per SIMD: its
+// register allocation would otherwise leave room for only two of them.
+constexpr int kGuidedWaves = 3;
+template <bool NODES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kGuidedWaves, kGuidedWaves))) void
+k_shade_guided(SceneDev S, WaveDev W, guide::GuideDev G, int qin, int32_t max_depth) {
+    shade_launch<LUCID_ALGO_GUIDED_MIS, NODES>(S, W, G, qin, max_depth);
+}
 
 // Joins the append segments of a shading launch into the dense next live
 // queue q[qout] (blockIdx.y 0) and shadow queue qs (blockIdx.y 1), and
@@ -1589,18 +1611,22 @@ static_assert(kShards == 64, "one shard per lane of a wavefront");
 template <int ALGO>
 void launch_shade(unsigned gs, hipStream_t st, const SceneDev& S, const WaveDev& W, const guide::GuideDev& G,
                   int qin, int32_t max_depth) {
-    if (S.prog)
-        k_shade<ALGO, true><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
-    else
-        k_shade<ALGO, false><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+    if constexpr (ALGO == LUCID_ALGO_GUIDED_MIS) {
+        if (S.prog)
+            k_shade_guided<true><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+        else
+            k_shade_guided<false><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+    } else {
+        if (S.prog)
+            k_shade<ALGO, true><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+        else
+            k_shade<ALGO, false><<<gs, kBlock, 0, st>>>(S, W, G, qin, max_depth);
+    }
 }
 
-// Shading workgroups per CU of the guided passes: the guided shading kernel
-// holds about 170 VGPRs, so two of its workgroups fit a CU at once, and it
-// runs best with twice that many (measured on the room at 1080p with a
-// trained network: 60.9 ms per pass at 4 groups per CU, 61.6 at 3, 69.3 at 2,
-// 70.1 at 8).
-constexpr int kGuidedShadeGroups = 4;
+// Shading workgroups per CU of the guided passes: as many as fit at
+// kGuidedWaves wavefronts per SIMD (4 SIMDs, 2 wavefronts per workgroup).
+constexpr int kGuidedShadeGroups = 2 * kGuidedWaves;
 
 // Surface point and shading normal of a closest hit (as the shade kernel
 // computes them).
@@ -1651,6 +1677,7 @@ constexpr int kInferTiles = 4;  // tiles of 16 samples per MLP pass
 // Wavefronts per workgroup: they share one LDS copy of the weights (31 KB);
 // with their rows a workgroup takes 68 KB, two per CU.
 constexpr int kInferWaves = 4;
+constexpr int kInferGroupsPerCU = 2;
 // A lane's LDS row (36 dwords, 144 bytes) holds, between trips, the staged
 // inputs of its next two trips, and during a trip its path's 64 BF16
 // features (dwords 0..31):
@@ -1861,33 +1888,52 @@ __global__ __launch_bounds__(256) void k_guide_collect(WaveDev W, int32_t n_path
     const int p = train_record_path(W, r);
     const bool live = p < n_paths;
     const float4 le = live ? W.rad[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // the samples of this lane's slots: slot k is kept when bit k is set
+    float wgt[kTrainSlots];
+    unsigned keep = 0;
+#pragma unroll
     for (int k = 0; k < kTrainSlots; ++k) {
-        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a, c = a, d = a, e = a;
-        if (live) {
-            a = *tslot_at(W, r, k, 0); b = *tslot_at(W, r, k, 1); c = *tslot_at(W, r, k, 2);
-            d = *tslot_at(W, r, k, 3); e = *tslot_at(W, r, k, 4);
-        }
-        float wgt = 0.0f;
+        wgt[k] = 0.0f;
+        if (!live) continue;
+        const float4 a = *tslot_at(W, r, k, 0), b = *tslot_at(W, r, k, 1), c = *tslot_at(W, r, k, 2);
+        const float4 d = *tslot_at(W, r, k, 3), e = *tslot_at(W, r, k, 4);
         if (d.w > 0.0f && e.w == 1.0f) {
             const float tr = a.w, tg = b.w, tb = c.w;
             const float cr = tr > 0.0f ? (le.x - e.x) / tr : 0.0f;
             const float cg = tg > 0.0f ? (le.y - e.y) / tg : 0.0f;
             const float cb = tb > 0.0f ? (le.z - e.z) / tb : 0.0f;
-            wgt = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
+            wgt[k] = 0.2126f * cr + 0.7152f * cg + 0.0722f * cb;
         }
-        const bool keep = wgt > 0.0f && isfinite(wgt);
-        // ring slots: one atomic per wavefront and bounce slot
-        const int slot = wave_append_u64(total, keep, capacity);
-        if (!keep) continue;
+        if (wgt[k] > 0.0f && isfinite(wgt[k])) keep |= 1u << k;
+    }
+    // ring positions for all kept samples of the wavefront: one atomic
+    // (lane-major: a lane's kept slots are consecutive)
+    const int mine = __popc(keep);
+    int x = mine;  // inclusive prefix sum over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane_id() >= o) x += y;
+    }
+    const int wave_total = __shfl(x, 63);
+    unsigned long long base = 0;
+    if (lane_id() == 0 && wave_total > 0) base = atomicAdd(total, (unsigned long long)wave_total);
+    base = __shfl(base, 0);
+    unsigned long long pos = base + (unsigned long long)(x - mine);
+#pragma unroll
+    for (int k = 0; k < kTrainSlots; ++k) {
+        if (!(keep & (1u << k))) continue;
+        const float4 a = *tslot_at(W, r, k, 0), b = *tslot_at(W, r, k, 1), c = *tslot_at(W, r, k, 2);
+        const float4 d = *tslot_at(W, r, k, 3);
         guide::Sample sm;
         sm.p[0] = a.x; sm.p[1] = a.y; sm.p[2] = a.z;
         sm.n[0] = b.x; sm.n[1] = b.y; sm.n[2] = b.z;
         sm.wo[0] = c.x; sm.wo[1] = c.y; sm.wo[2] = c.z;
         sm.wi[0] = d.x; sm.wi[1] = d.y; sm.wi[2] = d.z;
-        sm.weight = wgt;
+        sm.weight = wgt[k];
         sm.pbsdf = d.w;
         sm.pad[0] = sm.pad[1] = 0.0f;
-        ring[slot] = sm;
+        ring[(int)(pos % (unsigned long long)capacity)] = sm;
+        ++pos;
     }
 }
 
@@ -3103,11 +3149,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
     const int64_t cap_trav = refill ? std::min<int64_t>((int64_t)ctx->num_cu * tgroups,
                                                         blocks(P * B, kBlock * kMinRaysPerLane))
                                     : INT64_MAX;
-    int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
-    // guided passes shade with their own group count, whatever shape the
-    // plain kernels (which hardly depend on it) were tuned to
-    if (pr->algorithm == LUCID_ALGO_GUIDED_MIS && shade_groups > 0)
-        cap_shade = (int64_t)ctx->num_cu * kGuidedShadeGroups;
+    const int64_t cap_shade = shade_groups > 0 ? (int64_t)ctx->num_cu * shade_groups : INT64_MAX;
+    // the guided shading kernel runs with its own group count, whatever
+    // shape the plain kernels were tuned to
+    const int64_t cap_shade_guided = shade_groups > 0 ? (int64_t)ctx->num_cu * kGuidedShadeGroups : INT64_MAX;
     W.count_work = (ctx->profiling & LUCID_PROFILE_WORK) ? 1 : 0;
     W.stop = (mode == 0 && cancel_flag == &ctx->cancel && !capturing) ? ctx->cancel_dev : nullptr;
     if (mode != 3) {  // a session batch zeroes its lanes' counters once
@@ -3180,6 +3225,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         for (int depth = 0; depth < max_depth; ++depth) {
             const unsigned g = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_trav);
             const unsigned gs = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade);
+            const unsigned gsg = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade_guided);
             // the closest-hit pass reads the live queue through Wd
             WaveDev Wd = W;
             if (bin && depth > 0) {
@@ -3210,7 +3256,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             if (guide_here) {
                 ks = kev_begin(ctx, 3, st);
                 const unsigned gg =
-                    (unsigned)std::min<int64_t>(blocks(n, kInferWaves * 64), (int64_t)ctx->num_cu * 2);
+                    (unsigned)std::min<int64_t>(blocks(n, kInferWaves * 64), (int64_t)ctx->num_cu * kInferGroupsPerCU);
                 k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(SD, Wsh, G, q);
                 kev_end(ctx, ks, st);
             }
@@ -3229,9 +3275,21 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                     launch_shade<LUCID_ALGO_VOLUME_MIS>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
                 case LUCID_ALGO_GUIDED_MIS: {
+                    // Past the guided bounces (depth < gcfg.max_depth) a path
+                    // is a plain MIS path: the bounce after the last guided
+                    // one still weighs its emitter hits against that bounce's
+                    // guide density and closes its training slot, every later
+                    // one shades with the plain kernel, which draws the same
+                    // numbers and computes the same values as the guided
+                    // kernel with the guide off (rho = 1, no slots left open).
+                    const int gd = ctx->gcfg.max_depth;
+                    if (gd > 0 && depth > gd) {
+                        launch_shade<LUCID_ALGO_MIS>(gs, st, SD, Wsh, G, q, max_depth);
+                        break;
+                    }
                     guide::GuideDev Gd = G;
                     if (!guide_here) Gd.alpha = 0.0f;  // this bounce samples the BSDF only
-                    launch_shade<LUCID_ALGO_GUIDED_MIS>(gs, st, SD, Wsh, Gd, q, max_depth);
+                    launch_shade<LUCID_ALGO_GUIDED_MIS>(gsg, st, SD, Wsh, Gd, q, max_depth);
                     break;
                 }
                 case kAlgoDiagSimple:
@@ -4378,7 +4436,7 @@ int lucid_guiding_debug_render_infer(lucid_ctx* ctx, lucid_scene* scene, const f
     const guide::GuideDev G = guide::dev(ctx->gstate, scene->bmin, scene->bmax, ctx->gcfg.alpha,
                                          ctx->gcfg.learn_selection != 0);
     const unsigned gg = (unsigned)std::min<int64_t>(blocks(std::max<int32_t>(m, 1), kInferWaves * 64),
-                                                    (int64_t)ctx->num_cu * 2);
+                                                    (int64_t)ctx->num_cu * kInferGroupsPerCU);
     k_guide_infer<<<gg, kInferWaves * 64, 0, st>>>(scene->dev, W, G, 0);
     k_debug_frame<<<blocks(n, 256), 256, 0, st>>>(scene->dev, W, n);
     HIP_OK(hipGetLastError());

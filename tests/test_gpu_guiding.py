@@ -147,13 +147,15 @@ def test_online_training_lowers_loss(lucid):
     assert np.mean(losses[-3:]) < losses[0] - 0.1, losses
 
 
-@pytest.mark.parametrize("learn_selection", [False, True])
-def test_guided_is_unbiased(lucid, learn_selection):
+@pytest.mark.parametrize("learn_selection,guide_depth", [(False, 0), (True, 0), (True, 2)])
+def test_guided_is_unbiased(lucid, learn_selection, guide_depth):
+    """guide_depth 2: bounces 0 and 1 guided, bounce 2 weighs its emitter hits
+    against bounce 1's guide density, later bounces shade with the plain kernel."""
     W = H = 24
     n = 256
     g = _box_renderer(lucid, "guided")
     g.set_guiding(alpha=0.5, train_steps=16, batch=8192, learning_rate=5e-3, record_stride=1, seed=5,
-                  learn_selection=learn_selection)
+                  learn_selection=learn_selection, max_depth=guide_depth)
     for i in range(6):
         g.render_tile(0, 0, W, H, W, H, 16, 100000 + 16 * i, 8)
     sg = g.debug_render_samples(0, 0, W, H, W, H, n, 0, 8).astype(np.float64)
@@ -204,15 +206,19 @@ def test_guiding_gain_occluded_room(lucid):
     assert e_plain > 1.3 * e_guided
 
 
-def test_degenerate_guide_falls_back_to_plain(lucid):
+@pytest.mark.parametrize("guide_depth", [0, 2])
+def test_degenerate_guide_falls_back_to_plain(lucid, guide_depth):
     """A learned density that is not finite (here: weight logits beyond the
     FP16 range of the stored outputs) is not sampled: every guided vertex
     falls back to BSDF sampling and plain NEE weights, so the frozen guided
-    tracer renders exactly what plain MIS renders on the same seeds."""
+    tracer renders exactly what plain MIS renders on the same seeds.  With
+    guide_depth 2 the bounces past the guided ones run the plain shading
+    kernel: still the same image, bit for bit."""
     box = workloads.closed_box()
     cam = workloads.CLOSED_BOX_CAMERA
     g = make_renderer(lucid, box, cam, "guided")
-    g.set_guiding(alpha=0.7, train_steps=0, batch=4096, learning_rate=1e-3, record_stride=1, seed=4)
+    g.set_guiding(alpha=0.7, train_steps=0, batch=4096, learning_rate=1e-3, record_stride=1, seed=4,
+                  max_depth=guide_depth)
     params = np.asarray(g.get_guiding_params(), np.float32).copy()
     _, b3 = G.OFFS[3]
     for k in range(G.N_LOBES):
