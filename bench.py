@@ -400,7 +400,8 @@ def main():
     bytes_shadow = qs * (32 + 4) + p["nodes_shadow"] * 64 + p["tris_shadow"] * 48
     bytes_pass = bytes_closest + bytes_shade + bytes_shadow
     kernels = {
-        "k_shade": {"bound": "hbm", "limiter": "ALU / divergence", "bytes_per_path_bounce": 2 * 172 + 64 + 144,
+        "k_shade": {"bound": "hbm", "limiter": "latency: dependent gathers (queue entry, path state, triangle, "
+                                                "material, light record)", "bytes_per_path_bounce": 2 * 172 + 64 + 144,
                     "path_bounces": int(q), "launches": int(p["launches_shade"]), "ms": round(p["ms_shade"], 2),
                     "achieved": round(_rate(bytes_shade, p["ms_shade"]), 1),
                     "frac": round(_rate(bytes_shade, p["ms_shade"]) / HBM_PEAK_GBS, 4)},
@@ -418,10 +419,14 @@ def main():
                        "note": "closest hit + shade + shadow algorithmic bytes over the whole pass"},
     }
 
+    # measured HBM traffic of the closest-hit launch for the ray order this
+    # run used (the autotuner's choice; the library default bins this wave)
     traffic = None
+    ray_order = (shape or {}).get("ray_order", 1)
     if args.workload == "occluded_room" and spp == 16 and os.path.exists(TRAFFIC_PROFILE):
+        key = f"traffic_bytes_per_launch_ray_order_{1 if ray_order else 0}:"
         for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 traffic = int(line.split(":")[1])
     lights = {"occluded_room": "1 area light behind a 5 cm slit",
               "icospheres": "1 point + 1 sun + 4 area lights"}.get(args.workload, "")
