@@ -1578,12 +1578,13 @@ k_shade_guided(SceneDev S, WaveDev W, guide::GuideDev G, int qin, int32_t max_de
 // Joins the append segments of a shading launch into the dense next live
 // queue q[qout] (blockIdx.y 0) and shadow queue qs (blockIdx.y 1), and
 // writes their counts.  Workgroup (s, r) copies slice r of segment s to the
-// segment's offset, the sum of the counts before it.
-constexpr int kConcatSlices = 16;
+// segment's offset, the sum of the counts before it; gridDim.x is kShards
+// times the number of slices.
 __global__ __launch_bounds__(256) void k_queue_concat(WaveDev W, int qout) {
     __shared__ int s_off, s_cnt, s_total;
     const int w = blockIdx.y;
-    const int s = blockIdx.x / kConcatSlices, r = blockIdx.x % kConcatSlices;
+    const int slices = (int)gridDim.x / kShards;
+    const int s = blockIdx.x / slices, r = blockIdx.x % slices;
     if (threadIdx.x < 64) {
         // lane l: count of segment l (written by the atomics of the shading launch)
         const int c = __hip_atomic_load(W.cnt + kAppendBase + (w * kShards + (int)threadIdx.x) * kLineInts,
@@ -1599,12 +1600,17 @@ __global__ __launch_bounds__(256) void k_queue_concat(WaveDev W, int qout) {
     __syncthreads();
     const int32_t* src = (w ? W.aqs : W.aq) + (int64_t)s * W.seg_cap;
     int32_t* dst = (w ? W.qs : W.q[qout]) + s_off;
-    const int lo = (int)((int64_t)s_cnt * r / kConcatSlices), hi = (int)((int64_t)s_cnt * (r + 1) / kConcatSlices);
+    const int lo = (int)((int64_t)s_cnt * r / slices), hi = (int)((int64_t)s_cnt * (r + 1) / slices);
     for (int j = lo + (int)threadIdx.x; j < hi; j += 256) dst[j] = src[j];
     if (blockIdx.x == 0 && threadIdx.x == 0) W.cnt[w ? 2 : qout] = s_total;
 }
-inline void concat_queues(const WaveDev& W, int qout, hipStream_t st) {
-    k_queue_concat<<<dim3(kShards * kConcatSlices, 2), 256, 0, st>>>(W, qout);
+// n: the wave's paths (a bound on both queues); one slice per 32,768
+// entries a segment may hold, at most 16 (few workgroups for small waves:
+// on a lane they wait for the CUs other lanes' persistent launches hold)
+inline void concat_queues(const WaveDev& W, int qout, int64_t n, hipStream_t st) {
+    const int64_t per = (int64_t)kShards * 32768;
+    const int slices = (int)std::min<int64_t>(16, std::max<int64_t>(1, (n + per - 1) / per));
+    k_queue_concat<<<dim3(kShards * slices, 2), 256, 0, st>>>(W, qout);
 }
 static_assert(kShards == 64, "one shard per lane of a wavefront");
 
@@ -3305,7 +3311,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
                     launch_shade<LUCID_ALGO_NEE>(gs, st, SD, Wsh, G, q, max_depth);
                     break;
             }
-            concat_queues(W, 1 - q, st);
+            concat_queues(W, 1 - q, n, st);
             if (march) {
                 const unsigned gm = (unsigned)std::min<int64_t>(blocks(n, kBlock), (int64_t)ctx->num_cu * 16);
                 k_vol_march<<<gm, kBlock, 0, st>>>(SD, W, refill_lanes);
@@ -4962,7 +4968,7 @@ int fused_sessions_pass(lucid_ctx* ctx, lucid_session* const* ss, int32_t n, int
                 launch_shade_fused<LUCID_ALGO_NEE>(gs, st, SD, W, F, q);
             else
                 launch_shade_fused<LUCID_ALGO_SIMPLE>(gs, st, SD, W, F, q);
-            concat_queues(W, 1 - q, st);
+            concat_queues(W, 1 - q, np, st);
             kev_end(ctx, ks, st);
             ks = kev_begin(ctx, 2, st);
             k_trace_refill<1, true><<<g, kBlock, lds, st>>>(SD, W, 0, sd, refill, walk_min, leaf_tris);
