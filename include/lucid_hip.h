@@ -226,7 +226,7 @@ int lucid_synchronize(lucid_ctx* ctx);
    buffers live in a 256 MiB device array of the library that every GPU the
    library runs on holds until the process ends; the scene, the camera and
    the output are the caller's.  Same output as lucid_render_tile_device,
-   bit for bit.  Limits: tile_w * tile_h at most about 1.37M pixels (1.1M
+   bit for bit.  Limits: tile_w * tile_h at most about 1.31M pixels (1.06M
    for LUCID_ALGO_VOLUME_MIS), beyond which LUCID_ERR_INVALID says so; not
    for LUCID_ALGO_GUIDED_MIS or with diagnostics enabled.  All calls on one
    GPU share the array: calls that overlap on one GPU (other contexts,
@@ -289,7 +289,7 @@ void lucid_reset_cancel(lucid_ctx* ctx);
 int lucid_get_stats(const lucid_ctx* ctx, lucid_stats* out);
 
 /* Paths per wavefront batch (0 = automatic: up to 2^26 paths, bounded by a
-   quarter of free device memory at 172 B per path). */
+   quarter of free device memory at 188 B per path). */
 int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
 
 /* Launch shape of the traversal (closest and shadow) and shading kernels.

@@ -183,6 +183,42 @@ def test_traversal_launch_shapes_agree(case, tmp_path):
     ctx.close()
 
 
+def test_queue_sizes_and_shapes_keep_samples():
+    """Queues reach the persistent launches through 64 fetch ranges (chunks of
+    256 positions dealt round-robin) and leave shading through 64 append
+    segments (one per 128-position input chunk, modulo 64) joined by a
+    concatenation launch.  On the room at 160 x 120 and 8 spp every bounce's
+    queue has another length (Russian roulette), and waves of 1, 3 and 8
+    samples per pixel and three launch shapes move those lengths across
+    range and segment boundaries: every sample's radiance stays bit-identical,
+    and so do the ray counts."""
+    import ctypes as C
+    import lucid_capi
+
+    wl = workloads.WORKLOADS["occluded_room"]
+    ctx = lucid_capi.Context(0)
+    sh = ctx.scene(wl["make"]())
+    camd = lucid_capi.camera(wl["camera"])
+    W, H, spp = 160, 120, 8
+    prm = lucid_capi.params(0, 0, W, H, W, H, spp, 11, 12, "mis")
+    outs = []
+    for wave_b, groups, refill in ((0, 16, 48), (1, 16, 48), (3, 8, 60), (8, 24, 1)):
+        ctx.check(ctx.L.lucid_set_wave_paths(ctx.h, W * H * wave_b))
+        ctx.check(ctx.L.lucid_set_launch_shape(ctx.h, groups, 16))
+        ctx.check(ctx.L.lucid_set_refill_threshold(ctx.h, refill))
+        out = np.zeros(W * H * spp * 3, np.float32)
+        ctx.check(ctx.L.lucid_render_samples(ctx.h, sh, C.byref(camd), C.byref(prm),
+                                             out.ctypes.data_as(C.POINTER(C.c_float))))
+        st = ctx.stats()
+        outs.append((out, st["rays_closest"], st["rays_shadow"]))
+    for out, rc, rs in outs[1:]:
+        assert np.array_equal(out.view(np.int32), outs[0][0].view(np.int32))
+        assert (rc, rs) == outs[0][1:]
+    assert outs[0][1] > W * H * spp * 4  # paths live for several bounces
+    ctx.L.lucid_scene_release(sh)
+    ctx.close()
+
+
 def test_autotune_picks_a_shape_and_keeps_results(lucid):
     r = make_renderer(lucid, BOX, BOX_CAM, "mis")
     before = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
