@@ -421,11 +421,12 @@ def main():
     }
 
     # measured HBM traffic of the closest-hit launch for the ray order this
-    # run used (the autotuner's choice; the library default bins this wave)
+    # run used (the autotuner's choice; the library default bins this wave;
+    # order 3 bins the shadow rays only)
     traffic = None
     ray_order = (shape or {}).get("ray_order", 1)
     if args.workload == "occluded_room" and spp == 16 and os.path.exists(TRAFFIC_PROFILE):
-        key = f"traffic_bytes_per_launch_ray_order_{1 if ray_order else 0}:"
+        key = f"traffic_bytes_per_launch_ray_order_{1 if ray_order == 1 else 0}:"
         for line in open(TRAFFIC_PROFILE):
             if line.startswith(key):
                 traffic = int(line.split(":")[1])

@@ -3208,6 +3208,10 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
         W.sh_t = static_cast<float4*>(L.vallocs[2]);
     }
     const bool bin = ray_order == 1 || (ray_order == 2 && P * B >= ((int64_t)1 << 24));
+    // ray order 3: only the shadow rays are regrouped (the closest-hit
+    // traversal, which walks nodes until most lanes hold a leaf, gains little
+    // from it, the shadow rays about their sort's cost twice over)
+    const bool bin_bounce = bin, bin_shadow = bin || ray_order == 3;
     BinBox bin_box;
     for (int k = 0; k < 3; ++k) {
         bin_box.lo[k] = scene->bmin[k];
@@ -3234,7 +3238,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             const unsigned gsg = (unsigned)std::min<int64_t>(blocks(n, kBlock), cap_shade_guided);
             // the closest-hit pass reads the live queue through Wd
             WaveDev Wd = W;
-            if (bin && depth > 0) {
+            if (bin_bounce && depth > 0) {
                 const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinBounce>(W, BinSrc{W.q[q], W.cnt + q, W.ray_o, W.ray_d}, bin_box, n, ctx->num_cu, st);
                 kev_end(ctx, kb, st);
@@ -3319,7 +3323,7 @@ int run_waves(lucid_ctx* ctx, Lane& L, lucid_scene* scene, const lucid_camera* c
             kev_end(ctx, ks, st);
             // the shadow pass reads the NEE queue through Ws
             WaveDev Ws = W;
-            if (bin && depth > 0) {
+            if (bin_shadow && depth > 0) {
                 const int kb = kev_begin(ctx, 5, st);
                 bin_rays<kBinShadow>(W, BinSrc{W.qs, W.cnt + 2, W.sh_o, W.sh_d}, bin_box, n, ctx->num_cu, st);
                 kev_end(ctx, kb, st);
@@ -4083,9 +4087,9 @@ int lucid_autotune(lucid_ctx* ctx, lucid_scene* scene, const lucid_camera* cam, 
     int32_t ro = 0, tg = ctx->traverse_groups, rf = ctx->refill, sg = ctx->shade_groups;
     double t_best = 0.0;
     rc = measure(ro, tg, rf, sg, &t_best);
-    {
+    for (int32_t v : {1, 3}) {  // every ray binned, shadow rays binned
         double t = 0.0;
-        if (!rc && !(rc = measure(1, tg, rf, sg, &t)) && t < t_best) { t_best = t; ro = 1; }
+        if (!rc && !(rc = measure(v, tg, rf, sg, &t)) && t < t_best) { t_best = t; ro = v; }
     }
     {
         double t = 0.0;
@@ -4194,7 +4198,7 @@ int lucid_set_session_graphs(lucid_ctx* ctx, int32_t on) {
 }
 
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order) {
-    if (!ctx || order < 0 || order > 2) return LUCID_ERR_INVALID;
+    if (!ctx || order < 0 || order > 3) return LUCID_ERR_INVALID;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->ray_order = order;
     return LUCID_OK;

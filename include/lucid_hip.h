@@ -303,7 +303,7 @@ int lucid_set_wave_paths(lucid_ctx* ctx, int64_t paths);
 int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32_t shade_groups_per_cu);
 /* Autotuner (no counterpart in the reference): times launch shapes on
    renders of `params` (discarded; guided requests are tuned on plain MIS)
-   by coordinate descent -- the ray order (both values), traversal groups
+   by coordinate descent -- the ray order (0, 1 and 3), traversal groups
    per CU in {8, 12, 16, 24}, then
    the refill threshold in {32, 48, 60}, then the node-walk threshold in
    {0, 8, 16, 32} and the triangles per leaf phase in {1, 2, 4}, then
@@ -314,7 +314,7 @@ int lucid_set_launch_shape(lucid_ctx* ctx, int32_t traverse_groups_per_cu, int32
    enabled. */
 typedef struct lucid_launch_shape {
     int32_t traverse_groups, refill, shade_groups;
-    int32_t ray_order;  /* lucid_set_ray_order: 0 or 1 */
+    int32_t ray_order;  /* lucid_set_ray_order: 0, 1 or 3 */
     int32_t shade_order;  /* lucid_set_shade_order: 0 or 1 */
     double ms;          /* device time of the chosen shape's render */
     int64_t wave_paths; /* lucid_set_wave_paths: 0 automatic, or paths per batch */
@@ -358,7 +358,8 @@ int lucid_set_session_fusion(lucid_ctx* ctx, int32_t enabled);
    cell (8x8x8 over the scene box) before the closest-hit pass, and the
    shadow rays by origin cell (16x16x16) before the shadow pass; 0 keeps the
    order in which shading queued them; 2 (default) regroups in waves of at
-   least 2^24 paths only.  Results do not depend on it. */
+   least 2^24 paths only; 3 regroups the shadow rays only.  Results do not
+   depend on it. */
 int lucid_set_ray_order(lucid_ctx* ctx, int32_t order);
 
 /* Persistent traversal (no counterpart in the reference): a wavefront walks

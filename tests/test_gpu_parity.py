@@ -164,7 +164,7 @@ def test_traversal_launch_shapes_agree(case, tmp_path):
     prm = lucid_capi.params(0, 0, W, H, W, H, spp, 7, 8, algo)
     outs = []
     for shape, refill, order, shade in ((0, 48, 0, 0), (16, 48, 0, 0), (16, 1, 1, 1), (3, 64, 1, 0), (0, 48, 1, 1),
-                                        (16, 48, 2, 1)):
+                                        (16, 48, 2, 1), (16, 48, 3, 0)):
         ctx.check(ctx.L.lucid_set_launch_shape(ctx.h, shape, 16))
         ctx.check(ctx.L.lucid_set_refill_threshold(ctx.h, refill))
         ctx.check(ctx.L.lucid_set_ray_order(ctx.h, order))
@@ -224,7 +224,7 @@ def test_autotune_picks_a_shape_and_keeps_results(lucid):
     before = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
     shape = r.autotune(64, 48, 4, 8)
     assert shape["traverse_groups"] in (8, 12, 16, 24) and shape["refill"] in (32, 48, 60)
-    assert shape["shade_groups"] in (4, 8, 16) and shape["ray_order"] in (0, 1) and shape["ms"] > 0
+    assert shape["shade_groups"] in (4, 8, 16) and shape["ray_order"] in (0, 1, 3) and shape["ms"] > 0
     assert shape["shade_order"] in (0, 1) and shape["wave_paths"] == 0  # too few paths to split
     after = np.asarray(r.render_tile(0, 0, 64, 48, 64, 48, 8, 5, 8))
     assert np.array_equal(before, after)
