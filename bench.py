@@ -86,23 +86,53 @@ def _host_info():
     return info
 
 
-def _cpu_rate(o, W, H, depth, threads, seconds_target):
+def _cpu_rate(o, W, H, depth, threads, seconds_target, algo="mis"):
     """Mrays/s of the oracle on a strip of rows of the frame sized to about
     `seconds_target` seconds; returns (Mrays/s, sample text)."""
     rows, samples = 8, 2
     t0 = time.perf_counter()
-    o.render_tile("mis", 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
+    o.render_tile(algo, 0, H // 2, W, rows, W, H, samples=samples, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0
     total = seconds_target / max(dt / (rows * samples), 1e-9)  # row-samples that fit the target
     rows2 = int(max(1, min(H, total / samples)))
     samples = int(max(samples, total / rows2))  # a full frame that is quick gets more samples
     y0 = max(0, H // 2 - rows2 // 2)
     t0 = time.perf_counter()
-    _, ex = o.render_tile("mis", 0, y0, W, rows2, W, H, samples=samples, max_depth=depth, threads=threads)
+    _, ex = o.render_tile(algo, 0, y0, W, rows2, W, H, samples=samples, max_depth=depth, threads=threads)
     dt = time.perf_counter() - t0
     rays = int(ex["counts"].sum())
     return rays / dt / 1e6, (f"{W}x{rows2} rows {y0}..{y0 + rows2 - 1} of the {W}x{H} frame, {samples} spp, "
-                             f"depth {depth}, mis; {rays} queries in {dt:.1f} s")
+                             f"depth {depth}, {algo}; {rays} queries in {dt:.1f} s")
+
+
+def cpu_configs(threads, seconds_target=5.0):
+    """The oracle (-O3 -ffast-math) on bounded strips of the other configs at
+    reduced spp: the 1M-triangle icospheres (1024 x 1024, depth 12, mis) and
+    the largest view of the 4M-triangle smoke scene (256^3 grid, depth 8,
+    volume_mis).  Per-sample rates: the GPU figures of the same configs are
+    whole-frame rates of the same algorithm."""
+    import tempfile
+
+    import oracle
+    import workloads
+
+    out = {}
+    ico = workloads.WORKLOADS["icospheres"]
+    o = oracle.OracleScene(ico["make"](), fast=True)
+    c = ico["camera"]
+    o.set_camera(c["pos"], c["dir"], c["up"], c["fov"])
+    v, smp = _cpu_rate(o, ico["width"], ico["height"], ico["max_depth"], threads, seconds_target)
+    out["icospheres_1m"] = {"value": round(v, 3), "unit": "Mrays/s", "sample": smp}
+    del o
+    with tempfile.TemporaryDirectory() as td:
+        scene = workloads.volume_scene(os.path.join(td, "smoke256.bin"))
+        cam, w, h = max(workloads.volume_sessions(), key=lambda t: t[1] * t[2])
+        o = oracle.OracleScene(scene, fast=True)
+        o.set_camera(cam["pos"], cam["dir"], cam["up"], cam["fov"])
+        v, smp = _cpu_rate(o, w, h, 8, threads, seconds_target, "volume_mis")
+        out["volume_4m_256grid"] = {"value": round(v, 3), "unit": "Mrays/s", "sample": smp}
+        del o
+    return out
 
 
 def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
@@ -135,6 +165,7 @@ def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
             "ieee_build": {"value": round(ieee, 3), "unit": "Mrays/s", "sample": sample_i + " (oracle -O2, IEEE)"},
             "closed_box_full": {"seconds": round(t_box, 3), "mrays_s": round(box_rays / t_box / 1e6, 3),
                                 "queries": box_rays, "config": "34 triangles, 256x256, 16 spp, depth 8, mis, full"},
+            "other_configs": cpu_configs(threads),
             "host": _host_info()}
 
 
