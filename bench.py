@@ -169,6 +169,73 @@ def cpu_baseline(scene, cam, W, H, depth, seconds_target=15.0):
             "host": _host_info()}
 
 
+def heldout_leg(ctx, sc, args, spp, depth, W=480, H=270, batches=16, batch_spp=8192):
+    """Time to relMSE 0.01 on a view of the room the guiding defaults were not
+    chosen on.  The reference r is plain MIS at batches x batch_spp samples per
+    pixel from seeds neither tracer uses; its own expected error (the
+    variance of its batch means) is subtracted from each relMSE, as for the
+    oracle fixture.  Both tracers then render 16-spp passes until their
+    running estimate crosses the target (every pixel counts)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    import lucid_capi
+    import workloads
+
+    L = ctx.L
+    camd = lucid_capi.camera(workloads.OCCLUDED_ROOM_HELDOUT_CAMERA)
+    out = torch.empty(W * H * 4, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def render(algorithm, n_spp, offset):
+        prm = lucid_capi.params(0, 0, W, H, W, H, n_spp, offset, depth, algorithm)
+        ctx.check(L.lucid_render_tile_device(ctx.h, sc, C.byref(camd), C.byref(prm), C.c_void_p(out.data_ptr()),
+                                             C.c_void_p(stream)))
+        ctx.check(L.lucid_synchronize(ctx.h))
+        return out.view(H, W, 4)[..., :3].double()  # sums of n_spp samples
+
+    t0 = time.perf_counter()
+    means = torch.stack([render("mis", batch_spp, 400_000_000 + k * batch_spp) / batch_spp for k in range(batches)])
+    R = means.mean(0)
+    ref_err = float((means.var(0, unbiased=True) / batches / (R * R + 0.01)).mean())
+    t_ref = time.perf_counter() - t0
+    log(f"held-out reference: {batches * batch_spp} spp in {t_ref:.1f} s, own relMSE {ref_err:.5f}")
+
+    def crossing(algorithm):
+        acc = torch.zeros_like(R)
+        t_run, n, i = 0.0, 0, 0
+        while t_run < args.max_seconds:
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            x = render(algorithm, spp, args.seed * 10_000_000 + i * spp)
+            torch.cuda.synchronize()
+            t_run += time.perf_counter() - ts
+            acc += x
+            n += spp
+            i += 1
+            e = float(((acc / n - R) ** 2 / (R * R + 0.01)).mean()) - ref_err
+            if e <= TARGET:
+                return t_run, n, "measured"
+        return t_run, n, "not reached"
+
+    tp, np_, mp = crossing("mis")
+    gc = lucid_capi.GuidingConfig(args.alpha, args.train_steps, args.batch, args.lr, args.record_stride,
+                                  7 + args.seed, args.guide_depth, args.learn_selection)
+    ctx.check(L.lucid_set_guiding(ctx.h, C.byref(gc)))
+    ctx.check(L.lucid_guiding_reset(ctx.h))
+    tg, ng, mg = crossing("guided")
+    log(f"held-out view: plain {tp:.1f} s ({np_} spp), guided {tg:.1f} s ({ng} spp)")
+    return {"metric": METRIC, "mode": "heldout", "view": workloads.OCCLUDED_ROOM_HELDOUT_CAMERA,
+            "width": W, "height": H, "seed": args.seed, "target": TARGET, "pass_spp": spp,
+            "reference": {"spp": batches * batch_spp, "kind": "plain MIS on the GPU, held-out seeds",
+                          "self_relmse": round(ref_err, 5), "seconds": round(t_ref, 1)},
+            "plain_s": round(tp, 2), "plain_spp": np_, "plain_method": mp,
+            "guided_s": round(tg, 2), "guided_spp": ng, "guided_method": mg,
+            "speedup": round(tp / tg, 3) if mp == mg == "measured" else None}
+
+
 def time_to_target(curve):
     """curve: [(seconds, relMSE)].  Measured crossing (linear interpolation
     between checkpoints) or extrapolation from relMSE * t over the last half."""
@@ -325,6 +392,9 @@ def main():
     ap.add_argument("--max-seconds", type=float, default=540.0, help="--crossing: cap per tracer")
     ap.add_argument("--guided-only", action="store_true",
                     help="time-to-relMSE leg: the guided tracer only (parameter sweeps)")
+    ap.add_argument("--heldout", action="store_true",
+                    help="time to relMSE 0.01 on a second view of the room (480x270) against a plain MIS reference "
+                         "from held-out seeds, both crossings measured")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -557,6 +627,12 @@ def main():
                             "lr": args.lr, "record_stride": args.record_stride},
                 "plain_curve": [(round(t, 2), round(e, 5)) for t, e in plain_curve[::8]],
                 "guided_curve": [(round(t, 2), round(e, 5)) for t, e in guided_curve[::8]]}), flush=True)
+            L.lucid_scene_release(sc)
+            ctx.close()
+            return
+
+        if args.heldout:
+            print(json.dumps(heldout_leg(ctx, sc, args, spp, depth)), flush=True)
             L.lucid_scene_release(sc)
             ctx.close()
             return

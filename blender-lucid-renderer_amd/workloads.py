@@ -444,6 +444,10 @@ def occluded_room(seed: int = 5, grid: int = 150, spheres: int = 20) -> dict:
 
 
 OCCLUDED_ROOM_CAMERA = {"pos": (0.0, 1.5, 1.9), "dir": (0.0, 0.08, -1.0), "up": (0.0, 1.0, 0.0), "fov": 65.0}
+# a second view of the room (from the front-left corner towards the far right
+# of the front part), not used to choose the guiding defaults
+OCCLUDED_ROOM_HELDOUT_CAMERA = {"pos": (-3.4, 2.3, 1.6), "dir": (0.75, -0.3, -0.6), "up": (0.0, 1.0, 0.0),
+                                "fov": 60.0}
 
 
 def relmse_pixels(width: int = 1920, height: int = 1080, nx: int = 128, ny: int = 128):
