@@ -243,3 +243,15 @@ def test_autotune_wave_size_keeps_results(lucid):
     assert np.array_equal(before, after)
     r.set_wave_paths(W * H * 4)  # four waves of 4 samples
     assert np.array_equal(np.asarray(r.render_tile(0, 0, W, H, W, H, 16, 3, 8)), before)
+
+
+def test_ray_order_arguments():
+    """Ray orders 0-3 are accepted (3: shadow rays only), others refused."""
+    import lucid_capi
+
+    ctx = lucid_capi.Context(0)
+    for order in (0, 1, 2, 3):
+        assert ctx.L.lucid_set_ray_order(ctx.h, order) == 0
+    for order in (-1, 4):
+        assert ctx.L.lucid_set_ray_order(ctx.h, order) == 1
+    ctx.close()
